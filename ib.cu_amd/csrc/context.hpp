@@ -1,0 +1,80 @@
+// Library context: device, stream, scratch workspaces and error reporting.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstdio>
+#include <cstdarg>
+#include "ibm.h"
+
+struct ibm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string error = "";
+
+    // Named scratch buffers, grown on demand and kept until ibm_destroy.
+    struct buffer { void* ptr = nullptr; size_t bytes = 0; };
+    std::vector<buffer> scratch;
+
+    // Pinned host word used to read device status flags back.
+    int* host_flag = nullptr;
+
+    // Optional per-kernel timing: hipEvent pairs recorded around named
+    // launches on ctx->stream while profiling is enabled.
+    struct prof_record { const char* name; hipEvent_t start, stop; };
+    bool profiling = false;
+    std::vector<prof_record> prof;
+    size_t prof_used = 0;
+};
+
+namespace ibm {
+
+enum scratch_slot {
+    SLOT_BIN_COUNT = 0,
+    SLOT_BIN_START,
+    SLOT_KEY,
+    SLOT_RANK,
+    SLOT_TMP,
+    SLOT_PERM,
+    SLOT_CODE,
+    SLOT_SORTED,
+    SLOT_SCAN_TMP,
+    SLOT_FLAGS,
+    SLOT_MG_BASE,       // multigrid hierarchies own their memory; slots above
+    SLOT_COUNT = 64
+};
+
+ibm_status fail(ibm_ctx* ctx, ibm_status code, const char* fmt, ...);
+ibm_status hip_fail(ibm_ctx* ctx, hipError_t e, const char* what);
+// Returns a device buffer of at least `bytes` for `slot`, reallocating when
+// the request grows.  Contents are not preserved across growth.
+ibm_status scratch(ibm_ctx* ctx, int slot, size_t bytes, void** out);
+
+// Brackets a kernel launch with timing events when profiling is on.
+struct prof_scope {
+    ibm_ctx* ctx;
+    int index = -1;
+    prof_scope(ibm_ctx* c, const char* name);
+    ~prof_scope();
+};
+
+} // namespace ibm
+
+#define IBM_HIP(ctx, call)                                                  \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) return ibm::hip_fail((ctx), e_, #call);       \
+    } while (0)
+
+#define IBM_TRY(call)                                                       \
+    do {                                                                    \
+        ibm_status s_ = (call);                                             \
+        if (s_ != IBM_OK) return s_;                                        \
+    } while (0)
+
+#define IBM_LAUNCH_CHECK(ctx)                                               \
+    do {                                                                    \
+        hipError_t e_ = hipGetLastError();                                  \
+        if (e_ != hipSuccess) return ibm::hip_fail((ctx), e_, "kernel launch"); \
+    } while (0)

@@ -1,0 +1,918 @@
+// Spreading and interpolation between Lagrangian markers and the MAC grid.
+//
+// Reference algorithm (ib/novel.h:58-168, ib/interpolate.h:49-76): per
+// component, sort markers by cell, segmented-sum 8 stencil values per sweep
+// into 8 plane buffers, combine the planes; interpolation is a per-marker
+// 64-point gather.  The values computed are the same:
+//     spread:  fe_c(g) = h^-3 sum_m f_c(m) prod_j phi(g_j - X_j(m)/h)
+//     interp:  U_c(m)  = sum_g u_c(g) prod_j phi(g_j - X_j(m)/h)
+// with the reference's index conventions (geometry.hpp).
+//
+// MI355X design:
+//   * markers are counting-sorted once into grid bins (shared by the three
+//     components), canonically ordered inside a bin, so every later pass is
+//     independent of scheduling;
+//   * spreading is owner-computes: one workgroup per 8x8x8 brick of grid
+//     points gathers the markers of the bins whose stencils reach the brick,
+//     evaluates their separable weights once per chunk into LDS, and one
+//     wavefront per component adds each marker's 4x4x4 = 64 stencil values
+//     with one lane per stencil point.  The brick is written once with plain
+//     stores: no zero fill, no global atomics, bitwise reproducible.
+//   * interpolation is one lane per marker with the separable sum
+//     sum_z wz sum_y wy sum_x wx u.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <climits>
+#include "context.hpp"
+#include "deltas.hpp"
+
+namespace ibm {
+namespace {
+
+constexpr int MAX_SETS = 16;
+constexpr int CHUNK = 64;
+constexpr int SPREAD_THREADS = 256;
+
+struct kaxis {
+    double length, resolution, shift;
+    double s_over_r;   // shift / resolution, rounded once as fd::discretization::units does per call
+    int cells, points, solid, pad;
+};
+
+struct kgrids {
+    kaxis a[3][3];     // [component][dimension]
+    double res3[3];    // product of the resolutions of each component grid
+    int points[3][3];
+    int cand[2][3];    // per dimension, the components whose axes are the (at most two) distinct ones
+    int sel[3][3];     // sel[c][j]: which candidate component c uses along dimension j
+};
+
+struct marker_sets {
+    int nsets;
+    int total;
+    int offset[MAX_SETS + 1];
+    const double* x[MAX_SETS];
+    const double* f[MAX_SETS];
+    double* out[MAX_SETS];
+};
+
+struct bin_desc {
+    int nb[3];
+    int qmin[3];
+    int fc[3];
+    int fs[3];
+    int lo[3], hi[3];
+    int nbins;
+};
+
+// Statistics gathered while binning; they bound the spread accumulators.
+// Maxima are spread over STAT_SLOTS words (by block index) so the atomics
+// do not all hit one address; readers reduce the slots.
+constexpr int STAT_SLOTS = 64;
+struct bin_stats {
+    unsigned long long max_abs_f[STAT_SLOTS];   // bit patterns of max |f| (non-negative doubles order as integers)
+    unsigned long long max_count[STAT_SLOTS];   // largest number of markers in one bin
+};
+
+__device__ inline int set_of(const marker_sets& s, int m)
+{
+    int k = 0;
+    while (k + 1 < s.nsets && m >= s.offset[k + 1]) ++k;
+    return k;
+}
+
+// axis_stencil() with the per-axis quotient shift/resolution precomputed and
+// the periodic wrap (i + cells) % cells written for i in [-cells, 2 cells).
+__device__ inline stencil_axis kstencil(const kaxis& a, double x)
+{
+#pragma clang fp contract(off)
+    double y = x - a.s_over_r;
+    if (!a.solid) {
+        // For y in [-L, 2L), floor(y / L) is decided by comparisons alone
+        // (correctly rounded division is monotone and y / L < 1 whenever
+        // y < L), so modulo(y, L) = y - L * floor(y / L) is reproduced
+        // exactly without the division.
+        double L = a.length;
+        if (y >= 0 && y < L) {
+        } else if (y < 0 && y >= -L) {
+            y = y + L;
+        } else if (y >= L && y < 2 * L) {
+            y = y - L;
+        } else {
+            y = ref_modulo(y, L);
+        }
+    }
+    double diff = (y * a.resolution + a.shift) - a.shift;
+    double fl = ref_floor(diff);
+    int i = (int) fl;
+    int j = i;
+    if (!a.solid) j = i < 0 ? i + a.cells : (i >= a.cells ? i - a.cells : i);
+    int w = a.points + a.solid;
+    int k = j + a.solid;
+    k = k < 0 ? 0 : (k >= w ? w - 1 : k);
+    stencil_axis s;
+    s.base = k - a.solid;
+    s.d = fl - diff;
+    return s;
+}
+
+// Maximum over the 64 lanes of a wavefront (all lanes must be active).
+__device__ inline unsigned long long wave_max(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// ----------------------------------------------------------------------
+// Binning: key = bin of the marker's face-aligned cell.  For dimension j the
+// face component fc[j] is aligned on cell faces there; its (clamped) base
+// cell plus the integer part of its shift is q_j.  Every component's base
+// cell then lies in q_j + [lo_j, hi_j], which bounds the bins a brick reads.
+// ----------------------------------------------------------------------
+__global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int with_forces,
+                                 int* __restrict__ key, int* __restrict__ rank,
+                                 int* __restrict__ code, int* __restrict__ count,
+                                 bin_stats* __restrict__ stats)
+{
+    // No early return: every lane takes part in the wavefront reduction.
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long fbits = 0;
+    if (m < sets.total) {
+        int k = set_of(sets, m);
+        int l = m - sets.offset[k];
+        int n = sets.offset[k + 1] - sets.offset[k];
+        const double* x = sets.x[k];
+        double p[3] = {x[l], x[n + l], x[2 * n + l]};
+        int q[3];
+        int bin = 0;
+        int stride = 1;
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = kstencil(g.a[bd.fc[j]][j], p[j]);
+            q[j] = st.base + bd.fs[j];
+            int qb = q[j] - bd.qmin[j];
+            qb = qb < 0 ? 0 : (qb >= bd.nb[j] ? bd.nb[j] - 1 : qb);
+            bin += qb * stride;
+            stride *= bd.nb[j];
+        }
+        int c9 = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) {
+                stencil_axis st = kstencil(g.a[c][j], p[j]);
+                c9 |= ((st.base - q[j]) & 1) << (3 * c + j);
+            }
+        key[m] = bin;
+        code[m] = c9;
+        rank[m] = atomicAdd(&count[bin], 1);
+        if (with_forces) {
+            const double* f = sets.f[k];
+            double a = fmax(fabs(f[l]), fmax(fabs(f[n + l]), fabs(f[2 * n + l])));
+            fbits = (unsigned long long) __double_as_longlong(a);
+        }
+    }
+    if (with_forces) {
+        fbits = wave_max(fbits);
+        if ((threadIdx.x & 63) == 0 && fbits) atomicMax(&stats->max_abs_f[blockIdx.x % STAT_SLOTS], fbits);
+    }
+}
+
+__global__ void bin_scatter_kernel(int total, const int* __restrict__ key, const int* __restrict__ rank,
+                                   const int* __restrict__ start, int* __restrict__ tmp)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= total) return;
+    tmp[start[key[m]] + rank[m]] = m;
+}
+
+// Canonical order inside a bin: by (code, marker id).  Writes the permutation
+// and sorted copies of positions and forces (SoA, 6 * total doubles).
+__global__ void bin_canon_kernel(marker_sets sets, int with_forces,
+                                 const int* __restrict__ key, const int* __restrict__ code,
+                                 const int* __restrict__ start, const int* __restrict__ tmp,
+                                 int* __restrict__ perm, double* __restrict__ sorted,
+                                 bin_stats* __restrict__ stats)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = sets.total;
+    unsigned long long cnt = 0;
+    if (p < total) {
+        int m = tmp[p];
+        int b = key[m];
+        int s = start[b], e = start[b + 1];
+        long long mine = ((long long) code[m] << 32) | (unsigned) m;
+        int r = 0;
+        for (int i = s; i < e; ++i) {
+            int m2 = tmp[i];
+            long long other = ((long long) code[m2] << 32) | (unsigned) m2;
+            r += other < mine;
+        }
+        int dst = s + r;
+        perm[dst] = m;
+        cnt = (unsigned long long) (e - s);
+        int k = set_of(sets, m);
+        int l = m - sets.offset[k];
+        int n = sets.offset[k + 1] - sets.offset[k];
+        for (int j = 0; j < 3; ++j) sorted[(size_t) j * total + dst] = sets.x[k][(size_t) j * n + l];
+        if (with_forces)
+            for (int j = 0; j < 3; ++j) sorted[(size_t) (3 + j) * total + dst] = sets.f[k][(size_t) j * n + l];
+    }
+    cnt = wave_max(cnt);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicMax(&stats->max_count[blockIdx.x % STAT_SLOTS], cnt);
+}
+
+// ----------------------------------------------------------------------
+// Spreading
+//
+// One workgroup per brick of BX x BY x BZ grid points (all three
+// components).  The markers of the bins whose stencils reach the brick are
+// walked in canonical order in chunks of 64; every wavefront owns a stream
+// of chunks.  For a chunk, lanes are markers: each computes its separable
+// weights once and, if its stencil meets the brick, stores them in the
+// wavefront's LDS slice (compacted with a ballot).  Then lanes are the 64
+// stencil points (sx, sy, sz): the wavefront walks the compacted markers,
+// sums the contributions of consecutive markers of one cell in a register
+// and adds the cell's 64 partial sums to the brick tile.
+//
+// The tile holds 64-bit fixed-point integers (two's complement): integer
+// addition is associative, so the four wavefronts may add in any order and
+// the result is still bitwise reproducible.  The quantum q is a power of
+// two chosen so that |sum| < 2^62 q for every point:
+//     sum <= 64 cells * max_count * max|f| * h^-3 * max phi^3  (phi^3 <= 0.3),
+// and each added cell sum is rounded to the nearest multiple of q, so a
+// point's error is at most 64 * q / 2 per flush group, about 2^-55 of that
+// bound.
+// ----------------------------------------------------------------------
+struct region_dim {
+    int first;   // first bin coordinate (may be negative before wrapping)
+    int count;   // number of bins
+    int wrap;    // periodic wrap modulus (0 = none)
+};
+
+template <int B>
+__device__ inline region_dim brick_region(const bin_desc& bd, const kaxis& fa, int j, int P0)
+{
+    region_dim r;
+    int nb = bd.nb[j];
+    if (!fa.solid) {
+        int lo = P0 - 2 - bd.hi[j];
+        int hi = P0 + B - bd.lo[j];
+        if (hi - lo + 1 >= nb) { r.first = 0; r.count = nb; r.wrap = 0; }
+        else { r.first = lo - bd.qmin[j]; r.count = hi - lo + 1; r.wrap = nb; }
+    } else {
+        int lo = P0 - 2 - bd.hi[j] - bd.qmin[j];
+        int hi = P0 + B - bd.lo[j] - bd.qmin[j];
+        lo = lo < 0 ? 0 : lo;
+        hi = hi >= nb ? nb - 1 : hi;
+        r.first = lo; r.count = hi >= lo ? hi - lo + 1 : 0; r.wrap = 0;
+    }
+    return r;
+}
+
+// Offset of stencil point (base + o - 1) inside the brick along one axis, or -1.
+template <int B>
+__device__ inline int brick_offset(const kaxis& a, int base, int o, int P0)
+{
+    int v = base + o - 1 - P0;
+    if (!a.solid) {
+        if (v < 0) v += a.cells;
+        else if (v >= a.cells) v -= a.cells;
+    }
+    int lim = a.points - P0;
+    lim = lim < B ? lim : B;
+    return (v >= 0 && v < lim) ? v : -1;
+}
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int BX = 8, BY = 8, BZ = 8;
+constexpr int BRICK_PTS = BX * BY * BZ;
+constexpr int WAVES = SPREAD_THREADS / 64;
+constexpr int MAX_ROWS = (BY + 4) * (BZ + 4);
+constexpr int MAX_SEG2 = 2 * MAX_ROWS;
+constexpr int WROW = CHUNK + 1;      // padded row: rows of one wavefront's weights start in different banks
+// FAST: every periodic axis has at least BX + 8 cells, so a stencil that
+// reaches the brick has a unique brick-relative origin in [-3, B).
+constexpr int MIN_FAST_CELLS = BX + 8;
+// Bias that keeps the flat brick index of a stencil origin in [-3, B)^3 positive (12 bits).
+constexpr int ORG_BIAS = 3 + BX * (3 + BY * 3);
+static_assert(ORG_BIAS + BRICK_PTS < 4096, "flat origin must fit in 12 bits");
+
+// Whether the stencil of base cell `base` meets the brick along one axis;
+// with FAST, also its brick-relative origin (offset of stencil point 0).
+template <bool FAST, int B>
+__device__ inline bool axis_hit(const kaxis& a, int base, int P0, int& origin)
+{
+    int lim = a.points - P0;
+    lim = lim < B ? lim : B;
+    if (FAST) {
+        int v = base - 1 - P0;
+        if (!a.solid) {
+            if (v < -4) v += a.cells;
+            else if (v >= a.cells - 4) v -= a.cells;
+        }
+        origin = v;
+        return v + 3 >= 0 && v < lim;
+    } else {
+        origin = 0;
+        bool m = false;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) m = m || brick_offset<B>(a, base, o, P0) >= 0;
+        return m;
+    }
+}
+
+template <bool MAC, bool FAST>
+__global__ void __launch_bounds__(SPREAD_THREADS)
+spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
+                    const int* __restrict__ start, const double* __restrict__ sorted,
+                    const bin_stats* __restrict__ stats,
+                    double* __restrict__ fe0, double* __restrict__ fe1, double* __restrict__ fe2,
+                    int bricks_x, int bricks_y, int nbricks, int xcd_chunk)
+{
+    __shared__ unsigned long long acc[3][BRICK_PTS];
+    __shared__ double W[WAVES][20][WROW];     // rows 0-3: wx[sx]; rows 4-19: wz[a] * wy[b] * f at 4 + 4a + b
+    __shared__ int org[WAVES][CHUNK];
+    __shared__ int seg_begin[MAX_SEG2];
+    __shared__ int seg_pref[MAX_SEG2 + 1];
+    __shared__ double bound_s;
+
+    const int tid = threadIdx.x;
+    int bid = blockIdx.x;
+    if (xcd_chunk > 0) bid = (bid & 7) * xcd_chunk + (bid >> 3);
+    if (bid >= nbricks) return;
+    const int P0x = (bid % bricks_x) * BX;
+    const int P0y = ((bid / bricks_x) % bricks_y) * BY;
+    const int P0z = (bid / (bricks_x * bricks_y)) * BZ;
+
+    for (int i = tid; i < 3 * BRICK_PTS; i += SPREAD_THREADS) (&acc[0][0])[i] = 0ull;
+
+    region_dim rx = brick_region<BX>(bd, g.a[bd.fc[0]][0], 0, P0x);
+    region_dim ry = brick_region<BY>(bd, g.a[bd.fc[1]][1], 1, P0y);
+    region_dim rz = brick_region<BZ>(bd, g.a[bd.fc[2]][2], 2, P0z);
+    const int rows = ry.count * rz.count;          // <= MAX_ROWS (checked on the host)
+    const int nseg = 2 * rows;
+    for (int r = tid; r < rows; r += SPREAD_THREADS) {
+        int yy = ry.first + r % ry.count;
+        int zz = rz.first + r / ry.count;
+        if (ry.wrap) yy = (yy % ry.wrap + ry.wrap) % ry.wrap;
+        if (rz.wrap) zz = (zz % rz.wrap + rz.wrap) % rz.wrap;
+        int row = (zz * bd.nb[1] + yy) * bd.nb[0];
+        int x0 = rx.first;
+        if (rx.wrap) x0 = (x0 % rx.wrap + rx.wrap) % rx.wrap;
+        int c0 = rx.count, c1 = 0;
+        if (rx.wrap && x0 + rx.count > bd.nb[0]) { c0 = bd.nb[0] - x0; c1 = rx.count - c0; }
+        int s0 = start[row + x0], e0 = start[row + x0 + c0];
+        int s1 = start[row], e1 = c1 ? start[row + c1] : s1;
+        seg_begin[2 * r] = s0;     seg_pref[2 * r + 1] = e0 - s0;
+        seg_begin[2 * r + 1] = s1; seg_pref[2 * r + 2] = e1 - s1;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        constexpr int PER = (MAX_SEG2 + 63) / 64;
+        int vals[PER];
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int idx = tid * PER + i;
+            vals[i] = idx < nseg ? seg_pref[idx + 1] : 0;
+            sum += vals[i];
+        }
+        int incl = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += y;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int idx = tid * PER + i;
+            run += vals[i];
+            if (idx < nseg) seg_pref[idx + 1] = run;
+        }
+        if (tid == 0) seg_pref[0] = 0;
+        unsigned long long fb = wave_max(stats->max_abs_f[tid]);
+        unsigned long long cm = wave_max(stats->max_count[tid]);
+        if (tid == 0) {
+            double res3 = fmax(g.res3[0], fmax(g.res3[1], g.res3[2]));
+            bound_s = 64.0 * (double) cm * __longlong_as_double((long long) fb) * res3 * 0.3;
+        }
+    }
+    __syncthreads();
+    const int T = seg_pref[nseg];
+    int e2 = 0;
+    if (bound_s > 0) frexp(bound_s, &e2);
+    const double inv_q = ldexp(1.0, 62 - e2);
+
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int sx = lane & 3, sy = (lane >> 2) & 3, sz = lane >> 4;
+    const unsigned long long lanemask_lt = (1ull << lane) - 1;
+
+    // Sorted position of region marker idx (binary search over the segments).
+    auto locate = [&](int idx) {
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (seg_pref[mid] <= idx) lo = mid; else hi = mid;
+        }
+        return seg_begin[lo] + (idx - seg_pref[lo]);
+    };
+    // Positions and forces of the next chunk are loaded while the current
+    // one is processed.
+    double np_[3] = {0, 0, 0}, nf_[3] = {0, 0, 0};
+    {
+        int idx = wave * CHUNK + lane;
+        if (idx < T) {
+            int pp = locate(idx);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                np_[j] = sorted[(size_t) j * total + pp];
+                nf_[j] = sorted[(size_t) (3 + j) * total + pp];
+            }
+        }
+    }
+    for (int cb = wave * CHUNK; cb < T; cb += WAVES * CHUNK) {
+        // ---- lanes are markers ----
+        int idx = cb + lane;
+        bool valid = idx < T;
+        double p[3] = {np_[0], np_[1], np_[2]}, fv[3] = {nf_[0], nf_[1], nf_[2]};
+        {
+            int nidx = idx + WAVES * CHUNK;
+            if (nidx < T) {
+                int pp = locate(nidx);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    np_[j] = sorted[(size_t) j * total + pp];
+                    nf_[j] = sorted[(size_t) (3 + j) * total + pp];
+                }
+            }
+        }
+        double w0[3][4], w1[3][4];
+        int b0[3], b1[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = kstencil(g.a[g.cand[0][j]][j], p[j]);
+            b0[j] = st.base;
+            delta4(delta, st.d, w0[j]);
+            if (MAC || g.cand[1][j] != g.cand[0][j]) {
+                st = kstencil(g.a[g.cand[1][j]][j], p[j]);
+                b1[j] = st.base;
+                delta4(delta, st.d, w1[j]);
+            } else {
+                b1[j] = b0[j];
+#pragma unroll
+                for (int o = 0; o < 4; ++o) w1[j][o] = w0[j][o];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int bb[3];
+            double w[3][4];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                bool one = MAC ? (c == j) : (g.sel[c][j] != 0);
+                bb[j] = one ? b1[j] : b0[j];
+#pragma unroll
+                for (int o = 0; o < 4; ++o) w[j][o] = one ? w1[j][o] : w0[j][o];
+            }
+            // Does the stencil meet the brick?  Brick-relative origin of stencil point 0.
+            int ow0, ow1, ow2;
+            bool hit = valid && axis_hit<FAST, BX>(g.a[c][0], bb[0], P0x, ow0)
+                             && axis_hit<FAST, BY>(g.a[c][1], bb[1], P0y, ow1)
+                             && axis_hit<FAST, BZ>(g.a[c][2], bb[2], P0z, ow2);
+            const unsigned long long hits = __ballot(hit);
+            const int nh = __popcll(hits);
+            unsigned long long below = hits & lanemask_lt;
+            int prev = below ? 63 - __clzll((long long) below) : lane;
+            int pb0 = __shfl(bb[0], prev, 64), pb1 = __shfl(bb[1], prev, 64), pb2 = __shfl(bb[2], prev, 64);
+            bool head = hit && (!below || pb0 != bb[0] || pb1 != bb[1] || pb2 != bb[2]);
+            const int slot = __popcll(below);
+            // Head flags in compacted order: OR-reduce (head << slot).
+            unsigned long long hm = head ? (1ull << slot) : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) hm |= __shfl_xor(hm, off, 64);
+            if (hit) {
+                double fc = fv[c] * g.res3[c];
+#pragma unroll
+                for (int o = 0; o < 4; ++o) W[wave][o][slot] = w[0][o];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    double wyf = w[1][b] * fc;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) W[wave][4 + 4 * a + b][slot] = w[2][a] * wyf;
+                }
+                if (FAST) {
+                    // flat brick index of stencil point 0 (biased to stay positive) and,
+                    // per axis, which of the four offsets fall inside the brick
+                    int lx = min(g.a[c][0].points - P0x, BX), ly = min(g.a[c][1].points - P0y, BY);
+                    int lz = min(g.a[c][2].points - P0z, BZ);
+                    int mx = 0, my = 0, mz = 0;
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) {
+                        mx |= ((unsigned) (ow0 + o) < (unsigned) lx) << o;
+                        my |= ((unsigned) (ow1 + o) < (unsigned) ly) << o;
+                        mz |= ((unsigned) (ow2 + o) < (unsigned) lz) << o;
+                    }
+                    int flat = ow0 + BX * (ow1 + BY * ow2) + ORG_BIAS;
+                    org[wave][slot] = flat | (mx << 12) | (my << 16) | (mz << 20);
+                } else {
+                    org[wave][slot] = (bb[0] + 1) | ((bb[1] + 1) << 10) | ((bb[2] + 1) << 20);
+                }
+            }
+            const unsigned long long headmask =
+                ((unsigned long long) __builtin_amdgcn_readfirstlane((unsigned) (hm >> 32)) << 32) |
+                (unsigned) __builtin_amdgcn_readfirstlane((unsigned) hm);
+            wave_sync();
+            // ---- lanes are stencil points ----
+            const kaxis& ax = g.a[c][0];
+            const kaxis& ay = g.a[c][1];
+            const kaxis& az = g.a[c][2];
+            const double* wa = &W[wave][sx][0];
+            const double* wb = &W[wave][4 + 4 * sz + sy][0];
+            const int lane_off = sx + BX * (sy + BY * sz) - ORG_BIAS;
+            double P = 0.0;
+            int target = -1;
+#pragma unroll 8
+            for (int k = 0; k < nh; ++k) {
+                double a = wa[k];
+                double b = wb[k];
+                if ((headmask >> k) & 1) {
+                    if (target >= 0)
+                        atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
+                    P = 0.0;
+                    int o = org[wave][k];
+                    if (FAST) {
+                        int ok = (o >> (12 + sx)) & (o >> (16 + sy)) & (o >> (20 + sz)) & 1;
+                        target = ok ? (o & 4095) + lane_off : -1;
+                    } else {
+                        int ox = brick_offset<BX>(ax, (o & 1023) - 1, sx, P0x);
+                        int oy = brick_offset<BY>(ay, ((o >> 10) & 1023) - 1, sy, P0y);
+                        int oz = brick_offset<BZ>(az, ((o >> 20) & 1023) - 1, sz, P0z);
+                        target = (ox >= 0 && oy >= 0 && oz >= 0) ? ox + BX * (oy + BY * oz) : -1;
+                    }
+                }
+                P = fma(a, b, P);
+            }
+            if (target >= 0)
+                atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
+            wave_sync();
+        }
+    }
+    __syncthreads();
+
+    // Each grid point belongs to exactly one brick: plain stores.
+    const double q = ldexp(1.0, e2 - 62);
+    double* fe[3] = {fe0, fe1, fe2};
+    for (int i = tid; i < BRICK_PTS; i += SPREAD_THREADS) {
+        int tx = i % BX, ty = (i / BX) % BY, tz = i / (BX * BY);
+        int X = P0x + tx, Y = P0y + ty, Z = P0z + tz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int* pt = g.points[c];
+            if (X < pt[0] && Y < pt[1] && Z < pt[2])
+                fe[c][(size_t) X + (size_t) pt[0] * ((size_t) Y + (size_t) pt[1] * Z)] =
+                    (double) (long long) acc[c][i] * q;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------
+// Interpolation: one lane per marker.
+// ----------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+interp_kernel(marker_sets sets, kgrids g, int delta,
+              const double* __restrict__ u0, const double* __restrict__ u1,
+              const double* __restrict__ u2)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= sets.total) return;
+    int k = set_of(sets, m);
+    int l = m - sets.offset[k];
+    int n = sets.offset[k + 1] - sets.offset[k];
+    const double* x = sets.x[k];
+    double p[3] = {x[l], x[n + l], x[2 * n + l]};
+    const double* ue[3] = {u0, u1, u2};
+    for (int c = 0; c < 3; ++c) {
+        double w[3][4];
+        int ix[4], oy[4], oz[4];
+        const int* pt = g.points[c];
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = axis_stencil(g.a[c][j], p[j]);
+            delta4(delta, st.d, w[j]);
+            for (int o = 0; o < 4; ++o) {
+                int idx = axis_interp_index(g.a[c][j], st.base + o - 1);
+                if (j == 0) ix[o] = idx;
+                else if (j == 1) oy[o] = idx * pt[0];
+                else oz[o] = idx * pt[0] * pt[1];
+            }
+        }
+        const double* u = ue[c];
+        double acc = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            double sy = 0.0;
+            for (int b = 0; b < 4; ++b) {
+                const double* row = u + oz[a] + oy[b];
+                double sx = w[0][0] * row[ix[0]];
+                sx = fma(w[0][1], row[ix[1]], sx);
+                sx = fma(w[0][2], row[ix[2]], sx);
+                sx = fma(w[0][3], row[ix[3]], sx);
+                sy = fma(w[1][b], sx, sy);
+            }
+            acc = fma(w[2][a], sy, acc);
+        }
+        sets.out[k][(size_t) c * n + l] = acc;
+    }
+}
+
+// ----------------------------------------------------------------------
+// Host side
+// ----------------------------------------------------------------------
+kaxis to_kaxis(const ibm_axis& a)
+{
+    kaxis k;
+    k.length = a.length; k.resolution = a.resolution; k.shift = a.shift;
+    k.s_over_r = a.shift / a.resolution;
+    k.cells = a.cells; k.points = a.points; k.solid = a.solid; k.pad = 0;
+    return k;
+}
+
+ibm_status make_kgrids(ibm_ctx* ctx, const ibm_grid* grids, kgrids* out)
+{
+    if (!grids) return fail(ctx, IBM_ERR_ARG, "grids is null");
+    for (int c = 0; c < 3; ++c) {
+        if (grids[c].dims != 3)
+            return fail(ctx, IBM_ERR_ARG, "spread/interpolate support 3-dimensional grids (got %d)", grids[c].dims);
+        double r = 1.0;
+        for (int j = 0; j < 3; ++j) {
+            out->a[c][j] = to_kaxis(grids[c].axis[j]);
+            out->points[c][j] = grids[c].axis[j].points;
+            r *= grids[c].axis[j].resolution;
+            if (grids[c].axis[j].cells < 1 || grids[c].axis[j].points < 1)
+                return fail(ctx, IBM_ERR_BAD_GRID, "empty grid axis");
+        }
+        out->res3[c] = r;
+    }
+    // Along each dimension the component grids may use at most two distinct
+    // discretisations (MAC: the face-aligned one and the centred one).
+    for (int j = 0; j < 3; ++j) {
+        out->cand[0][j] = 0;
+        out->cand[1][j] = 0;
+        int ncand = 1;
+        for (int c = 0; c < 3; ++c) {
+            const ibm_axis& a = grids[c].axis[j];
+            int found = -1;
+            for (int k = 0; k < ncand; ++k) {
+                const ibm_axis& b = grids[out->cand[k][j]].axis[j];
+                if (a.shift == b.shift && a.resolution == b.resolution && a.cells == b.cells &&
+                    a.points == b.points && a.solid == b.solid && a.length == b.length) {
+                    found = k;
+                    break;
+                }
+            }
+            if (found < 0) {
+                if (ncand == 2)
+                    return fail(ctx, IBM_ERR_BAD_GRID, "more than two staggerings along dimension %d", j);
+                out->cand[1][j] = c;
+                found = ncand++;
+            }
+            out->sel[c][j] = found;
+        }
+        // Canonical order for a MAC-shaped axis (component j alone differs):
+        // candidate 1 is component j, candidate 0 the shared centred axis.
+        if (ncand == 2) {
+            int other = (j + 1) % 3, third = (j + 2) % 3;
+            if (out->sel[j][j] != out->sel[other][j] && out->sel[other][j] == out->sel[third][j]) {
+                out->cand[0][j] = other;
+                out->cand[1][j] = j;
+                for (int c = 0; c < 3; ++c) out->sel[c][j] = (c == j) ? 1 : 0;
+            }
+        }
+    }
+    return IBM_OK;
+}
+
+ibm_status make_sets(ibm_ctx* ctx, int nsets, const int* n, const double* const* x,
+                     const double* const* f, double* const* out, marker_sets* s)
+{
+    if (nsets < 0 || nsets > MAX_SETS) return fail(ctx, IBM_ERR_ARG, "nsets must be in [0, %d]", MAX_SETS);
+    if (nsets > 0 && (!n || !x)) return fail(ctx, IBM_ERR_ARG, "null marker arrays");
+    s->nsets = nsets;
+    long long total = 0;
+    s->offset[0] = 0;
+    for (int k = 0; k < nsets; ++k) {
+        if (n[k] < 0) return fail(ctx, IBM_ERR_SIZE, "negative marker count");
+        if (n[k] > 0 && !x[k]) return fail(ctx, IBM_ERR_ARG, "null positions for set %d", k);
+        if (n[k] > 0 && f && !f[k]) return fail(ctx, IBM_ERR_ARG, "null forces for set %d", k);
+        if (n[k] > 0 && out && !out[k]) return fail(ctx, IBM_ERR_ARG, "null output for set %d", k);
+        s->x[k] = x[k];
+        s->f[k] = f ? f[k] : nullptr;
+        s->out[k] = out ? out[k] : nullptr;
+        total += n[k];
+        s->offset[k + 1] = (int) total;
+    }
+    if (total > (1LL << 30)) return fail(ctx, IBM_ERR_SIZE, "too many markers");
+    s->total = (int) total;
+    return IBM_OK;
+}
+
+bin_desc make_bins(const ibm_grid* grids)
+{
+    bin_desc bd;
+    int nbins = 1;
+    for (int j = 0; j < 3; ++j) {
+        int fc = -1;
+        for (int c = 0; c < 3; ++c)
+            if (grids[c].axis[j].alignment == 0.0) { fc = c; break; }
+        if (fc < 0) fc = 0;
+        const ibm_axis& fa = grids[fc].axis[j];
+        int fs = (int) fa.shift;                  // 0 or 1 for face grids, 0 otherwise
+        bd.fc[j] = fc;
+        bd.fs[j] = fs;
+        if (fa.solid) { bd.qmin[j] = -1 + fs; bd.nb[j] = fa.points + 1; }
+        else { bd.qmin[j] = 0; bd.nb[j] = fa.cells; }
+        // base_c - q = floor(y - D) - floor(y) - fs with y = x/h - fs and
+        // D = shift_c - fs, i.e. in {-ceil(D) - fs, -floor(D) - fs}.
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int c = 0; c < 3; ++c) {
+            double delta = grids[c].axis[j].shift - fs;
+            int a = -(int) ceil(delta) - fs, b = -(int) floor(delta) - fs;
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        bd.lo[j] = lo;
+        bd.hi[j] = hi;
+        nbins *= bd.nb[j];
+    }
+    bd.nbins = nbins;
+    return bd;
+}
+
+int blocks_for(long long n, int threads) { return (int) ((n + threads - 1) / threads); }
+
+// Counting sort of all markers into bins; leaves `start` (nbins+1), `perm`
+// and sorted positions (and forces) in scratch.
+ibm_status bin_markers(ibm_ctx* ctx, const marker_sets& sets, const kgrids& kg, const bin_desc& bd,
+                       int with_forces, int** start_out, int** perm_out, double** sorted_out,
+                       bin_stats** stats_out)
+{
+    const int total = sets.total;
+    const int nbins = bd.nbins;
+    int *count, *start, *key, *rank, *tmp, *perm, *code;
+    double* sorted;
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_BIN_COUNT, sizeof(int) * (size_t) (nbins + 1), &p)); count = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_BIN_START, sizeof(int) * (size_t) (nbins + 1), &p)); start = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_KEY, sizeof(int) * (size_t) total, &p)); key = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_RANK, sizeof(int) * (size_t) total, &p)); rank = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_TMP, sizeof(int) * (size_t) total, &p)); tmp = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_PERM, sizeof(int) * (size_t) total, &p)); perm = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_CODE, sizeof(int) * (size_t) total, &p)); code = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_SORTED, sizeof(double) * 6 * (size_t) total, &p)); sorted = (double*) p;
+    bin_stats* stats;
+    IBM_TRY(scratch(ctx, SLOT_FLAGS, sizeof(bin_stats), &p)); stats = (bin_stats*) p;
+
+    prof_scope prof(ctx, "bin");
+    IBM_HIP(ctx, hipMemsetAsync(count, 0, sizeof(int) * (size_t) (nbins + 1), ctx->stream));
+    IBM_HIP(ctx, hipMemsetAsync(stats, 0, sizeof(bin_stats), ctx->stream));
+    if (total > 0) {
+        bin_count_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, with_forces, key, rank,
+                                                                           code, count, stats);
+        IBM_LAUNCH_CHECK(ctx);
+    }
+    size_t tbytes = 0;
+    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, count, start, nbins + 1, ctx->stream));
+    IBM_TRY(scratch(ctx, SLOT_SCAN_TMP, tbytes, &p));
+    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(p, tbytes, count, start, nbins + 1, ctx->stream));
+    if (total > 0) {
+        bin_scatter_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(total, key, rank, start, tmp);
+        IBM_LAUNCH_CHECK(ctx);
+        bin_canon_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, with_forces, key, code, start,
+                                                                           tmp, perm, sorted, stats);
+        IBM_LAUNCH_CHECK(ctx);
+    }
+    *start_out = start;
+    *perm_out = perm;
+    *sorted_out = sorted;
+    *stats_out = stats;
+    return IBM_OK;
+}
+
+ibm_status check_delta(ibm_ctx* ctx, int delta)
+{
+    if (delta != IBM_DELTA_COSINE && delta != IBM_DELTA_PESKIN && delta != IBM_DELTA_BSPLINE3)
+        return fail(ctx, IBM_ERR_ARG, "unknown delta kind %d", delta);
+    return IBM_OK;
+}
+
+} // namespace
+
+ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const marker_sets& sets,
+                       double* const* fe)
+{
+    kgrids kg;
+    IBM_TRY(make_kgrids(ctx, grids, &kg));
+    IBM_TRY(check_delta(ctx, delta));
+    if (!fe || !fe[0] || !fe[1] || !fe[2]) return fail(ctx, IBM_ERR_ARG, "null output grid");
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j)
+            if (!kg.a[c][j].solid && kg.a[c][j].cells < 4)
+                return fail(ctx, IBM_ERR_BAD_GRID, "periodic dimension needs at least 4 cells for a 4-point stencil");
+    bin_desc bd = make_bins(grids);
+    for (int j = 0; j < 3; ++j)
+        if (bd.hi[j] - bd.lo[j] > 1 || bd.hi[j] < bd.lo[j])
+            return fail(ctx, IBM_ERR_BAD_GRID, "component grids are not a staggering of one cell grid");
+    int *start, *perm;
+    double* sorted;
+    bin_stats* stats;
+    IBM_TRY(bin_markers(ctx, sets, kg, bd, 1, &start, &perm, &sorted, &stats));
+    int maxp[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j) maxp[j] = kg.points[c][j] > maxp[j] ? kg.points[c][j] : maxp[j];
+    int bx = (maxp[0] + BX - 1) / BX, by = (maxp[1] + BY - 1) / BY, bz = (maxp[2] + BZ - 1) / BZ;
+    long long nbricks = (long long) bx * by * bz;
+    if (nbricks > 0x7fffffff) return fail(ctx, IBM_ERR_SIZE, "grid too large");
+    int xcd_chunk = (nbricks % 8 == 0) ? (int) (nbricks / 8) : 0;
+    bool mac = true, fast = true;
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j) {
+            if (kg.sel[c][j] != (c == j ? 1 : 0) || kg.cand[1][j] != j || kg.cand[0][j] == j) mac = false;
+            if (!kg.a[c][j].solid && kg.a[c][j].cells < MIN_FAST_CELLS) fast = false;
+        }
+    if (!fast)
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j)
+                if (kg.a[c][j].points > 1022)
+                    return fail(ctx, IBM_ERR_SIZE, "small periodic axes with more than 1022 points on another axis");
+    prof_scope prof(ctx, "spread_brick");
+    auto launch = [&](auto kernel) {
+        kernel<<<(int) nbricks, SPREAD_THREADS, 0, ctx->stream>>>(
+            kg, bd, delta, sets.total, start, sorted, stats, fe[0], fe[1], fe[2], bx, by, (int) nbricks, xcd_chunk);
+    };
+    if (mac && fast) launch(spread_brick_kernel<true, true>);
+    else if (mac) launch(spread_brick_kernel<true, false>);
+    else if (fast) launch(spread_brick_kernel<false, true>);
+    else launch(spread_brick_kernel<false, false>);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status interp_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const marker_sets& sets,
+                       const double* const* ue)
+{
+    kgrids kg;
+    IBM_TRY(make_kgrids(ctx, grids, &kg));
+    IBM_TRY(check_delta(ctx, delta));
+    if (!ue || !ue[0] || !ue[1] || !ue[2]) return fail(ctx, IBM_ERR_ARG, "null input grid");
+    if (sets.total == 0) return IBM_OK;
+    prof_scope prof(ctx, "interp");
+    interp_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, kg, delta, ue[0], ue[1], ue[2]);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+} // namespace ibm
+
+extern "C" {
+
+ibm_status ibm_spread_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nsets,
+                           const int* n, const double* const* x, const double* const* f,
+                           double* const* fe)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (nsets > 0 && !f) return ibm::fail(ctx, IBM_ERR_ARG, "null force arrays");
+    ibm::marker_sets s;
+    IBM_TRY(ibm::make_sets(ctx, nsets, n, x, f, nullptr, &s));
+    return ibm::spread_sets(ctx, grids, (int) delta, s, fe);
+}
+
+ibm_status ibm_spread(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int n,
+                      const double* x, const double* f, double* const* fe)
+{
+    return ibm_spread_sets(ctx, grids, delta, 1, &n, &x, &f, fe);
+}
+
+ibm_status ibm_interpolate_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nsets,
+                                const int* n, const double* const* x, const double* const* ue,
+                                double* const* ul)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (nsets > 0 && !ul) return ibm::fail(ctx, IBM_ERR_ARG, "null output arrays");
+    ibm::marker_sets s;
+    IBM_TRY(ibm::make_sets(ctx, nsets, n, x, nullptr, ul, &s));
+    return ibm::interp_sets(ctx, grids, (int) delta, s, ue);
+}
+
+ibm_status ibm_interpolate(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int n,
+                           const double* x, const double* const* ue, double* ul)
+{
+    return ibm_interpolate_sets(ctx, grids, delta, 1, &n, &x, ue, &ul);
+}
+
+} // extern "C"

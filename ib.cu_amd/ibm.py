@@ -1,0 +1,320 @@
+"""Python host binding of libibm.so (ctypes).
+
+Mirrors the reference's operator interface for the hot path so the tests and
+the benchmark read like the reference's programs:
+
+    grids = mac_grids(domain, refinement)              # fd::grid per component
+    fe = spread(ctx, grids, delta, x, f)               # ib::spread (ib/novel.h:236-244)
+    ul = interpolate(ctx, grids, delta, x, fe)         # ib::interpolate (ib/interpolate.h:124-131)
+
+Device memory comes from torch tensors (plumbing only); every computation
+runs in libibm.so.  Importing this module does not need a GPU; calling into
+the library does.  There is no fallback: a missing library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibm.so")
+
+# ----------------------------------------------------------------------
+# Units (units.h:16-18,221-276): 1 length unit = 10 um, 1 mass unit = 1 ug,
+# 1 time unit = 1 ms.
+# ----------------------------------------------------------------------
+um = 0.1
+cm = 1000.0
+m = 1e5
+s = 1000.0
+ms = 1.0
+ns = 1e-6
+g = 1e6
+cP = 0.01
+mL = cm ** 3
+
+DELTA = {"cosine": 0, "peskin": 1, "bspline3": 2}
+
+
+class ibm_dimension(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_double), ("solid", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("lower", ctypes.c_double * 2), ("upper", ctypes.c_double * 2)]
+
+
+class ibm_axis(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_double), ("resolution", ctypes.c_double),
+                ("alignment", ctypes.c_double), ("shift", ctypes.c_double),
+                ("cells", ctypes.c_int), ("points", ctypes.c_int), ("solid", ctypes.c_int),
+                ("reserved", ctypes.c_int),
+                ("boundary", ctypes.c_double * 2), ("interior", ctypes.c_double * 2),
+                ("laplacian", ctypes.c_double * 2), ("robin", (ctypes.c_double * 2) * 2)]
+
+
+class ibm_grid(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int), ("refinement", ctypes.c_int), ("unit", ctypes.c_double),
+                ("axis", ibm_axis * 3)]
+
+    @property
+    def points(self):
+        return tuple(self.axis[i].points for i in range(self.dims))
+
+    @property
+    def size(self):
+        n = 1
+        for p in self.points:
+            n *= p
+        return n
+
+    @property
+    def resolution(self):
+        return self.axis[0].resolution
+
+
+class IbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ibm status {code}: {msg}")
+        self.code = code
+
+
+OK, ERR_HIP, ERR_ARG, ERR_SIZE, ERR_NOT_CONVERGED, ERR_DT_TOO_SMALL, ERR_BAD_GRID = range(7)
+
+_lib = None
+
+# (name, restype, argtypes) of every exported C entry point used from Python.
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_SIGS = {
+    "ibm_domain_unit": (_I, [ctypes.POINTER(ibm_dimension), _I, _PD]),
+    "ibm_grid_make": (_I, [ctypes.POINTER(ibm_dimension), _I, _I, _PD, ctypes.POINTER(ibm_grid)]),
+    "ibm_mac_grids": (_I, [ctypes.POINTER(ibm_dimension), _I, _I, ctypes.POINTER(ibm_grid)]),
+    "ibm_centered_grid": (_I, [ctypes.POINTER(ibm_dimension), _I, _I, ctypes.POINTER(ibm_grid)]),
+    "ibm_grid_coarsen": (_I, [ctypes.POINTER(ibm_grid), ctypes.POINTER(ibm_grid)]),
+    "ibm_grid_size": (ctypes.c_int64, [ctypes.POINTER(ibm_grid)]),
+    "ibm_create": (_I, [ctypes.POINTER(_VP), _I]),
+    "ibm_destroy": (_I, [_VP]),
+    "ibm_set_stream": (_I, [_VP, _VP]),
+    "ibm_synchronize": (_I, [_VP]),
+    "ibm_last_error": (ctypes.c_char_p, [_VP]),
+    "ibm_profile_enable": (_I, [_VP, _I]),
+    "ibm_profile_read": (_I, [_VP, ctypes.c_char_p, _PD, ctypes.POINTER(_I)]),
+    "ibm_spread": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
+    "ibm_spread_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
+                             ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ibm_interpolate": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, ctypes.POINTER(_VP), _VP]),
+    "ibm_interpolate_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
+                                  ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+}
+
+
+def lib():
+    """Loads libibm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(code, ctx=None):
+    if code != OK:
+        if ctx is None:
+            raise IbmError(code, "grid construction failed")
+        msg = lib().ibm_last_error(ctx.handle)
+        raise IbmError(code, msg.decode() if msg else "")
+
+
+# ----------------------------------------------------------------------
+# Domains and grids (fd::dimension, fd::domain, fd::grid)
+# ----------------------------------------------------------------------
+def periodic(length):
+    """fd::dimension{length, fd::boundary::periodic}"""
+    d = ibm_dimension()
+    d.length = length
+    d.solid = 0
+    return d
+
+
+def robin(length, lower=(1.0, 0.0), upper=None):
+    """fd::dimension{length, robin lower[, upper]}; default Dirichlet (1, 0)."""
+    upper = lower if upper is None else upper
+    d = ibm_dimension()
+    d.length = length
+    d.solid = 1
+    d.lower[0], d.lower[1] = lower
+    d.upper[0], d.upper[1] = upper
+    return d
+
+
+def dirichlet(length):
+    return robin(length, (1.0, 0.0))
+
+
+def neumann(length):
+    return robin(length, (0.0, 1.0))
+
+
+def _dims(domain):
+    arr = (ibm_dimension * len(domain))(*domain)
+    return arr, len(domain)
+
+
+def domain_unit(domain):
+    arr, n = _dims(domain)
+    out = ctypes.c_double()
+    _check(lib().ibm_domain_unit(arr, n, ctypes.byref(out)))
+    return out.value
+
+
+def grid_make(domain, refinement, alignment):
+    arr, n = _dims(domain)
+    al = (ctypes.c_double * 3)(*alignment)
+    out = ibm_grid()
+    _check(lib().ibm_grid_make(arr, n, refinement, al, ctypes.byref(out)))
+    return out
+
+
+def mac_grids(domain, refinement):
+    arr, n = _dims(domain)
+    out = (ibm_grid * n)()
+    _check(lib().ibm_mac_grids(arr, n, refinement, out))
+    return out
+
+
+def centered_grid(domain, refinement):
+    arr, n = _dims(domain)
+    out = ibm_grid()
+    _check(lib().ibm_centered_grid(arr, n, refinement, ctypes.byref(out)))
+    return out
+
+
+def grid_coarsen(grid):
+    out = ibm_grid()
+    _check(lib().ibm_grid_coarsen(ctypes.byref(grid), ctypes.byref(out)))
+    return out
+
+
+# ----------------------------------------------------------------------
+# Context
+# ----------------------------------------------------------------------
+class Context:
+    """Owns an ibm_ctx bound to a device; work is enqueued on `stream`
+    (a torch.cuda.Stream or raw hipStream_t int; default torch's current)."""
+
+    def __init__(self, device=0, stream=None):
+        h = ctypes.c_void_p()
+        code = lib().ibm_create(ctypes.byref(h), device)
+        if code != OK:
+            raise IbmError(code, "ibm_create failed")
+        self.handle = h
+        self.set_stream(stream)
+
+    def set_stream(self, stream=None):
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        _check(lib().ibm_set_stream(self.handle, ctypes.c_void_p(stream)), self)
+
+    def synchronize(self):
+        _check(lib().ibm_synchronize(self.handle), self)
+
+    def profile(self, on=True):
+        _check(lib().ibm_profile_enable(self.handle, 1 if on else 0), self)
+
+    def profile_read(self, name):
+        """(total milliseconds, launches) recorded for a kernel name."""
+        t = ctypes.c_double()
+        k = ctypes.c_int()
+        _check(lib().ibm_profile_read(self.handle, name.encode(), ctypes.byref(t), ctypes.byref(k)), self)
+        return t.value, k.value
+
+    def close(self):
+        if self.handle:
+            lib().ibm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def _check_markers(x, n):
+    import torch
+    if x.dtype != torch.float64 or not x.is_contiguous() or x.numel() != 3 * n:
+        raise ValueError("marker arrays must be contiguous float64 with 3*n entries")
+
+
+# ----------------------------------------------------------------------
+# ib::spread / ib::interpolate
+# ----------------------------------------------------------------------
+def spread(ctx, grids, delta, x, f, out=None):
+    """Spreads SoA forces f (3*n) at SoA positions x (3*n) onto the three
+    component grids; returns (or fills) three float64 tensors."""
+    import torch
+    n = x.numel() // 3
+    _check_markers(x, n)
+    _check_markers(f, n)
+    if out is None:
+        out = [torch.empty(grids[c].size, dtype=torch.float64, device=x.device) for c in range(3)]
+    code = lib().ibm_spread(ctx.handle, grids, DELTA.get(delta, delta), n, _ptr(x), _ptr(f), _ptrs(out))
+    _check(code, ctx)
+    return out
+
+
+def spread_sets(ctx, grids, delta, xs, fs, out=None):
+    import torch
+    ns = [x.numel() // 3 for x in xs]
+    for x, f, n in zip(xs, fs, ns):
+        _check_markers(x, n)
+        _check_markers(f, n)
+    if out is None:
+        out = [torch.empty(grids[c].size, dtype=torch.float64, device=xs[0].device) for c in range(3)]
+    code = lib().ibm_spread_sets(ctx.handle, grids, DELTA.get(delta, delta), len(xs),
+                                 (ctypes.c_int * len(ns))(*ns), _ptrs(xs), _ptrs(fs), _ptrs(out))
+    _check(code, ctx)
+    return out
+
+
+def interpolate(ctx, grids, delta, x, ue, out=None):
+    import torch
+    n = x.numel() // 3
+    _check_markers(x, n)
+    if out is None:
+        out = torch.empty(3 * n, dtype=torch.float64, device=x.device)
+    code = lib().ibm_interpolate(ctx.handle, grids, DELTA.get(delta, delta), n, _ptr(x), _ptrs(ue), _ptr(out))
+    _check(code, ctx)
+    return out
+
+
+def interpolate_sets(ctx, grids, delta, xs, ue, outs=None):
+    import torch
+    ns = [x.numel() // 3 for x in xs]
+    for x, n in zip(xs, ns):
+        _check_markers(x, n)
+    if outs is None:
+        outs = [torch.empty(3 * n, dtype=torch.float64, device=xs[0].device) for n in ns]
+    code = lib().ibm_interpolate_sets(ctx.handle, grids, DELTA.get(delta, delta), len(xs),
+                                      (ctypes.c_int * len(ns))(*ns), _ptrs(xs), _ptrs(ue), _ptrs(outs))
+    _check(code, ctx)
+    return outs

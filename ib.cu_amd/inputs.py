@@ -1,0 +1,203 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+All inputs are generated on the host in FP64 by this module, so the GPU path
+and the CPU oracle see identical bits:
+  * random numbers: SplitMix64 -> 53-bit uniforms -> Box-Muller normals;
+  * sphere sample sites: the Bauer spiral of bases/shapes/sphere.h:80-99 (the
+    reference falls back to it when its data/*.bin site files are absent,
+    include/rbc.h:52-58);
+  * cell shapes: RBC biconcave map (include/rbc.h:19-35), platelet ellipsoid
+    (include/platelet.h:21-34);
+  * poses: bases::rotate (Rodrigues, bases/transforms.h:102-128) then translate.
+Lengths are in library units (1 unit = 10 um).
+"""
+import math
+
+import numpy as np
+
+UM = 0.1
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea & Flood 2014) producing 53-bit uniforms."""
+
+    def __init__(self, seed):
+        self.state = np.uint64(seed & MASK64)
+
+    def next_u64(self, n):
+        # Vectorised: state_k = seed + k * golden, then the mix function.
+        golden = np.uint64(0x9E3779B97F4A7C15)
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + k * golden
+            self.state = self.state + np.uint64(n) * golden
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n):
+        """Uniforms in [0, 1) with 53 random bits."""
+        return (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+    def normal(self, n):
+        """N(0,1) by Box-Muller on pairs of uniforms (u1 in (0,1])."""
+        m = (n + 1) // 2
+        u = self.uniform(2 * m)
+        u1 = 1.0 - u[0::2]
+        u2 = u[1::2]
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.empty(2 * m)
+        z[0::2] = r * np.cos(2 * math.pi * u2)
+        z[1::2] = r * np.sin(2 * math.pi * u2)
+        return z[:n]
+
+
+def bauer_params(n):
+    """(theta, phi) of the n-point Bauer spiral (bases/shapes/sphere.h:80-99)."""
+    t = np.arange(n, dtype=np.float64)
+    l = math.sqrt(math.pi * n)
+    z = -1.0 + (2.0 * t + 1.0) / n
+    phi = np.arcsin(z)
+    theta = np.fmod(l * phi, 2 * math.pi)
+    return theta, phi
+
+
+def sphere_shape(theta, phi, radius=1.0):
+    x = np.cos(theta) * np.cos(phi)
+    y = np.sin(theta) * np.cos(phi)
+    z = np.sin(phi)
+    return np.stack([radius * x, radius * y, radius * z])
+
+
+RBC_RADIUS = 3.91 * UM
+PLT_MAJOR = 1.55 * UM
+PLT_MINOR = 0.50 * UM
+
+
+def rbc_shape(theta, phi):
+    """Biconcave RBC (include/rbc.h:19-35); returns 3 x n."""
+    x = np.cos(theta) * np.cos(phi)
+    y = np.sin(theta) * np.cos(phi)
+    z0 = np.sin(phi)
+    r2 = x * x + y * y
+    z = 0.5 * z0 * (0.21 + 2.0 * r2 - 1.12 * r2 * r2)
+    return np.stack([RBC_RADIUS * x, RBC_RADIUS * z, RBC_RADIUS * y])
+
+
+def platelet_shape(theta, phi):
+    """Platelet ellipsoid (include/platelet.h:96-108); returns 3 x n."""
+    x = np.cos(theta) * np.cos(phi)
+    y = np.sin(theta) * np.cos(phi)
+    z = np.sin(phi)
+    return np.stack([PLT_MAJOR * x, PLT_MINOR * y, PLT_MAJOR * z])
+
+
+def rotate(points, angle, axis):
+    """bases::rotate (Rodrigues) applied to 3 x n points."""
+    a = np.asarray(axis, dtype=np.float64)
+    l = math.sqrt(float(a @ a))
+    c, s = math.cos(angle), math.sin(angle)
+    d = (a @ points) / l
+    w = np.cross(a[:, None], points, axis=0)
+    return c * points + s * w / l + (1 - c) * d[None, :] * a[:, None] / l
+
+
+def soa(cells):
+    """Concatenates a list of 3 x n_k arrays into the reference's SoA
+    layout x[n*i + m] (ib/novel.h:83, bases/container.h:54)."""
+    pts = np.concatenate(cells, axis=1)
+    return np.ascontiguousarray(pts.reshape(-1))
+
+
+# ----------------------------------------------------------------------
+# Configurations
+# ----------------------------------------------------------------------
+def config_c1(seed=1):
+    """One sphere of 2562 Bauer-spiral markers, radius 3.9 um, centred in a
+    periodic 16 um box; forces iid N(0,1) per component."""
+    n = 2562
+    th, ph = bauer_params(n)
+    x = sphere_shape(th, ph, 3.9 * UM) + 8.0 * UM
+    rng = SplitMix64(seed)
+    f = rng.normal(3 * n)
+    return {"L": (16 * UM,) * 3, "walls": False, "refinement": 32, "x": soa([x]), "f": f, "n": n}
+
+
+def rbc_reference(ns):
+    th, ph = bauer_params(ns)
+    return rbc_shape(th, ph)
+
+
+def platelet_reference(ns):
+    th, ph = bauer_params(ns)
+    return platelet_shape(th, ph)
+
+
+def layered_cells(Lx, Ly, Lz, n_rbc, n_plt, ns_rbc, ns_plt, seed):
+    """Seeded non-overlapping placement: RBC layers (discs normal to y) on a
+    4x4-per-layer lattice with small jitter/tilt, platelets on a 6x6 lattice
+    in the gaps between RBC layers.  Returns (rbc_positions, plt_positions,
+    poses) with positions as lists of 3 x ns arrays."""
+    rng = SplitMix64(seed)
+    rbc_ref = rbc_reference(ns_rbc)
+    plt_ref = platelet_reference(ns_plt)
+    per_layer_x = max(int(Lx / (8.0 * UM)), 1)
+    per_layer_z = max(int(Lz / (8.0 * UM)), 1)
+    per_layer = per_layer_x * per_layer_z
+    layers = (n_rbc + per_layer - 1) // per_layer if n_rbc else 0
+    dy = Ly / max(layers, 1)
+    rbcs = []
+    for k in range(n_rbc):
+        layer, slot = divmod(k, per_layer)
+        ix, iz = slot % per_layer_x, slot // per_layer_x
+        u = rng.uniform(5)
+        cx = (ix + 0.5) * Lx / per_layer_x + (u[0] - 0.5) * 0.08 * UM
+        cz = (iz + 0.5) * Lz / per_layer_z + (u[1] - 0.5) * 0.08 * UM
+        cy = (layer + 0.5) * dy + (u[2] - 0.5) * 0.1 * UM
+        spin = 2 * math.pi * u[3]
+        tilt = math.radians(2.0) * (u[4] - 0.5)
+        p = rotate(rbc_ref, spin, (0.0, 1.0, 0.0))
+        p = rotate(p, tilt, (1.0, 0.0, 0.0))
+        rbcs.append(p + np.array([cx, cy, cz])[:, None])
+    plts = []
+    gaps = max(layers - 1, 1)
+    nx = max(int(Lx / (5.0 * UM)), 1)
+    nz = max(int(Lz / (5.0 * UM)), 1)
+    for k in range(n_plt):
+        gap, slot = divmod(k, nx * nz)
+        gap = gap % gaps
+        ix, iz = slot % nx, slot // nx
+        u = rng.uniform(3)
+        cx = (ix + 0.5) * Lx / nx + (u[0] - 0.5) * 0.5 * UM
+        cz = (iz + 0.5) * Lz / nz + (u[1] - 0.5) * 0.5 * UM
+        cy = (gap + 1) * dy if layers > 1 else 0.5 * Ly
+        p = rotate(plt_ref, 2 * math.pi * u[2], (0.0, 1.0, 0.0))
+        plts.append(p + np.array([cx, cy, cz])[:, None])
+    return rbcs, plts
+
+
+def config_c4(seed=4, ns_rbc=8832, ns_plt=1552):
+    """200 RBCs + 400 platelets in a 32 x 64 x 32 um box, walls in y,
+    fd::mac{128} over a 32 um base unit -> 128 x 256 x 128 cells."""
+    L = (32 * UM, 64 * UM, 32 * UM)
+    rbcs, plts = layered_cells(*L, 200, 400, ns_rbc, ns_plt, seed)
+    rng = SplitMix64(seed + 1000)
+    xr = soa(rbcs)
+    xp = soa(plts)
+    nr = xr.size // 3
+    npl = xp.size // 3
+    return {"L": L, "walls": True, "refinement": 128,
+            "x_rbc": xr, "x_plt": xp, "n_rbc": nr, "n_plt": npl,
+            "f_rbc": rng.normal(3 * nr), "f_plt": rng.normal(3 * npl)}
+
+
+def random_markers(n, L, seed, margin=0.0):
+    """n markers uniform in the box [margin, L - margin] per dimension (SoA)."""
+    rng = SplitMix64(seed)
+    out = []
+    for j in range(3):
+        u = rng.uniform(n)
+        out.append(margin + u * (L[j] - 2 * margin))
+    return np.ascontiguousarray(np.concatenate(out))

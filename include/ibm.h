@@ -1,0 +1,184 @@
+/*
+ * ibm.h — C ABI of the MI355X immersed-boundary timestep library (libibm.so).
+ *
+ * Every entry point is plain C: device pointers, sizes and POD descriptors,
+ * no C++ or HIP types in the signatures.  Work is enqueued on the context's
+ * HIP stream (ibm_set_stream) and is asynchronous unless a function says it
+ * returns a host scalar.  Errors are reported as ibm_status codes; the text
+ * of the last error on a context is returned by ibm_last_error().
+ *
+ * Conventions shared with the reference (sonwell/ib.cu):
+ *   - marker coordinates and forces are SoA, column-major n x 3:
+ *       x[n*i + m] is coordinate i of marker m            (ib/novel.h:83)
+ *   - grid vectors are linearised x-fastest:
+ *       k = i0 + p0*(i1 + p1*i2), p = points per dimension (ib/indexing.h:27-33)
+ *   - lengths are in library units: 1 unit = 10 um, 1 time unit = 1 ms
+ *                                                      (units.h:16-18)
+ */
+#ifndef IBM_H
+#define IBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes.  The C++ mirror headers map these to the reference's   */
+/* exception types (fd::bad_grid_points, linalg::mismatch, ...).        */
+/* ------------------------------------------------------------------ */
+typedef enum ibm_status {
+    IBM_OK = 0,
+    IBM_ERR_HIP = 1,            /* a HIP runtime call failed                      */
+    IBM_ERR_ARG = 2,            /* invalid argument (null pointer, bad enum, ...) */
+    IBM_ERR_SIZE = 3,           /* size mismatch           (linalg/size.h:17-23)  */
+    IBM_ERR_NOT_CONVERGED = 4,  /* iterative solve hit its iteration cap          */
+    IBM_ERR_DT_TOO_SMALL = 5,   /* time step bound below kmin  (ins/tracker.h:93) */
+    IBM_ERR_BAD_GRID = 6        /* bad dimension / grid        (fd/dimension.h:29)*/
+} ibm_status;
+
+/* Regularised delta kernels on a 4-point stencil.
+ *   COSINE   : phi(r) = (1 + cos(pi r / 2)) / 4          (ib/cosine.h:9-16)
+ *   PESKIN   : Peskin's 4-point kernel                   (ib/peskin.h:6-14)
+ *   BSPLINE3 : cubic cardinal B-spline                   (ib/bspline.h:13-27, n = 3) */
+typedef enum ibm_delta {
+    IBM_DELTA_COSINE = 0,
+    IBM_DELTA_PESKIN = 1,
+    IBM_DELTA_BSPLINE3 = 2
+} ibm_delta;
+
+/* One dimension of the physical box, fd::dimension (fd/dimension.h:36-88).
+ * A periodic dimension has solid = 0.  A solid dimension carries a Robin
+ * condition  alpha*u + beta*du/dn = gamma  at each wall (fd/boundary.h:31-42):
+ * Dirichlet = (1, 0), Neumann = (0, 1). */
+typedef struct ibm_dimension {
+    double length;
+    int solid;
+    int reserved;
+    double lower[2];
+    double upper[2];
+} ibm_dimension;
+
+/* Discretisation of one dimension for one grid, fd::discretization
+ * (fd/discretization.h:87-179).  Filled by ibm_grid_make(). */
+typedef struct ibm_axis {
+    double length;       /* dimension length                                   */
+    double resolution;   /* 1/h                                                */
+    double alignment;    /* staggering in [0,1): 0 = on cell faces, 0.5 = centres */
+    double shift;        /* alignment + (solid && on_boundary)      (:113)     */
+    int cells;           /* floor(length*resolution + 0.5)          (:161)     */
+    int points;          /* cells - (on_boundary && solid)          (:162)     */
+    int solid;
+    int reserved;
+    double boundary[2];  /* ghost-cell coefficient of gamma   [lower, upper] (:56) */
+    double interior[2];  /* ghost-cell coefficient of u_0     [lower, upper] (:60) */
+    double laplacian[2]; /* second-order identity correction  [lower, upper] (:66) */
+    double robin[2][2];  /* (alpha, beta) at [lower, upper], copied from the dimension */
+} ibm_axis;
+
+typedef struct ibm_grid {
+    int dims;            /* number of dimensions (1..3)                        */
+    int refinement;      /* cells per base unit               (fd/grid.h:43-45) */
+    double unit;         /* base unit = floating GCD of the lengths (fd/domain.h:31) */
+    ibm_axis axis[3];
+} ibm_grid;
+
+/* ------------------------------------------------------------------ */
+/* Grids (host only, no device work)                                    */
+/* ------------------------------------------------------------------ */
+
+/* Base unit of a domain: GCD of the dimension lengths (fd/domain.h:30-32,
+ * algo/gcd.h:27-34). */
+ibm_status ibm_domain_unit(const ibm_dimension* dims, int ndims, double* unit);
+
+/* Grid with the given per-dimension alignments and refinement; the
+ * counterpart of fd::grid{domain, refinement, cell} (fd/grid.h:77-82). */
+ibm_status ibm_grid_make(const ibm_dimension* dims, int ndims, int refinement,
+                         const double* alignment, ibm_grid* out);
+
+/* The ndims component grids of a MAC (staggered) tag: component i is aligned
+ * on cell faces in dimension i and on cell centres elsewhere
+ * (fd/cell.h:106-114,194-196); fd::grid{fd::mac{refinement}, domain, dim_i}. */
+ibm_status ibm_mac_grids(const ibm_dimension* dims, int ndims, int refinement,
+                         ibm_grid* out /* [ndims] */);
+
+/* The cell-centred grid (pressure), fd::grid{fd::shift::diagonally(mac), domain}
+ * (test/ib-whole.cc:179). */
+ibm_status ibm_centered_grid(const ibm_dimension* dims, int ndims, int refinement,
+                             ibm_grid* out);
+
+/* Coarsened copy of a grid: refinement >> 1 (solvers/multigrid.h:189-190). */
+ibm_status ibm_grid_coarsen(const ibm_grid* fine, ibm_grid* coarse);
+
+/* Total number of grid points, fd::size(grid) (fd/size.h:33-41). */
+int64_t ibm_grid_size(const ibm_grid* g);
+
+/* ------------------------------------------------------------------ */
+/* Context                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct ibm_ctx ibm_ctx;
+
+/* Creates a context bound to a HIP device.  The context owns its scratch
+ * workspaces (grown on demand) and uses the null stream until
+ * ibm_set_stream() is called.  One context per host thread. */
+ibm_status ibm_create(ibm_ctx** ctx, int device);
+ibm_status ibm_destroy(ibm_ctx* ctx);
+/* stream is a hipStream_t passed as void* (0 = null stream). */
+ibm_status ibm_set_stream(ibm_ctx* ctx, void* stream);
+ibm_status ibm_synchronize(ibm_ctx* ctx);
+/* Text of the last error reported on ctx (never NULL). */
+const char* ibm_last_error(const ibm_ctx* ctx);
+
+/* Kernel timing (no counterpart in the reference, whose cuda::timer of
+ * include/cuda/timer.h:9-19 is never instantiated).  While enabled, the
+ * library records a hipEvent pair on the context stream around each named
+ * kernel launch ("spread_brick", "interp", "bin", ...).  Enabling clears the
+ * records.  ibm_profile_read synchronises the stream and returns the summed
+ * event time and launch count of one kernel name. */
+ibm_status ibm_profile_enable(ibm_ctx* ctx, int on);
+ibm_status ibm_profile_read(ibm_ctx* ctx, const char* name, double* total_ms, int* launches);
+
+/* ------------------------------------------------------------------ */
+/* Immersed-boundary coupling (FP64)                                    */
+/* ------------------------------------------------------------------ */
+
+/* Spreads Lagrangian forces onto the component grids:
+ *     fe_c(g) = h^-3 * sum_m f_c(m) * prod_j phi((g_j - X_j(m)) / h)
+ * Replaces ib::novel::spread<domain, delta>::operator()(n, x, fl, fe)
+ * (ib/novel.h:197-227; algorithm :58-168).  x and f are SoA n x 3.
+ * Grid points whose stencil index falls outside a solid dimension are
+ * dropped (ib/indexing.h:35-62).  Every point of fe[c] is written (no
+ * accumulation into existing contents).  Deterministic: the result does not
+ * depend on scheduling (no floating-point atomics in global memory). */
+ibm_status ibm_spread(ibm_ctx* ctx, const ibm_grid* grids /* [3] */, ibm_delta delta,
+                      int n, const double* x, const double* f,
+                      double* const* fe /* [3] device pointers */);
+
+/* Batched form: nsets independent marker sets (e.g. every RBC in one set and
+ * every platelet in another, each with its own marker count) spread in one
+ * pass into the same fe.  x[k], f[k] are SoA n[k] x 3. */
+ibm_status ibm_spread_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
+                           int nsets, const int* n, const double* const* x,
+                           const double* const* f, double* const* fe);
+
+/* Interpolates grid velocities to markers:
+ *     U_c(m) = sum_g u_c(g) * prod_j phi((g_j - X_j(m)) / h)
+ * Replaces ib::interpolate<domain, delta>::operator()(n, x, ue, ul)
+ * (ib/interpolate.h:92-115; kernel :49-76).  Stencil indices outside a
+ * solid dimension are clamped to the nearest grid point (ib/interpolate.h:18).
+ * ul is SoA n x 3. */
+ibm_status ibm_interpolate(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
+                           int n, const double* x, const double* const* ue,
+                           double* ul);
+
+ibm_status ibm_interpolate_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
+                                int nsets, const int* n, const double* const* x,
+                                const double* const* ue, double* const* ul);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* IBM_H */
