@@ -61,15 +61,15 @@ struct prof_scope {
 
 } // namespace ibm
 
-#define IBM_HIP(ctx, call)                                                  \
+#define IBM_HIP(ctx, ...)                                                   \
     do {                                                                    \
-        hipError_t e_ = (call);                                             \
-        if (e_ != hipSuccess) return ibm::hip_fail((ctx), e_, #call);       \
+        hipError_t e_ = (__VA_ARGS__);                                      \
+        if (e_ != hipSuccess) return ibm::hip_fail((ctx), e_, #__VA_ARGS__); \
     } while (0)
 
-#define IBM_TRY(call)                                                       \
+#define IBM_TRY(...)                                                        \
     do {                                                                    \
-        ibm_status s_ = (call);                                             \
+        ibm_status s_ = (__VA_ARGS__);                                      \
         if (s_ != IBM_OK) return s_;                                        \
     } while (0)
 

@@ -1,0 +1,43 @@
+// Host launchers of the stencil, transfer, vector and reduction kernels
+// (implemented in fd_ops.hip; all enqueue on ctx->stream).
+#pragma once
+#include "context.hpp"
+#include "fd_ops.hpp"
+
+namespace ibm {
+
+// out = alpha * A x + beta * y + gamma * z   (y, z may be null)
+template <typename T>
+ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                          const T* z, T* out);
+template <typename T>
+ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse);
+template <typename T>
+ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine);
+// y = a x + b y
+template <typename T, typename U>
+ibm_status launch_axpby(ibm_ctx* ctx, size_t n, T a, const U* x, T b, T* y);
+template <typename T, typename U>
+ibm_status launch_convert(ibm_ctx* ctx, size_t n, const U* x, T* y);
+template <typename T>
+ibm_status launch_fill(ibm_ctx* ctx, size_t n, T v, T* y);
+// *out = sum x_i y_i in FP64, deterministic order; `partial` holds REDUCE_PARTS doubles
+template <typename T, typename U>
+ibm_status launch_dot(ibm_ctx* ctx, size_t n, const T* x, const U* y, double* partial, double* out);
+template <typename T>
+ibm_status launch_sum(ibm_ctx* ctx, size_t n, const T* x, double* partial, double* out);
+// y -= mean(y)
+template <typename T>
+ibm_status launch_remove_mean(ibm_ctx* ctx, size_t n, T* y, double* partial, double* sum);
+
+template <typename T>
+ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const T* x, T beta, T* out);
+template <typename T>
+ibm_status launch_hadamard(ibm_ctx* ctx, size_t n, const T* a, const T* b, T* out);
+
+// The differential (kind 0) or average (kind 1) of grid `src` along `view`.
+ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst);
+
+constexpr int REDUCE_PARTS = 1024;
+
+} // namespace ibm

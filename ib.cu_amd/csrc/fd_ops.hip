@@ -1,0 +1,513 @@
+// Matrix-free staggered-grid stencils, multigrid transfers, vector kernels
+// and deterministic reductions (FP64 and FP32).
+//
+// Replaces the CSR sparse matrix-vector products the reference issues through
+// cusparseDcsrmv for every operator application (linalg/blas.h:186-199): the
+// operators are evaluated from their Kronecker structure (fd_ops.hpp), so a
+// stencil pass moves only the vectors (16 B per point in FP64, 8 in FP32,
+// +8/+4 per extra input vector) instead of ~100 B of CSR data per row.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <vector>
+#include "context.hpp"
+#include "fd_ops.hpp"
+#include "fd_kernels.hpp"
+
+namespace ibm {
+
+// ---------------------------------------------------------------------------
+// Stencil: out = alpha * A x + beta * y + gamma * z   (y, z optional)
+// Thread (i, j) marches a chunk of KZ planes in k with the k-neighbours in
+// registers; i is the fastest index so rows are read coalesced.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T lap1(const op_axis<T>& a, int i, T xm, T x0, T xp)
+{
+    if (a.periodic) return a.r2 * (xm + xp - 2 * x0);
+    T d = -2;
+    if (i == 0) d += a.lap_lo;
+    if (i == a.n - 1) d += a.lap_hi;
+    T l = i > 0 ? xm : T(0);
+    T r = i < a.n - 1 ? xp : T(0);
+    return a.r2 * (l + r + d * x0);
+}
+
+template <typename T>
+__device__ inline T id1(const op_axis<T>& a, int i)
+{
+    T d = 1;
+    if (i == 0) d += a.id_lo;
+    if (i == a.n - 1) d += a.id_hi;
+    return d;
+}
+
+constexpr int STX = 64, STY = 4, KZ = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(STX * STY)
+stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
+               T gamma, const T* __restrict__ z, T* __restrict__ out)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int i = blockIdx.x * STX + threadIdx.x;
+    const int j = blockIdx.y * STY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    if (i >= nx || j >= ny || k0 >= nz) return;
+    const int k1 = min(k0 + KZ, nz);
+    const size_t sxy = (size_t) nx * ny;
+    const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
+    const int ip = A.a[0].periodic ? (i == nx - 1 ? 0 : i + 1) : (i < nx - 1 ? i + 1 : i);
+    const int jm = A.a[1].periodic ? (j == 0 ? ny - 1 : j - 1) : (j > 0 ? j - 1 : j);
+    const int jp = A.a[1].periodic ? (j == ny - 1 ? 0 : j + 1) : (j < ny - 1 ? j + 1 : j);
+    const T Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
+    const size_t col = (size_t) i + (size_t) nx * j;
+    auto at = [&](int k) -> size_t { return col + sxy * (size_t) k; };
+    int km = k0 - 1;
+    if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
+    T xm = x[at(km)];
+    T x0 = x[at(k0)];
+    for (int k = k0; k < k1; ++k) {
+        int kp = k + 1;
+        if (kp >= nz) kp = A.a[2].periodic ? 0 : nz - 1;
+        T xp = x[at(kp)];
+        const size_t p = at(k);
+        const size_t row = (size_t) nx * j + sxy * (size_t) k;
+        T lx = lap1(A.a[0], i, x[row + im], x0, x[row + ip]);
+        T ly = lap1(A.a[1], j, x[(size_t) i + (size_t) nx * jm + sxy * k], x0, x[(size_t) i + (size_t) nx * jp + sxy * k]);
+        T lz = lap1(A.a[2], k, xm, x0, xp);
+        T Iz = id1(A.a[2], k);
+        T ax = A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
+        T v = alpha * ax;
+        if (y) v += beta * y[p];
+        if (z) v += gamma * z[p];
+        out[p] = v;
+        xm = x0;
+        x0 = xp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transfers.  Along one axis the fine->coarse weights are w[t]/2 at fine
+// index 2c + t - 1 and the coarse->fine weights w[j - 2c + 1]; an index one
+// past either end wraps (periodic) or folds onto the end point times the fine
+// grid's ghost coefficient (solid), as the reference's boundary entries do
+// (solvers/mg/interpolation.h:79-80,116-118).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline int fold(const transfer_axis<T>& a, int idx, int n, T& w)
+{
+    if (idx < 0) {
+        if (a.periodic) return idx + n;
+        w *= a.int_lo;
+        return 0;
+    }
+    if (idx >= n) {
+        if (a.periodic) return idx - n;
+        w *= a.int_hi;
+        return n - 1;
+    }
+    return idx;
+}
+
+template <typename T>
+__global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* __restrict__ coarse)
+{
+    const size_t total = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int c[3];
+    c[0] = (int) (t % R.a[0].nc);
+    c[1] = (int) ((t / R.a[0].nc) % R.a[1].nc);
+    c[2] = (int) (t / ((size_t) R.a[0].nc * R.a[1].nc));
+    int idx[3][4];
+    T wt[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            T w = R.a[d].w[s] / 2;
+            idx[d][s] = fold(R.a[d], 2 * c[d] + s - 1, R.a[d].nf, w);
+            wt[d][s] = w;
+        }
+    const size_t nfx = R.a[0].nf, nfxy = nfx * R.a[1].nf;
+    T acc = 0;
+#pragma unroll
+    for (int sz = 0; sz < 4; ++sz) {
+        T az = 0;
+#pragma unroll
+        for (int sy = 0; sy < 4; ++sy) {
+            const T* rowp = fine + nfxy * idx[2][sz] + nfx * idx[1][sy];
+            T ax = wt[0][0] * rowp[idx[0][0]] + wt[0][1] * rowp[idx[0][1]] +
+                   wt[0][2] * rowp[idx[0][2]] + wt[0][3] * rowp[idx[0][3]];
+            az += wt[1][sy] * ax;
+        }
+        acc += wt[2][sz] * az;
+    }
+    coarse[t] = acc;
+}
+
+template <typename T>
+__global__ void prolong_add_kernel(transfer3<T> P, const T* __restrict__ coarse, T* __restrict__ fine)
+{
+    const size_t total = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int f[3];
+    f[0] = (int) (t % P.a[0].nf);
+    f[1] = (int) ((t / P.a[0].nf) % P.a[1].nf);
+    f[2] = (int) (t / ((size_t) P.a[0].nf * P.a[1].nf));
+    int idx[3][2];
+    T wt[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // coarse cells c with t = f - 2c + 1 in [0, 3]
+        int j = f[d];
+        int ca = (j + 1) >> 1;          // t = j - 2ca + 1 in {0, 1}... its partner is ca - 1
+        int cb = ca - 1;
+        T wa = P.a[d].w[j - 2 * ca + 1];
+        T wb = P.a[d].w[j - 2 * cb + 1];
+        idx[d][0] = fold(P.a[d], ca, P.a[d].nc, wa);
+        idx[d][1] = fold(P.a[d], cb, P.a[d].nc, wb);
+        wt[d][0] = wa;
+        wt[d][1] = wb;
+    }
+    const size_t ncx = P.a[0].nc, ncxy = ncx * P.a[1].nc;
+    T acc = 0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        T az = 0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const T* rowp = coarse + ncxy * idx[2][b] + ncx * idx[1][a];
+            az += wt[1][a] * (wt[0][0] * rowp[idx[0][0]] + wt[0][1] * rowp[idx[0][1]]);
+        }
+        acc += wt[2][b] * az;
+    }
+    fine[t] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// Two-point operators (differential / average along one axis)
+// out = alpha * op x + beta * out
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__ x, T beta, T* __restrict__ out)
+{
+    const size_t total = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int c[3];
+    c[0] = (int) (t % P.dn[0]);
+    c[1] = (int) ((t / P.dn[0]) % P.dn[1]);
+    c[2] = (int) (t / ((size_t) P.dn[0] * P.dn[1]));
+    const int v = P.view;
+    const int cols = P.sn[v];
+    size_t stride = 1;
+    for (int d = 0; d < v; ++d) stride *= (size_t) P.sn[d];
+    int base[3] = {c[0], c[1], c[2]};
+    base[v] = 0;
+    const size_t b0 = (size_t) base[0] + (size_t) P.sn[0] * ((size_t) base[1] + (size_t) P.sn[1] * base[2]);
+    auto fetch = [&](int idx) -> T {
+        if (idx >= 0 && idx < cols) return x[b0 + stride * idx];
+        if (P.periodic) return x[b0 + stride * (idx < 0 ? idx + cols : idx - cols)];
+        return idx < 0 ? P.int_lo * x[b0] : P.int_hi * x[b0 + stride * (cols - 1)];
+    };
+    const int r = c[v];
+    T val = P.c0 * fetch(r - P.s1) + P.c1 * fetch(r + 1 - P.s1);
+    T o = alpha * val;
+    if (beta != T(0)) o += beta * out[t];
+    out[t] = o;
+}
+
+// out_c = a * in1_c * in2_c (elementwise product, products of averages in ins::advection)
+template <typename T>
+__global__ void hadamard_kernel(size_t n, const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = a[i] * b[i];
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels
+// ---------------------------------------------------------------------------
+template <typename T, typename U>
+__global__ void axpby_kernel(size_t n, T a, const U* __restrict__ x, T b, T* __restrict__ y)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] = a * (T) x[i] + (b == T(0) ? T(0) : b * y[i]);
+}
+
+template <typename T, typename U>
+__global__ void convert_kernel(size_t n, const U* __restrict__ x, T* __restrict__ y)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] = (T) x[i];
+}
+
+template <typename T>
+__global__ void fill_kernel(size_t n, T v, T* __restrict__ y)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] = v;
+}
+
+// Deterministic reductions: a fixed grid of REDUCE_BLOCKS blocks writes
+// partial sums in block order, one block adds them in index order.
+constexpr int REDUCE_BLOCKS = 1024;
+constexpr int REDUCE_THREADS = 256;
+
+__device__ inline double block_sum(double v)
+{
+    __shared__ double part[REDUCE_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < REDUCE_THREADS / 64; ++w) s += part[w];
+    __syncthreads();
+    return s;
+}
+
+template <typename T, typename U>
+__global__ void __launch_bounds__(REDUCE_THREADS)
+dot_partial_kernel(size_t n, const T* __restrict__ x, const U* __restrict__ y, double* __restrict__ partial)
+{
+    double s = 0;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        s += (double) x[i] * (double) y[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(REDUCE_THREADS)
+sum_partial_kernel(size_t n, const T* __restrict__ x, double* __restrict__ partial)
+{
+    double s = 0;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        s += (double) x[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(REDUCE_THREADS)
+finish_kernel(int nparts, const double* __restrict__ partial, double* __restrict__ out)
+{
+    double s = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T>
+__global__ void shift_kernel(size_t n, const double* __restrict__ sum, double scale, T* __restrict__ y)
+{
+    // y -= (sum * scale): removes the mean of a vector on a singular operator
+    T m = (T) (*sum * scale);
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] -= m;
+}
+
+inline int grid_for(size_t n, int threads = 256)
+{
+    size_t b = (n + threads - 1) / threads;
+    return (int) (b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+} // namespace ibm
+
+// ---------------------------------------------------------------------------
+// Host launchers (fd_kernels.hpp)
+// ---------------------------------------------------------------------------
+namespace ibm {
+
+template <typename T>
+ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                          const T* z, T* out)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
+    dim3 block(STX, STY);
+    dim3 grid((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
+    prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32");
+    stencil_kernel<T><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse)
+{
+    size_t n = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
+    prof_scope prof(ctx, "restrict");
+    restrict_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(R, fine, coarse);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine)
+{
+    size_t n = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    prof_scope prof(ctx, "prolong");
+    prolong_add_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, coarse, fine);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const T* x, T beta, T* out)
+{
+    size_t n = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
+    if (!n) return IBM_OK;
+    prof_scope prof(ctx, "twopoint");
+    twopoint_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, alpha, x, beta, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_hadamard(ibm_ctx* ctx, size_t n, const T* a, const T* b, T* out)
+{
+    hadamard_kernel<T><<<grid_for(n), 256, 0, ctx->stream>>>(n, a, b, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T, typename U>
+ibm_status launch_axpby(ibm_ctx* ctx, size_t n, T a, const U* x, T b, T* y)
+{
+    axpby_kernel<T, U><<<grid_for(n), 256, 0, ctx->stream>>>(n, a, x, b, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T, typename U>
+ibm_status launch_convert(ibm_ctx* ctx, size_t n, const U* x, T* y)
+{
+    convert_kernel<T, U><<<grid_for(n), 256, 0, ctx->stream>>>(n, x, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_fill(ibm_ctx* ctx, size_t n, T v, T* y)
+{
+    fill_kernel<T><<<grid_for(n), 256, 0, ctx->stream>>>(n, v, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T, typename U>
+ibm_status launch_dot(ibm_ctx* ctx, size_t n, const T* x, const U* y, double* partial, double* out)
+{
+    dot_partial_kernel<T, U><<<REDUCE_BLOCKS, REDUCE_THREADS, 0, ctx->stream>>>(n, x, y, partial);
+    IBM_LAUNCH_CHECK(ctx);
+    finish_kernel<<<1, REDUCE_THREADS, 0, ctx->stream>>>(REDUCE_BLOCKS, partial, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_sum(ibm_ctx* ctx, size_t n, const T* x, double* partial, double* out)
+{
+    sum_partial_kernel<T><<<REDUCE_BLOCKS, REDUCE_THREADS, 0, ctx->stream>>>(n, x, partial);
+    IBM_LAUNCH_CHECK(ctx);
+    finish_kernel<<<1, REDUCE_THREADS, 0, ctx->stream>>>(REDUCE_BLOCKS, partial, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_remove_mean(ibm_ctx* ctx, size_t n, T* y, double* partial, double* sum)
+{
+    IBM_TRY(launch_sum<T>(ctx, n, y, partial, sum));
+    shift_kernel<T><<<grid_for(n), 256, 0, ctx->stream>>>(n, sum, 1.0 / (double) n, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+#define IBM_INSTANTIATE(T)                                                                              \
+    template ibm_status launch_stencil<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, T*); \
+    template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
+    template ibm_status launch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);            \
+    template ibm_status launch_fill<T>(ibm_ctx*, size_t, T, T*);                                       \
+    template ibm_status launch_sum<T>(ibm_ctx*, size_t, const T*, double*, double*);                   \
+    template ibm_status launch_remove_mean<T>(ibm_ctx*, size_t, T*, double*, double*);                 \
+    template ibm_status launch_twopoint<T>(ibm_ctx*, const twopoint_op<T>&, T, const T*, T, T*);      \
+    template ibm_status launch_hadamard<T>(ibm_ctx*, size_t, const T*, const T*, T*);
+IBM_INSTANTIATE(double)
+IBM_INSTANTIATE(float)
+template ibm_status launch_axpby<double, double>(ibm_ctx*, size_t, double, const double*, double, double*);
+template ibm_status launch_axpby<double, float>(ibm_ctx*, size_t, double, const float*, double, double*);
+template ibm_status launch_axpby<float, float>(ibm_ctx*, size_t, float, const float*, float, float*);
+template ibm_status launch_convert<float, double>(ibm_ctx*, size_t, const double*, float*);
+template ibm_status launch_convert<double, float>(ibm_ctx*, size_t, const float*, double*);
+template ibm_status launch_convert<double, double>(ibm_ctx*, size_t, const double*, double*);
+template ibm_status launch_dot<double, double>(ibm_ctx*, size_t, const double*, const double*, double*, double*);
+template ibm_status launch_dot<float, float>(ibm_ctx*, size_t, const float*, const float*, double*, double*);
+
+ibm_status shifted_grid(const ibm_grid& src, int view, ibm_grid* dst)
+{
+    ibm_dimension dims[3];
+    double al[3];
+    for (int i = 0; i < src.dims; ++i) {
+        const ibm_axis& a = src.axis[i];
+        dims[i].length = a.length;
+        dims[i].solid = a.solid;
+        dims[i].reserved = 0;
+        dims[i].lower[0] = a.robin[0][0]; dims[i].lower[1] = a.robin[0][1];
+        dims[i].upper[0] = a.robin[1][0]; dims[i].upper[1] = a.robin[1][1];
+        al[i] = a.alignment;
+    }
+    // fd/differential.h:49-50: the new alignment is shift() + 1/2 (mod 1)
+    double sh = src.axis[view].shift + 0.5;
+    al[view] = sh - std::floor(sh);
+    return ibm_grid_make(dims, src.dims, src.refinement, al, dst);
+}
+
+// Gershgorin bounds over row classes (see fd_ops.hpp).
+void gershgorin_bounds(const ibm_grid& g, double sigma, double tau, double* lo, double* hi)
+{
+    struct cls { double ld, lo, id; };
+    std::vector<cls> c[3];
+    for (int d = 0; d < 3; ++d) {
+        const ibm_axis& a = g.axis[d];
+        double r2 = a.resolution * a.resolution;
+        int n = a.points;
+        if (d >= g.dims) { c[d].push_back({0, 0, 1}); continue; }
+        if (!a.solid) {
+            if (n == 1) c[d].push_back({0, 0, 1});
+            else c[d].push_back({-2 * r2, 2 * r2, 1});
+            continue;
+        }
+        if (n == 1) {
+            c[d].push_back({(-2 + a.interior[0] + a.interior[1]) * r2, 0, 1 + a.laplacian[0] + a.laplacian[1]});
+            continue;
+        }
+        c[d].push_back({(-2 + a.interior[0]) * r2, r2, 1 + a.laplacian[0]});
+        c[d].push_back({(-2 + a.interior[1]) * r2, r2, 1 + a.laplacian[1]});
+        if (n >= 3) c[d].push_back({-2 * r2, 2 * r2, 1});
+    }
+    double mn = INFINITY, mx = -INFINITY;
+    for (auto& x : c[0])
+        for (auto& y : c[1])
+            for (auto& z : c[2]) {
+                double center = sigma * x.id * y.id * z.id +
+                                tau * (x.ld * y.id * z.id + x.id * y.ld * z.id + x.id * y.id * z.ld);
+                double radius = std::fabs(tau) * (x.lo * std::fabs(y.id * z.id) + y.lo * std::fabs(x.id * z.id) +
+                                                  z.lo * std::fabs(x.id * y.id));
+                mn = std::fmin(mn, center - radius);
+                mx = std::fmax(mx, center + radius);
+            }
+    *lo = mn;
+    *hi = mx;
+}
+
+} // namespace ibm

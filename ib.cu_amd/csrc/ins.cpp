@@ -1,0 +1,169 @@
+// Navier-Stokes building blocks on the staggered grid: fd::differential /
+// fd::average along one axis, ins::divergence, ins::gradient and
+// ins::advection, all matrix-free (two-point kernels in fd_ops.hip).
+//
+// Restates:
+//   fd::differential, fd::average          fd/differential.h:44-103, fd/average.h:44-102
+//   fd::shift::directionally               fd/cell.h:96-104,165-184
+//   ins::divergence / ins::gradient        ins/differential.h:46-115
+//   ins::average / ins::advection          ins/advection.h:14-125
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include "context.hpp"
+#include "fd_kernels.hpp"
+
+namespace ibm {
+
+ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst)
+{
+    if (view < 0 || view >= src.dims) return IBM_ERR_ARG;
+    ibm_status st = shifted_grid(src, view, dst);
+    if (st != IBM_OK) return st;
+    const ibm_axis& a = src.axis[view];
+    for (int d = 0; d < 3; ++d) {
+        op->sn[d] = src.axis[d].points;
+        op->dn[d] = dst->axis[d].points;
+    }
+    op->view = view;
+    op->s1 = a.shift > dst->axis[view].shift ? 1 : 0;
+    op->periodic = !a.solid;
+    double r = a.resolution;
+    op->c0 = kind == 0 ? -r : 0.5;
+    op->c1 = kind == 0 ? r : 0.5;
+    op->int_lo = a.solid ? a.interior[0] : 0.0;
+    op->int_hi = a.solid ? a.interior[1] : 0.0;
+    return IBM_OK;
+}
+
+static bool same_points(const ibm_grid& a, const ibm_grid& b)
+{
+    for (int d = 0; d < 3; ++d)
+        if (a.axis[d].points != b.axis[d].points) return false;
+    return true;
+}
+
+static ibm_status check3(ibm_ctx* ctx, const ibm_grid* g, int n)
+{
+    if (!g) return fail(ctx, IBM_ERR_ARG, "null grid");
+    for (int i = 0; i < n; ++i)
+        if (g[i].dims != 3) return fail(ctx, IBM_ERR_ARG, "3-dimensional grids required");
+    return IBM_OK;
+}
+
+} // namespace ibm
+
+using namespace ibm;
+
+extern "C" {
+
+ibm_status ibm_grid_shift(const ibm_grid* src, int view, ibm_grid* dst)
+{
+    if (!src || !dst || view < 0 || view >= src->dims) return IBM_ERR_ARG;
+    return shifted_grid(*src, view, dst);
+}
+
+static ibm_status twopoint_apply(ibm_ctx* ctx, const ibm_grid* src, int view, int kind, double alpha,
+                                 const double* x, double beta, double* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, src, 1));
+    if (!x || !y) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (x == y) return fail(ctx, IBM_ERR_ARG, "x and y must not alias");
+    twopoint_op<double> op;
+    ibm_grid dst;
+    if (make_twopoint(*src, view, kind, &op, &dst) != IBM_OK) return fail(ctx, IBM_ERR_ARG, "bad view %d", view);
+    return launch_twopoint<double>(ctx, op, alpha, x, beta, y);
+}
+
+ibm_status ibm_differential(ibm_ctx* ctx, const ibm_grid* src, int view, double alpha, const double* x,
+                            double beta, double* y)
+{
+    return twopoint_apply(ctx, src, view, 0, alpha, x, beta, y);
+}
+
+ibm_status ibm_average(ibm_ctx* ctx, const ibm_grid* src, int view, double alpha, const double* x, double beta,
+                       double* y)
+{
+    return twopoint_apply(ctx, src, view, 1, alpha, x, beta, y);
+}
+
+// ins::divergence (ins/differential.h:46-83): out = sum_i D_i u_i on the
+// grid reached by shifting u_i's grid along axis i (the cell centres for MAC).
+ibm_status ibm_divergence(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double* out)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, u_grids, 3));
+    if (!u || !out) return fail(ctx, IBM_ERR_ARG, "null vector");
+    ibm_grid first;
+    for (int i = 0; i < 3; ++i) {
+        if (!u[i]) return fail(ctx, IBM_ERR_ARG, "null component");
+        twopoint_op<double> op;
+        ibm_grid dst;
+        IBM_TRY(make_twopoint(u_grids[i], i, 0, &op, &dst));
+        if (i == 0) first = dst;
+        else if (!same_points(first, dst)) return fail(ctx, IBM_ERR_SIZE, "divergence terms on different grids");
+        IBM_TRY(launch_twopoint<double>(ctx, op, 1.0, u[i], i ? 1.0 : 0.0, out));
+    }
+    return IBM_OK;
+}
+
+// ins::gradient (ins/differential.h:88-112): out_i = D_i p, on p's grid
+// shifted along axis i.
+ibm_status ibm_gradient(ibm_ctx* ctx, const ibm_grid* p_grid, double alpha, const double* p, double beta,
+                        double* const* out)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, p_grid, 1));
+    if (!p || !out) return fail(ctx, IBM_ERR_ARG, "null vector");
+    for (int i = 0; i < 3; ++i) {
+        if (!out[i]) return fail(ctx, IBM_ERR_ARG, "null component");
+        twopoint_op<double> op;
+        ibm_grid dst;
+        IBM_TRY(make_twopoint(*p_grid, i, 0, &op, &dst));
+        IBM_TRY(launch_twopoint<double>(ctx, op, alpha, p, beta, out[i]));
+    }
+    return IBM_OK;
+}
+
+// ins::advection (ins/advection.h:58-125):
+//   H^i = sum_j D_j [ (A_i u^j) o (A_j u^i) ],
+// A_i averaging along axis i, D_j the difference along j on the grid of
+// component j shifted along i (fd::shift::directionally).
+ibm_status ibm_advection(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double* const* H)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, u_grids, 3));
+    if (!u || !H) return fail(ctx, IBM_ERR_ARG, "null vector");
+    size_t maxn = 0;
+    ibm_grid ga[3][3];
+    twopoint_op<double> avg_ij[3][3];      // A_i applied to u^j
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            IBM_TRY(make_twopoint(u_grids[j], i, 1, &avg_ij[i][j], &ga[i][j]));
+            maxn = std::max(maxn, (size_t) ibm_grid_size(&ga[i][j]));
+        }
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_MG_BASE + 1, sizeof(double) * 3 * maxn, &p));
+    double* a = (double*) p;
+    double* b = a + maxn;
+    double* prod = b + maxn;
+    for (int i = 0; i < 3; ++i) {
+        if (!u[i] || !H[i]) return fail(ctx, IBM_ERR_ARG, "null component");
+        for (int j = 0; j < 3; ++j) {
+            const ibm_grid& g = ga[i][j];                        // grid of A_i u^j
+            if (!same_points(g, ga[j][i])) return fail(ctx, IBM_ERR_SIZE, "advection products on different grids");
+            size_t n = (size_t) ibm_grid_size(&g);
+            IBM_TRY(launch_twopoint<double>(ctx, avg_ij[i][j], 1.0, u[j], 0.0, a));
+            IBM_TRY(launch_twopoint<double>(ctx, avg_ij[j][i], 1.0, u[i], 0.0, b));
+            IBM_TRY(launch_hadamard<double>(ctx, n, a, b, prod));
+            twopoint_op<double> dj;
+            ibm_grid back;
+            IBM_TRY(make_twopoint(g, j, 0, &dj, &back));
+            if (!same_points(back, u_grids[i])) return fail(ctx, IBM_ERR_SIZE, "advection result off the component grid");
+            IBM_TRY(launch_twopoint<double>(ctx, dj, 1.0, prod, j ? 1.0 : 0.0, H[i]));
+        }
+    }
+    return IBM_OK;
+}
+
+} // extern "C"

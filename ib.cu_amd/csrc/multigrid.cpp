@@ -1,0 +1,472 @@
+// Chebyshev smoothing, geometric multigrid and Chebyshev-preconditioned CG on
+// the matrix-free operators A = sigma * I + tau * L.
+//
+// Restates:
+//   algo::chebyshev + chebw<4>            algo/chebyshev.h:120-169
+//   algo::gershgorin                      algo/gershgorin.h:14-50
+//   solvers::mg::chebyshev (interval)     solvers/mg/chebyshev.h:14-40
+//   solvers::multigrid (levels, V-cycle,  solvers/multigrid.h:88-195
+//     nested iteration, direct solver)
+//   solvers::pcg / chebpcg                solvers/pcg.h:11-44, solvers/chebpcg.h:9-23,
+//                                         solvers/preconditioners/chebyshev.h:9-26
+// Design choices (the reference does not pin them):
+//   * transfers act per physical axis (x fastest, like the grid vectors); the
+//     reference's kron(kron(Px, Py), Pz) applies Pz along the fastest index,
+//     which coincides with this only when the x and z axes share a staggering;
+//   * stopping rules are relative: ||r|| <= rtol ||b|| (the reference stops on
+//     absolute norms, solvers/multigrid.h:96, solvers/pcg.h:21);
+//   * mixed precision: FP32 V-cycles inside an FP64 defect correction
+//     x += V(b - A x), with the residual, its norm and the update in FP64.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <vector>
+#include "context.hpp"
+#include "fd_kernels.hpp"
+
+namespace ibm {
+
+// chebw<4> (algo/chebyshev.h:120-139): weights p_i(a, b) / q(a, b) with
+// polynomial(c...)(a, b) = sum_i c_i a^i b^(n-1-i).
+static double poly(const std::vector<double>& c, double a, double b)
+{
+    double v = 0;
+    int n = (int) c.size();
+    for (int i = 0; i < n; ++i) {
+        double ai = 1, bi = 1;
+        for (int k = 0; k < i; ++k) ai *= a;
+        for (int k = 0; k < n - 1 - i; ++k) bi *= b;
+        v += c[i] * ai * bi;
+    }
+    return v;
+}
+
+void chebyshev_weights(double a, double b, double w[4])
+{
+    double q = poly({1, 28, 70, 28, 1}, a, b);
+    w[0] = poly({128}, a, b) / q;
+    w[1] = poly({256, 256}, a, b) / q;
+    w[2] = poly({160, 448, 160}, a, b) / q;
+    w[3] = poly({32, 224, 224, 32}, a, b) / q;
+}
+
+// solvers/mg/chebyshev.h:14-37: smoothing interval [a + (b - a) * prop, b]
+// with prop = prod_d (min_length / 2) / length_d, after ordering |a| <= |b|.
+static void smoother_interval(const ibm_grid& g, double sigma, double tau, double* a, double* b)
+{
+    double lo, hi;
+    gershgorin_bounds(g, sigma, tau, &lo, &hi);
+    if (std::fabs(lo) > std::fabs(hi)) std::swap(lo, hi);
+    double smallest = INFINITY;
+    for (int d = 0; d < g.dims; ++d) smallest = std::fmin(smallest, g.axis[d].length);
+    smallest /= 2;
+    double prop = 1.0;
+    for (int d = g.dims - 1; d >= 0; --d) prop = (smallest / g.axis[d].length) * prop;
+    *a = lo + (hi - lo) * prop;
+    *b = hi;
+}
+
+// solvers/multigrid.h:153-172: an axis can be coarsened if it has an even
+// number of cells, more than 2 if periodic and more than 4 if walled.
+static bool refinable(const ibm_grid& g)
+{
+    for (int d = 0; d < g.dims; ++d) {
+        int c = g.axis[d].cells;
+        if (c & 1) return false;
+        if (!g.axis[d].solid ? c <= 2 : c <= 4) return false;
+    }
+    return g.refinement >= 2;
+}
+
+template <typename T>
+struct level_bufs {
+    T *x = nullptr, *b = nullptr, *r = nullptr, *s1 = nullptr, *s2 = nullptr;
+};
+
+struct mg_level {
+    ibm_grid g;
+    size_t n;
+    double w[4];
+    op3<double> A64;
+    op3<float> A32;
+    transfer3<double> T64;   // this level <-> next coarser
+    transfer3<float> T32;
+};
+
+} // namespace ibm
+
+struct ibm_mg {
+    int precision = 64;
+    double sigma = 0, tau = 0;
+    bool singular = false;
+    std::vector<ibm::mg_level> lv;
+    std::vector<ibm::level_bufs<double>> b64;
+    std::vector<ibm::level_bufs<float>> b32;
+    double* r = nullptr;        // outer FP64 residual
+    double* rhs = nullptr;      // projected right-hand side (singular operators)
+    double* partial = nullptr;  // reduction scratch
+    double* scalars = nullptr;  // device scalars
+    std::vector<void*> owned;
+};
+
+namespace ibm {
+
+template <typename T>
+static op3<T> level_op(const mg_level& l);
+template <>
+op3<double> level_op<double>(const mg_level& l) { return l.A64; }
+template <>
+op3<float> level_op<float>(const mg_level& l) { return l.A32; }
+template <typename T>
+static transfer3<T> level_tr(const mg_level& l);
+template <>
+transfer3<double> level_tr<double>(const mg_level& l) { return l.T64; }
+template <>
+transfer3<float> level_tr<float>(const mg_level& l) { return l.T32; }
+template <typename T>
+static std::vector<level_bufs<T>>& bufs(ibm_mg* m);
+template <>
+std::vector<level_bufs<double>>& bufs<double>(ibm_mg* m) { return m->b64; }
+template <>
+std::vector<level_bufs<float>>& bufs<float>(ibm_mg* m) { return m->b32; }
+
+// out = p(A) v, the degree-4 Chebyshev polynomial by Horner's rule
+// (algo/chebyshev.h:149-163): s0 = w0 v, s_k = w_k v - A s_{k-1}.
+// With add = true, out += p(A) v instead (out must not alias v, s1, s2).
+template <typename T>
+static ibm_status cheb_apply(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* s1, T* s2,
+                             T* out, bool add)
+{
+    IBM_TRY(launch_stencil<T>(ctx, A, (T) -w[0], v, 0, nullptr, (T) w[1], v, s1));
+    IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, s1, 0, nullptr, (T) w[2], v, s2));
+    if (add) return launch_stencil<T>(ctx, A, (T) -1, s2, (T) 1, out, (T) w[3], v, out);
+    return launch_stencil<T>(ctx, A, (T) -1, s2, 0, nullptr, (T) w[3], v, out);
+}
+
+// solvers/multigrid.h:120-128 (iterative levels) and :46-63 (coarsest):
+//   x = S b; r = b - A x; x += P vcycle(R r); x += S(b - A x)
+template <typename T>
+static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l)
+{
+    const mg_level& L = m->lv[l];
+    level_bufs<T>& B = bufs<T>(m)[l];
+    op3<T> A = level_op<T>(L);
+    IBM_TRY(cheb_apply<T>(ctx, A, L.w, B.b, B.s1, B.s2, B.x, false));
+    if (l + 1 == m->lv.size()) return IBM_OK;
+    level_bufs<T>& C = bufs<T>(m)[l + 1];
+    IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
+    IBM_TRY(launch_restrict<T>(ctx, level_tr<T>(L), B.r, C.b));
+    IBM_TRY(vcycle<T>(ctx, m, l + 1));
+    IBM_TRY(launch_prolong_add<T>(ctx, level_tr<T>(L), C.x, B.x));
+    IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
+    return cheb_apply<T>(ctx, A, L.w, B.r, B.s1, B.s2, B.x, true);
+}
+
+static ibm_status read_scalar(ibm_ctx* ctx, const double* d, double* h)
+{
+    IBM_HIP(ctx, hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IBM_OK;
+}
+
+static ibm_status norm2(ibm_ctx* ctx, ibm_mg* m, size_t n, const double* x, double* out)
+{
+    IBM_TRY(launch_dot<double, double>(ctx, n, x, x, m->partial, m->scalars));
+    double v;
+    IBM_TRY(read_scalar(ctx, m->scalars, &v));
+    *out = std::sqrt(v);
+    return IBM_OK;
+}
+
+template <typename T>
+static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, int maxit,
+                             int* iters, double* relres)
+{
+    const mg_level& L0 = m->lv[0];
+    const size_t n = L0.n;
+    level_bufs<T>& B0 = bufs<T>(m)[0];
+    if (m->singular) {
+        // Solutions of a singular (periodic / Neumann) Laplacian are defined
+        // up to a constant: solve with the mean-free part of b, return the
+        // mean-free solution.
+        IBM_TRY(launch_convert<double, double>(ctx, n, b, m->rhs));
+        IBM_TRY(launch_remove_mean<double>(ctx, n, m->rhs, m->partial, m->scalars + 1));
+        b = m->rhs;
+    }
+    double nb;
+    IBM_TRY(norm2(ctx, m, n, b, &nb));
+    IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
+    IBM_TRY(launch_convert<double, double>(ctx, n, b, m->r));
+    int it = 0;
+    double nr = nb;
+    while (nb > 0 && it < maxit) {
+        IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
+        IBM_TRY(vcycle<T>(ctx, m, 0));
+        IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
+        if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
+        IBM_TRY(launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
+        ++it;
+        IBM_TRY(norm2(ctx, m, n, m->r, &nr));
+        if (nr <= rtol * nb) break;
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = nb > 0 ? nr / nb : 0.0;
+    if (nb > 0 && nr > rtol * nb) return fail(ctx, IBM_ERR_NOT_CONVERGED, "multigrid: relative residual %.3e after %d cycles", nr / nb, it);
+    return IBM_OK;
+}
+
+template <typename T>
+static ibm_status alloc(ibm_ctx* ctx, ibm_mg* m, size_t n, T** p)
+{
+    void* v = nullptr;
+    IBM_HIP(ctx, hipMalloc(&v, sizeof(T) * (n ? n : 1)));
+    m->owned.push_back(v);
+    *p = (T*) v;
+    return IBM_OK;
+}
+
+static ibm_status check_grid(ibm_ctx* ctx, const ibm_grid* g)
+{
+    if (!g) return fail(ctx, IBM_ERR_ARG, "null grid");
+    if (g->dims != 3) return fail(ctx, IBM_ERR_ARG, "operators support 3-dimensional grids (got %d)", g->dims);
+    for (int d = 0; d < 3; ++d)
+        if (g->axis[d].points < 1) return fail(ctx, IBM_ERR_BAD_GRID, "empty axis");
+    return IBM_OK;
+}
+
+} // namespace ibm
+
+using namespace ibm;
+
+extern "C" {
+
+ibm_status ibm_gershgorin(const ibm_grid* g, double sigma, double tau, double* lo, double* hi)
+{
+    if (!g || !lo || !hi) return IBM_ERR_ARG;
+    gershgorin_bounds(*g, sigma, tau, lo, hi);
+    return IBM_OK;
+}
+
+ibm_status ibm_chebyshev_weights(double a, double b, double* w)
+{
+    if (!w) return IBM_ERR_ARG;
+    chebyshev_weights(a, b, w);
+    return IBM_OK;
+}
+
+ibm_status ibm_op_apply(ibm_ctx* ctx, const ibm_grid* g, double sigma, double tau, double alpha,
+                        const double* x, double beta, double* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check_grid(ctx, g));
+    if (!x || !y) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (x == y) return fail(ctx, IBM_ERR_ARG, "x and y must not alias");
+    return launch_stencil<double>(ctx, make_op<double>(*g, sigma, tau), alpha, x, beta, beta != 0 ? y : nullptr,
+                                  0.0, nullptr, y);
+}
+
+ibm_status ibm_op_apply_f32(ibm_ctx* ctx, const ibm_grid* g, double sigma, double tau, float alpha,
+                            const float* x, float beta, float* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check_grid(ctx, g));
+    if (!x || !y) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (x == y) return fail(ctx, IBM_ERR_ARG, "x and y must not alias");
+    return launch_stencil<float>(ctx, make_op<float>(*g, sigma, tau), alpha, x, beta, beta != 0 ? y : nullptr,
+                                 0.0f, nullptr, y);
+}
+
+ibm_status ibm_chebyshev_apply(ibm_ctx* ctx, const ibm_grid* g, double sigma, double tau, double a, double b,
+                               const double* r, double* out, double* work /* 2 * size */)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check_grid(ctx, g));
+    if (!r || !out || !work) return fail(ctx, IBM_ERR_ARG, "null vector");
+    double w[4];
+    chebyshev_weights(a, b, w);
+    size_t n = (size_t) ibm_grid_size(g);
+    return cheb_apply<double>(ctx, make_op<double>(*g, sigma, tau), w, r, work, work + n, out, false);
+}
+
+ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* g, double sigma, double tau, int precision, ibm_mg** out)
+{
+    if (!ctx || !out) return IBM_ERR_ARG;
+    *out = nullptr;
+    IBM_TRY(check_grid(ctx, g));
+    if (precision != 64 && precision != 32) return fail(ctx, IBM_ERR_ARG, "precision must be 64 or 32");
+    auto* m = new ibm_mg;
+    m->precision = precision;
+    m->sigma = sigma;
+    m->tau = tau;
+    bool singular = sigma == 0;
+    for (int d = 0; d < 3; ++d)
+        if (g->axis[d].solid && (g->axis[d].interior[0] != 1.0 || g->axis[d].interior[1] != 1.0)) singular = false;
+    m->singular = singular;
+    ibm_grid cur = *g;
+    ibm_status st = IBM_OK;
+    while (true) {
+        mg_level L;
+        L.g = cur;
+        L.n = (size_t) ibm_grid_size(&cur);
+        double a, b;
+        smoother_interval(cur, sigma, tau, &a, &b);
+        chebyshev_weights(a, b, L.w);
+        L.A64 = make_op<double>(cur, sigma, tau);
+        L.A32 = make_op<float>(cur, sigma, tau);
+        bool more = refinable(cur);
+        ibm_grid next;
+        if (more && ibm_grid_coarsen(&cur, &next) == IBM_OK) {
+            L.T64 = make_transfer<double>(cur, next);
+            L.T32 = make_transfer<float>(cur, next);
+        } else {
+            more = false;
+        }
+        m->lv.push_back(L);
+        if (!more) break;
+        cur = next;
+    }
+    for (auto& L : m->lv) {
+        if (precision == 64) {
+            level_bufs<double> B;
+            if ((st = alloc(ctx, m, L.n, &B.x)) || (st = alloc(ctx, m, L.n, &B.b)) || (st = alloc(ctx, m, L.n, &B.r)) ||
+                (st = alloc(ctx, m, L.n, &B.s1)) || (st = alloc(ctx, m, L.n, &B.s2)))
+                break;
+            m->b64.push_back(B);
+        } else {
+            level_bufs<float> B;
+            if ((st = alloc(ctx, m, L.n, &B.x)) || (st = alloc(ctx, m, L.n, &B.b)) || (st = alloc(ctx, m, L.n, &B.r)) ||
+                (st = alloc(ctx, m, L.n, &B.s1)) || (st = alloc(ctx, m, L.n, &B.s2)))
+                break;
+            m->b32.push_back(B);
+        }
+    }
+    if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->r);
+    if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->rhs);
+    if (st == IBM_OK) st = alloc(ctx, m, REDUCE_PARTS, &m->partial);
+    if (st == IBM_OK) st = alloc(ctx, m, 8, &m->scalars);
+    if (st != IBM_OK) {
+        for (void* p : m->owned) hipFree(p);
+        delete m;
+        return st;
+    }
+    *out = m;
+    return IBM_OK;
+}
+
+ibm_status ibm_mg_destroy(ibm_mg* m)
+{
+    if (!m) return IBM_OK;
+    hipDeviceSynchronize();
+    for (void* p : m->owned) hipFree(p);
+    delete m;
+    return IBM_OK;
+}
+
+ibm_status ibm_mg_levels(const ibm_mg* m, int* levels, ibm_grid* grids, int max_grids)
+{
+    if (!m || !levels) return IBM_ERR_ARG;
+    *levels = (int) m->lv.size();
+    for (int i = 0; grids && i < max_grids && i < (int) m->lv.size(); ++i) grids[i] = m->lv[i].g;
+    return IBM_OK;
+}
+
+ibm_status ibm_mg_vcycle(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x)
+{
+    if (!ctx || !m) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    size_t n = m->lv[0].n;
+    if (m->precision == 64) {
+        IBM_TRY(launch_convert<double, double>(ctx, n, b, m->b64[0].b));
+        IBM_TRY(vcycle<double>(ctx, m, 0));
+        return launch_convert<double, double>(ctx, n, m->b64[0].x, x);
+    }
+    IBM_TRY(launch_convert<float, double>(ctx, n, b, m->b32[0].b));
+    IBM_TRY(vcycle<float>(ctx, m, 0));
+    return launch_convert<double, float>(ctx, n, m->b32[0].x, x);
+}
+
+ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, int maxit, int* iters,
+                        double* relres)
+{
+    if (!ctx || !m) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
+    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, maxit, iters, relres);
+    return solve_loop<float>(ctx, m, b, x, rtol, maxit, iters, relres);
+}
+
+// solvers::chebpcg (solvers/chebpcg.h:9-23 -> solvers/pcg.h:11-44): CG
+// preconditioned by the Chebyshev polynomial over the Gershgorin interval of
+// A, stopping when ||r|| <= rtol ||b||.
+ibm_status ibm_chebpcg_solve(ibm_ctx* ctx, const ibm_grid* g, double sigma, double tau, const double* b, double* x,
+                             double rtol, int maxit, double* work /* 6 * size */, int* iters, double* relres)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check_grid(ctx, g));
+    if (!b || !x || !work) return fail(ctx, IBM_ERR_ARG, "null vector");
+    const size_t n = (size_t) ibm_grid_size(g);
+    op3<double> A = make_op<double>(*g, sigma, tau);
+    double lo, hi, w[4];
+    gershgorin_bounds(*g, sigma, tau, &lo, &hi);
+    chebyshev_weights(hi, lo, w);     // preconditioners/chebyshev.h:11-12 passes (b, a)
+    double* r = work;
+    double* d = work + n;
+    double* q = work + 2 * n;
+    double* s1 = work + 3 * n;
+    double* s2 = work + 4 * n;
+    double* red = work + 5 * n;       // REDUCE_PARTS + 4 doubles (n >= 1029 not assumed)
+    double* partial;
+    double* scal;
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_MG_BASE, sizeof(double) * (REDUCE_PARTS + 8), &p));
+    partial = (double*) p;
+    scal = partial + REDUCE_PARTS;
+    (void) red;
+    auto dot = [&](const double* u, const double* v, double* out) -> ibm_status {
+        IBM_TRY(launch_dot<double, double>(ctx, n, u, v, partial, scal));
+        return read_scalar(ctx, scal, out);
+    };
+    IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
+    IBM_TRY(launch_convert<double, double>(ctx, n, b, r));
+    double nb2;
+    IBM_TRY(dot(b, b, &nb2));
+    double nb = std::sqrt(nb2), nr = nb;
+    IBM_TRY(cheb_apply<double>(ctx, A, w, r, s1, s2, d, false));      // d = P r
+    double delta;
+    IBM_TRY(dot(r, d, &delta));
+    int it = 0;
+    while (nb > 0 && it < maxit && nr > rtol * nb) {
+        IBM_TRY(launch_stencil<double>(ctx, A, 1.0, d, 0.0, nullptr, 0.0, nullptr, q));   // q = A d
+        double nu;
+        IBM_TRY(dot(d, q, &nu));
+        if (std::fabs(nu) == 0) break;
+        double alpha = delta / nu;
+        IBM_TRY(launch_axpby<double, double>(ctx, n, alpha, d, 1.0, x));
+        IBM_TRY(launch_axpby<double, double>(ctx, n, -alpha, q, 1.0, r));
+        IBM_TRY(cheb_apply<double>(ctx, A, w, r, s1, s2, q, false));  // q = P r
+        double dold = delta;
+        IBM_TRY(dot(r, q, &delta));
+        double beta = delta / dold;
+        IBM_TRY(launch_axpby<double, double>(ctx, n, 1.0, q, beta, d)); // d = q + beta d
+        ++it;
+        double rr;
+        IBM_TRY(dot(r, r, &rr));
+        nr = std::sqrt(rr);
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = nb > 0 ? nr / nb : 0.0;
+    if (nb > 0 && nr > rtol * nb) return fail(ctx, IBM_ERR_NOT_CONVERGED, "chebpcg: relative residual %.3e after %d iterations", nr / nb, it);
+    return IBM_OK;
+}
+
+ibm_status ibm_dot(ibm_ctx* ctx, int64_t n, const double* x, const double* y, double* out)
+{
+    if (!ctx || !out) return IBM_ERR_ARG;
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(ctx, IBM_ERR_ARG, "bad vectors");
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_MG_BASE, sizeof(double) * (REDUCE_PARTS + 8), &p));
+    double* partial = (double*) p;
+    IBM_TRY(launch_dot<double, double>(ctx, (size_t) n, x, y, partial, partial + REDUCE_PARTS));
+    return read_scalar(ctx, partial + REDUCE_PARTS, out);
+}
+
+} // extern "C"

@@ -296,7 +296,12 @@ constexpr int BRICK_PTS = BX * BY * BZ;
 constexpr int WAVES = SPREAD_THREADS / 64;
 constexpr int MAX_ROWS = (BY + 4) * (BZ + 4);
 constexpr int MAX_SEG2 = 2 * MAX_ROWS;
-constexpr int WROW = CHUNK + 1;      // padded row: rows of one wavefront's weights start in different banks
+// Padded weight row: 2*WROW dwords is 2 mod 4, so any 16 consecutive rows
+// start in distinct LDS banks, and the 8 extra entries absorb the reads of a
+// final partial batch.
+constexpr int WROW = CHUNK + 9;
+constexpr int VISIT_BATCH = 8;
+constexpr int POS_LIST = 2048;      // region markers listed in LDS; larger regions search the segments
 // FAST: every periodic axis has at least BX + 8 cells, so a stencil that
 // reaches the brick has a unique brick-relative origin in [-3, B).
 constexpr int MIN_FAST_CELLS = BX + 8;
@@ -338,9 +343,10 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
 {
     __shared__ unsigned long long acc[3][BRICK_PTS];
     __shared__ double W[WAVES][20][WROW];     // rows 0-3: wx[sx]; rows 4-19: wz[a] * wy[b] * f at 4 + 4a + b
-    __shared__ int org[WAVES][CHUNK];
+    __shared__ int org[WAVES][CHUNK + VISIT_BATCH];
     __shared__ int seg_begin[MAX_SEG2];
     __shared__ int seg_pref[MAX_SEG2 + 1];
+    __shared__ int pos_list[POS_LIST];
     __shared__ double bound_s;
 
     const int tid = threadIdx.x;
@@ -406,6 +412,14 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
     }
     __syncthreads();
     const int T = seg_pref[nseg];
+    // Flattened sorted positions of the region's markers (when they fit).
+    const bool listed = T <= POS_LIST;
+    if (listed)
+        for (int i = tid; i < nseg; i += SPREAD_THREADS) {
+            int b = seg_begin[i], o = seg_pref[i], n = seg_pref[i + 1] - o;
+            for (int k = 0; k < n; ++k) pos_list[o + k] = b + k;
+        }
+    __syncthreads();
     int e2 = 0;
     if (bound_s > 0) frexp(bound_s, &e2);
     const double inv_q = ldexp(1.0, 62 - e2);
@@ -417,6 +431,7 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
 
     // Sorted position of region marker idx (binary search over the segments).
     auto locate = [&](int idx) {
+        if (listed) return pos_list[idx];
         int lo = 0, hi = nseg;
         while (hi - lo > 1) {
             int mid = (lo + hi) >> 1;
@@ -473,26 +488,30 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            int bb[3];
+            const bool s0 = MAC ? (c == 0) : (g.sel[c][0] != 0);
+            const bool s1 = MAC ? (c == 1) : (g.sel[c][1] != 0);
+            const bool s2 = MAC ? (c == 2) : (g.sel[c][2] != 0);
+            const int bb0 = s0 ? b1[0] : b0[0];
+            const int bb1 = s1 ? b1[1] : b0[1];
+            const int bb2 = s2 ? b1[2] : b0[2];
             double w[3][4];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                bool one = MAC ? (c == j) : (g.sel[c][j] != 0);
-                bb[j] = one ? b1[j] : b0[j];
-#pragma unroll
-                for (int o = 0; o < 4; ++o) w[j][o] = one ? w1[j][o] : w0[j][o];
+            for (int o = 0; o < 4; ++o) {
+                w[0][o] = s0 ? w1[0][o] : w0[0][o];
+                w[1][o] = s1 ? w1[1][o] : w0[1][o];
+                w[2][o] = s2 ? w1[2][o] : w0[2][o];
             }
             // Does the stencil meet the brick?  Brick-relative origin of stencil point 0.
             int ow0, ow1, ow2;
-            bool hit = valid && axis_hit<FAST, BX>(g.a[c][0], bb[0], P0x, ow0)
-                             && axis_hit<FAST, BY>(g.a[c][1], bb[1], P0y, ow1)
-                             && axis_hit<FAST, BZ>(g.a[c][2], bb[2], P0z, ow2);
+            bool hit = valid && axis_hit<FAST, BX>(g.a[c][0], bb0, P0x, ow0)
+                             && axis_hit<FAST, BY>(g.a[c][1], bb1, P0y, ow1)
+                             && axis_hit<FAST, BZ>(g.a[c][2], bb2, P0z, ow2);
             const unsigned long long hits = __ballot(hit);
             const int nh = __popcll(hits);
             unsigned long long below = hits & lanemask_lt;
             int prev = below ? 63 - __clzll((long long) below) : lane;
-            int pb0 = __shfl(bb[0], prev, 64), pb1 = __shfl(bb[1], prev, 64), pb2 = __shfl(bb[2], prev, 64);
-            bool head = hit && (!below || pb0 != bb[0] || pb1 != bb[1] || pb2 != bb[2]);
+            int pb0 = __shfl(bb0, prev, 64), pb1 = __shfl(bb1, prev, 64), pb2 = __shfl(bb2, prev, 64);
+            bool head = hit && (!below || pb0 != bb0 || pb1 != bb1 || pb2 != bb2);
             const int slot = __popcll(below);
             // Head flags in compacted order: OR-reduce (head << slot).
             unsigned long long hm = head ? (1ull << slot) : 0ull;
@@ -523,7 +542,7 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
                     int flat = ow0 + BX * (ow1 + BY * ow2) + ORG_BIAS;
                     org[wave][slot] = flat | (mx << 12) | (my << 16) | (mz << 20);
                 } else {
-                    org[wave][slot] = (bb[0] + 1) | ((bb[1] + 1) << 10) | ((bb[2] + 1) << 20);
+                    org[wave][slot] = (bb0 + 1) | ((bb1 + 1) << 10) | ((bb2 + 1) << 20);
                 }
             }
             const unsigned long long headmask =
@@ -539,26 +558,37 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
             const int lane_off = sx + BX * (sy + BY * sz) - ORG_BIAS;
             double P = 0.0;
             int target = -1;
-#pragma unroll 8
-            for (int k = 0; k < nh; ++k) {
-                double a = wa[k];
-                double b = wb[k];
-                if ((headmask >> k) & 1) {
-                    if (target >= 0)
-                        atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
-                    P = 0.0;
-                    int o = org[wave][k];
-                    if (FAST) {
-                        int ok = (o >> (12 + sx)) & (o >> (16 + sy)) & (o >> (20 + sz)) & 1;
-                        target = ok ? (o & 4095) + lane_off : -1;
-                    } else {
-                        int ox = brick_offset<BX>(ax, (o & 1023) - 1, sx, P0x);
-                        int oy = brick_offset<BY>(ay, ((o >> 10) & 1023) - 1, sy, P0y);
-                        int oz = brick_offset<BZ>(az, ((o >> 20) & 1023) - 1, sz, P0z);
-                        target = (ox >= 0 && oy >= 0 && oz >= 0) ? ox + BX * (oy + BY * oz) : -1;
+            for (int k0 = 0; k0 < nh; k0 += VISIT_BATCH) {
+                // all loads of the batch first (entries past nh are never used)
+                double av[VISIT_BATCH], bv[VISIT_BATCH];
+                int ov[VISIT_BATCH];
+#pragma unroll
+                for (int i = 0; i < VISIT_BATCH; ++i) {
+                    av[i] = wa[k0 + i];
+                    bv[i] = wb[k0 + i];
+                    ov[i] = org[wave][k0 + i];
+                }
+#pragma unroll
+                for (int i = 0; i < VISIT_BATCH; ++i) {
+                    if (k0 + i < nh) {
+                        if ((headmask >> (k0 + i)) & 1) {
+                            if (target >= 0)
+                                atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
+                            P = 0.0;
+                            int o = ov[i];
+                            if (FAST) {
+                                int ok = (o >> (12 + sx)) & (o >> (16 + sy)) & (o >> (20 + sz)) & 1;
+                                target = ok ? (o & 4095) + lane_off : -1;
+                            } else {
+                                int ox = brick_offset<BX>(ax, (o & 1023) - 1, sx, P0x);
+                                int oy = brick_offset<BY>(ay, ((o >> 10) & 1023) - 1, sy, P0y);
+                                int oz = brick_offset<BZ>(az, ((o >> 20) & 1023) - 1, sz, P0z);
+                                target = (ox >= 0 && oy >= 0 && oz >= 0) ? ox + BX * (oy + BY * oz) : -1;
+                            }
+                        }
+                        P = fma(av[i], bv[i], P);
                     }
                 }
-                P = fma(a, b, P);
             }
             if (target >= 0)
                 atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));

@@ -103,6 +103,25 @@ _SIGS = {
     "ibm_interpolate": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, ctypes.POINTER(_VP), _VP]),
     "ibm_interpolate_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                                   ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ibm_op_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, _D, _VP, _D, _VP]),
+    "ibm_op_apply_f32": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, ctypes.c_float, _VP, ctypes.c_float, _VP]),
+    "ibm_gershgorin": (_I, [ctypes.POINTER(ibm_grid), _D, _D, _PD, _PD]),
+    "ibm_chebyshev_weights": (_I, [_D, _D, _PD]),
+    "ibm_chebyshev_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, _D, _D, _VP, _VP, _VP]),
+    "ibm_dot": (_I, [_VP, ctypes.c_int64, _VP, _VP, _PD]),
+    "ibm_mg_create": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, _I, ctypes.POINTER(_VP)]),
+    "ibm_mg_destroy": (_I, [_VP]),
+    "ibm_mg_levels": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(ibm_grid), _I]),
+    "ibm_mg_vcycle": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_mg_solve": (_I, [_VP, _VP, _VP, _VP, _D, _I, ctypes.POINTER(_I), _PD]),
+    "ibm_chebpcg_solve": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, _VP, _VP, _D, _I, _VP,
+                               ctypes.POINTER(_I), _PD]),
+    "ibm_grid_shift": (_I, [ctypes.POINTER(ibm_grid), _I, ctypes.POINTER(ibm_grid)]),
+    "ibm_differential": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _D, _VP, _D, _VP]),
+    "ibm_average": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _D, _VP, _D, _VP]),
+    "ibm_divergence": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP), _VP]),
+    "ibm_gradient": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _VP, _D, ctypes.POINTER(_VP)]),
+    "ibm_advection": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
 }
 
 
@@ -318,3 +337,156 @@ def interpolate_sets(ctx, grids, delta, xs, ue, outs=None):
                                       (ctypes.c_int * len(ns))(*ns), _ptrs(xs), _ptrs(ue), _ptrs(outs))
     _check(code, ctx)
     return outs
+
+
+# ----------------------------------------------------------------------
+# Operators and solvers (fd::laplacian, fd::identity, ins::diffusion's
+# Helmholtz operator, algo::chebyshev, solvers::multigrid, solvers::chebpcg)
+# ----------------------------------------------------------------------
+LAPLACIAN = (0.0, 1.0)
+IDENTITY = (1.0, 0.0)
+
+
+def helmholtz(lam):
+    """I2 - lambda L (ins/diffusion.h:21-35)."""
+    return (1.0, -lam)
+
+
+def op_apply(ctx, grid, op, x, alpha=1.0, beta=0.0, y=None):
+    import torch
+    if y is None:
+        y = torch.zeros_like(x)
+    sigma, tau = op
+    fn = lib().ibm_op_apply if x.dtype == torch.float64 else lib().ibm_op_apply_f32
+    _check(fn(ctx.handle, ctypes.byref(grid), sigma, tau, alpha, _ptr(x), beta, _ptr(y)), ctx)
+    return y
+
+
+def gershgorin(grid, op):
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    _check(lib().ibm_gershgorin(ctypes.byref(grid), op[0], op[1], ctypes.byref(lo), ctypes.byref(hi)))
+    return lo.value, hi.value
+
+
+def chebyshev_weights(a, b):
+    w = (ctypes.c_double * 4)()
+    _check(lib().ibm_chebyshev_weights(a, b, w))
+    return list(w)
+
+
+def chebyshev_apply(ctx, grid, op, a, b, r):
+    import torch
+    out = torch.empty_like(r)
+    work = torch.empty(2 * r.numel(), dtype=torch.float64, device=r.device)
+    _check(lib().ibm_chebyshev_apply(ctx.handle, ctypes.byref(grid), op[0], op[1], a, b, _ptr(r), _ptr(out),
+                                     _ptr(work)), ctx)
+    return out
+
+
+def dot(ctx, x, y):
+    out = ctypes.c_double()
+    _check(lib().ibm_dot(ctx.handle, x.numel(), _ptr(x), _ptr(y), ctypes.byref(out)), ctx)
+    return out.value
+
+
+class Multigrid:
+    """solvers::multigrid(grid, tol, op, smoother) over A = sigma*I2 + tau*L."""
+
+    def __init__(self, ctx, grid, op, precision=64):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().ibm_mg_create(ctx.handle, ctypes.byref(grid), op[0], op[1], precision, ctypes.byref(h)), ctx)
+        self.handle = h
+
+    def levels(self):
+        n = ctypes.c_int()
+        grids = (ibm_grid * 32)()
+        _check(lib().ibm_mg_levels(self.handle, ctypes.byref(n), grids, 32))
+        return [grids[i] for i in range(n.value)]
+
+    def vcycle(self, b):
+        import torch
+        x = torch.empty_like(b)
+        _check(lib().ibm_mg_vcycle(self.ctx.handle, self.handle, _ptr(b), _ptr(x)), self.ctx)
+        return x
+
+    def solve(self, b, rtol=1e-10, maxit=100, x=None):
+        import torch
+        if x is None:
+            x = torch.empty_like(b)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        _check(lib().ibm_mg_solve(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, maxit,
+                                  ctypes.byref(it), ctypes.byref(rr)), self.ctx)
+        return x, it.value, rr.value
+
+    def close(self):
+        if self.handle:
+            lib().ibm_mg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def chebpcg_solve(ctx, grid, op, b, rtol=1e-10, maxit=1000):
+    import torch
+    x = torch.empty_like(b)
+    work = torch.empty(6 * b.numel(), dtype=torch.float64, device=b.device)
+    it = ctypes.c_int()
+    rr = ctypes.c_double()
+    _check(lib().ibm_chebpcg_solve(ctx.handle, ctypes.byref(grid), op[0], op[1], _ptr(b), _ptr(x), rtol, maxit,
+                                   _ptr(work), ctypes.byref(it), ctypes.byref(rr)), ctx)
+    return x, it.value, rr.value
+
+
+# ----------------------------------------------------------------------
+# ins::divergence / ins::gradient / ins::advection, fd::differential / average
+# ----------------------------------------------------------------------
+def grid_shift(grid, view):
+    out = ibm_grid()
+    _check(lib().ibm_grid_shift(ctypes.byref(grid), view, ctypes.byref(out)))
+    return out
+
+
+def differential(ctx, grid, view, x, alpha=1.0, beta=0.0, y=None):
+    import torch
+    if y is None:
+        y = torch.zeros(grid_shift(grid, view).size, dtype=torch.float64, device=x.device)
+    _check(lib().ibm_differential(ctx.handle, ctypes.byref(grid), view, alpha, _ptr(x), beta, _ptr(y)), ctx)
+    return y
+
+
+def average(ctx, grid, view, x, alpha=1.0, beta=0.0, y=None):
+    import torch
+    if y is None:
+        y = torch.zeros(grid_shift(grid, view).size, dtype=torch.float64, device=x.device)
+    _check(lib().ibm_average(ctx.handle, ctypes.byref(grid), view, alpha, _ptr(x), beta, _ptr(y)), ctx)
+    return y
+
+
+def divergence(ctx, grids, u, out=None):
+    import torch
+    if out is None:
+        out = torch.empty(grid_shift(grids[0], 0).size, dtype=torch.float64, device=u[0].device)
+    _check(lib().ibm_divergence(ctx.handle, grids, _ptrs(u), _ptr(out)), ctx)
+    return out
+
+
+def gradient(ctx, pgrid, p, out=None, alpha=1.0, beta=0.0):
+    import torch
+    if out is None:
+        out = [torch.zeros(grid_shift(pgrid, i).size, dtype=torch.float64, device=p.device) for i in range(3)]
+    _check(lib().ibm_gradient(ctx.handle, ctypes.byref(pgrid), alpha, _ptr(p), beta, _ptrs(out)), ctx)
+    return out
+
+
+def advection(ctx, grids, u, out=None):
+    import torch
+    if out is None:
+        out = [torch.empty(grids[i].size, dtype=torch.float64, device=u[0].device) for i in range(3)]
+    _check(lib().ibm_advection(ctx.handle, grids, _ptrs(u), _ptrs(out)), ctx)
+    return out

@@ -177,6 +177,102 @@ ibm_status ibm_interpolate_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta d
                                 int nsets, const int* n, const double* const* x,
                                 const double* const* ue, double* const* ul);
 
+
+/* ------------------------------------------------------------------ */
+/* Staggered-grid operators (matrix-free)                               */
+/*                                                                      */
+/* A = sigma * I2 + tau * L on one grid, where L is fd::laplacian(grid) */
+/* (fd/laplacian.h:53-68) and I2 is fd::identity(grid, second_order)    */
+/* (fd/identity.h:55-66), both as the reference assembles them with     */
+/* Kronecker sums/products, x fastest (fd/combine.h:35-49):             */
+/*   Laplacian: (0, 1); corrected identity: (1, 0);                     */
+/*   Helmholtz I2 - lambda L of ins::diffusion (ins/diffusion.h:21-35): */
+/*   (1, -lambda).                                                      */
+/* ------------------------------------------------------------------ */
+
+/* y = alpha * A x + beta * y.  Replaces gemv(alpha, A, x, beta, y) on the
+ * assembled CSR operator (linalg/blas.h:186-199, cusparseDcsrmv). */
+ibm_status ibm_op_apply(ibm_ctx* ctx, const ibm_grid* grid, double sigma, double tau, double alpha,
+                        const double* x, double beta, double* y);
+ibm_status ibm_op_apply_f32(ibm_ctx* ctx, const ibm_grid* grid, double sigma, double tau, float alpha,
+                            const float* x, float beta, float* y);
+
+/* Gershgorin interval of A (algo/gershgorin.h:14-50); host only. */
+ibm_status ibm_gershgorin(const ibm_grid* grid, double sigma, double tau, double* lo, double* hi);
+
+/* The four Horner weights of the degree-4 Chebyshev polynomial over [a, b]
+ * (algo/chebyshev.h:120-139, chebw<4>); host only. */
+ibm_status ibm_chebyshev_weights(double a, double b, double* w /* [4] */);
+
+/* out = p(A) r, algo::chebyshev(a, b, A)(r) (algo/chebyshev.h:145-169).
+ * work holds 2 * size(grid) doubles. */
+ibm_status ibm_chebyshev_apply(ibm_ctx* ctx, const ibm_grid* grid, double sigma, double tau, double a, double b,
+                               const double* r, double* out, double* work);
+
+/* <x, y> in FP64 with a fixed summation order; returns the host scalar
+ * (linalg::dot, linalg/blas.h:140-155). */
+ibm_status ibm_dot(ibm_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
+
+/* ------------------------------------------------------------------ */
+/* Solvers                                                              */
+/* ------------------------------------------------------------------ */
+
+/* Geometric multigrid for A on `grid`, solvers::multigrid (solvers/multigrid.h:
+ * 176-212): levels by halving the refinement while every axis has an even
+ * number of cells (> 2 periodic, > 4 walled), operators re-discretised on
+ * each level, degree-4 Chebyshev smoothing over the upper part of the
+ * Gershgorin interval (solvers/mg/chebyshev.h:12-51), 4-point transfers
+ * (solvers/mg/interpolation.h:19-152), smoother-only coarsest level.
+ * precision = 64: FP64 throughout; precision = 32: FP32 V-cycles inside an
+ * FP64 defect correction.  A singular operator (sigma = 0 with only periodic
+ * and Neumann axes) is solved on mean-free vectors. */
+typedef struct ibm_mg ibm_mg;
+ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* grid, double sigma, double tau, int precision,
+                         ibm_mg** mg);
+ibm_status ibm_mg_destroy(ibm_mg* mg);
+/* Number of levels and (optionally) their grids, finest first. */
+ibm_status ibm_mg_levels(const ibm_mg* mg, int* levels, ibm_grid* grids, int max_grids);
+/* x = V(b): one V-cycle from a zero guess (solvers/multigrid.h:120-128). */
+ibm_status ibm_mg_vcycle(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x);
+/* Solves A x = b: x += V(b - A x) until ||b - A x|| <= rtol ||b|| (FP64
+ * norms; the reference's loop at solvers/multigrid.h:88-104 with a relative
+ * stopping rule).  Returns IBM_ERR_NOT_CONVERGED after maxit cycles. */
+ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, double rtol, int maxit,
+                        int* iters, double* relres);
+
+/* Chebyshev-preconditioned CG, solvers::chebpcg (solvers/chebpcg.h:9-23,
+ * solvers/pcg.h:11-44): the Helmholtz solver of ins::diffusion.  Stops when
+ * ||r|| <= rtol ||b||.  work holds 6 * size(grid) doubles. */
+ibm_status ibm_chebpcg_solve(ibm_ctx* ctx, const ibm_grid* grid, double sigma, double tau, const double* b,
+                             double* x, double rtol, int maxit, double* work, int* iters, double* relres);
+
+/* ------------------------------------------------------------------ */
+/* Navier-Stokes building blocks                                        */
+/* ------------------------------------------------------------------ */
+
+/* The grid of `src` with axis `view` re-aligned by 1/2: the destination of a
+ * difference or average along `view` (fd::shift::directionally,
+ * fd/cell.h:165-184; fd/differential.h:49-52).  Host only. */
+ibm_status ibm_grid_shift(const ibm_grid* src, int view, ibm_grid* dst);
+
+/* y = alpha * D x + beta * y with D = fd::differential(src, view)
+ * (fd/differential.h:86-103), y on ibm_grid_shift(src, view). */
+ibm_status ibm_differential(ibm_ctx* ctx, const ibm_grid* src, int view, double alpha, const double* x,
+                            double beta, double* y);
+/* y = alpha * A x + beta * y with A = fd::average(src, view) (fd/average.h:85-102). */
+ibm_status ibm_average(ibm_ctx* ctx, const ibm_grid* src, int view, double alpha, const double* x, double beta,
+                       double* y);
+
+/* out = sum_i D_i u_i, ins::divergence (ins/differential.h:46-83). */
+ibm_status ibm_divergence(ibm_ctx* ctx, const ibm_grid* u_grids /* [3] */, const double* const* u,
+                          double* out);
+/* out_i = alpha * D_i p + beta * out_i, ins::gradient (ins/differential.h:88-112). */
+ibm_status ibm_gradient(ibm_ctx* ctx, const ibm_grid* p_grid, double alpha, const double* p, double beta,
+                        double* const* out /* [3] */);
+/* H^i = sum_j D_j[(A_i u^j)(A_j u^i)], ins::advection (ins/advection.h:58-125). */
+ibm_status ibm_advection(ibm_ctx* ctx, const ibm_grid* u_grids /* [3] */, const double* const* u,
+                         double* const* H /* [3] */);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
