@@ -41,7 +41,10 @@ enum scratch_slot {
     SLOT_SORTED,
     SLOT_SCAN_TMP,
     SLOT_FLAGS,
-    SLOT_MG_BASE,       // multigrid hierarchies own their memory; slots above
+    SLOT_REDUCE,        // reduction partials of the solvers (REDUCE_PARTS + 8 doubles)
+    SLOT_INS_TEMP,      // advection temporaries
+    SLOT_RBF_X,         // packed data-site coordinates of an RBF evaluation
+    SLOT_STEP_BASE,     // timestep workspaces; slots above are the driver's
     SLOT_COUNT = 64
 };
 

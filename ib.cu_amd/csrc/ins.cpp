@@ -143,7 +143,7 @@ ibm_status ibm_advection(ibm_ctx* ctx, const ibm_grid* u_grids, const double* co
             maxn = std::max(maxn, (size_t) ibm_grid_size(&ga[i][j]));
         }
     void* p;
-    IBM_TRY(scratch(ctx, SLOT_MG_BASE + 1, sizeof(double) * 3 * maxn, &p));
+    IBM_TRY(scratch(ctx, SLOT_INS_TEMP, sizeof(double) * 3 * maxn, &p));
     double* a = (double*) p;
     double* b = a + maxn;
     double* prod = b + maxn;

@@ -417,7 +417,7 @@ ibm_status ibm_chebpcg_solve(ibm_ctx* ctx, const ibm_grid* g, double sigma, doub
     double* partial;
     double* scal;
     void* p;
-    IBM_TRY(scratch(ctx, SLOT_MG_BASE, sizeof(double) * (REDUCE_PARTS + 8), &p));
+    IBM_TRY(scratch(ctx, SLOT_REDUCE, sizeof(double) * (REDUCE_PARTS + 8), &p));
     partial = (double*) p;
     scal = partial + REDUCE_PARTS;
     (void) red;
@@ -463,7 +463,7 @@ ibm_status ibm_dot(ibm_ctx* ctx, int64_t n, const double* x, const double* y, do
     if (!ctx || !out) return IBM_ERR_ARG;
     if (n < 0 || (n > 0 && (!x || !y))) return fail(ctx, IBM_ERR_ARG, "bad vectors");
     void* p;
-    IBM_TRY(scratch(ctx, SLOT_MG_BASE, sizeof(double) * (REDUCE_PARTS + 8), &p));
+    IBM_TRY(scratch(ctx, SLOT_REDUCE, sizeof(double) * (REDUCE_PARTS + 8), &p));
     double* partial = (double*) p;
     IBM_TRY(launch_dot<double, double>(ctx, (size_t) n, x, y, partial, partial + REDUCE_PARTS));
     return read_scalar(ctx, partial + REDUCE_PARTS, out);

@@ -122,6 +122,11 @@ _SIGS = {
     "ibm_divergence": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP), _VP]),
     "ibm_gradient": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _VP, _D, ctypes.POINTER(_VP)]),
     "ibm_advection": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ibm_rbf_create": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_VP), _I, ctypes.POINTER(_VP)]),
+    "ibm_rbf_destroy": (_I, [_VP]),
+    "ibm_rbf_eval": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_I), ctypes.POINTER(_VP),
+                          ctypes.POINTER(_VP)]),
+    "ibm_surface_geometry": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
 }
 
 
@@ -490,3 +495,70 @@ def advection(ctx, grids, u, out=None):
         out = [torch.empty(grids[i].size, dtype=torch.float64, device=u[0].device) for i in range(3)]
     _check(lib().ibm_advection(ctx.handle, grids, _ptrs(u), _ptrs(out)), ctx)
     return out
+
+
+# ----------------------------------------------------------------------
+# RBF surface operators
+# ----------------------------------------------------------------------
+RBF_PRECISION = {"f64": 64, "f32": 32, "bf16x3": 16, 64: 64, 32: 32, 16: 16}
+
+
+class RbfOperator:
+    """Packed operators of one cell type (ibm_rbf_create).  `mats` are
+    float64 CUDA tensors of shape (M, lde) (sample-major: the reference's
+    lde x M column-major matrix), of which the first K columns are used."""
+
+    def __init__(self, ctx, precision, mats, K):
+        import torch
+        mats = list(mats)
+        M, lde = mats[0].shape
+        for m in mats:
+            if m.shape != (M, lde) or not m.is_contiguous() or m.dtype != torch.float64:
+                raise ValueError("operators must be contiguous float64 (M, lde) tensors of one shape")
+        self.M, self.K, self.nops = M, K, len(mats)
+        self.precision = RBF_PRECISION.get(precision, precision)
+        h = _VP()
+        _check(lib().ibm_rbf_create(ctx.handle, self.precision, M, K, len(mats), _ptrs(mats), lde,
+                                    ctypes.byref(h)), ctx)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().ibm_rbf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rbf_eval(ctx, ops, xs, ys=None):
+    """Y_b = [E_j^T X_b]_j for each (operator set, X) pair in one launch.
+    X_b: float64 CUDA tensor holding K x N column-major (any shape with
+    K*N elements, e.g. (N, K) row-major).  Returns tensors (nops, N, M)."""
+    import torch
+    ns = []
+    for op, x in zip(ops, xs):
+        if x.numel() % op.K or not x.is_contiguous() or x.dtype != torch.float64:
+            raise ValueError("X must be contiguous float64 with a multiple of K elements")
+        ns.append(x.numel() // op.K)
+    if ys is None:
+        ys = [torch.empty((op.nops, n, op.M), dtype=torch.float64, device=xs[0].device) for op, n in zip(ops, ns)]
+    handles = (_VP * len(ops))(*[op.handle.value for op in ops])
+    _check(lib().ibm_rbf_eval(ctx.handle, len(ops), handles, (_I * len(ns))(*ns), _ptrs(xs), _ptrs(ys)), ctx)
+    return ys
+
+
+def surface_geometry(ctx, ns, nc, y, t1, t2, w):
+    """Normals (3, nc, ns), sigma (nc, ns), area (nc), volume (nc) in FP64."""
+    import torch
+    dev = y.device
+    normal = torch.empty((3, nc, ns), dtype=torch.float64, device=dev)
+    sigma = torch.empty((nc, ns), dtype=torch.float64, device=dev)
+    area = torch.empty(nc, dtype=torch.float64, device=dev)
+    volume = torch.empty(nc, dtype=torch.float64, device=dev)
+    _check(lib().ibm_surface_geometry(ctx.handle, ns, nc, _ptr(y), _ptr(t1), _ptr(t2), _ptr(w), _ptr(normal),
+                                      _ptr(sigma), _ptr(area), _ptr(volume)), ctx)
+    return normal, sigma, area, volume

@@ -273,6 +273,45 @@ ibm_status ibm_gradient(ibm_ctx* ctx, const ibm_grid* p_grid, double alpha, cons
 ibm_status ibm_advection(ibm_ctx* ctx, const ibm_grid* u_grids /* [3] */, const double* const* u,
                          double* const* H /* [3] */);
 
+/* ---------------------------------------------------------------------
+ * RBF surface operators (bases/operators.h, bases/geometry.h)
+ * ------------------------------------------------------------------- */
+typedef struct ibm_rbf ibm_rbf;
+
+/* Arithmetic of an RBF evaluation. */
+enum {
+    IBM_RBF_F64 = 64,      /* FP64 matrix cores */
+    IBM_RBF_F32 = 32,      /* FP32 operands and accumulation */
+    IBM_RBF_BF16X3 = 16    /* split BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+};
+
+/* Prepares nops operators of one cell type for evaluation in `precision`:
+ * E[j] is a device matrix of lde x M doubles, column-major, whose first K
+ * rows are used: the slice E[0:K, 0:M] that bases::operators holds as its
+ * evaluator / derivative matrices, K = data sites, M = sample sites
+ * (bases/operators.h:145-177; the LU solve that produces E is setup).
+ * The operator keeps its own packed copy; E may be freed afterwards. */
+ibm_status ibm_rbf_create(ibm_ctx* ctx, int precision, int M, int K, int nops, const double* const* E, int lde,
+                          ibm_rbf** out);
+ibm_status ibm_rbf_destroy(ibm_rbf* op);
+
+/* Y_b[j] = E_j^T X_b for every batch entry b and operator j of ops[b], in
+ * one launch: `slice * matrix` (bases/operators.h:95-109).  X[b] is
+ * K x N[b] column-major (ld = K; column comp * ncells + cell, as
+ * bases/container.h:54 lays out data-site coordinates); Y[b] receives
+ * nops consecutive M x N[b] column-major blocks.  All entries share one
+ * precision. */
+ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N, const double* const* X,
+                        double* const* Y);
+
+/* Surface geometry of nc cells with ns sample sites, in FP64, from the
+ * evaluated positions y and tangents t1, t2 (each ns x 3nc as above) and
+ * the parameter-space quadrature weights w (ns): unit normals (ns x 3nc),
+ * patch areas sigma = w |t1 x t2| (ns x nc) (bases/geometry.h:95-119), and
+ * per cell the area sum(sigma) and volume sum(x . n sigma) / 3. */
+ibm_status ibm_surface_geometry(ibm_ctx* ctx, int ns, int nc, const double* y, const double* t1, const double* t2,
+                                const double* w, double* normal, double* sigma, double* area, double* volume);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -26,6 +26,15 @@ def load_inputs():
     return _load("ibm_inputs", os.path.join(ROOT, "ib.cu_amd", "inputs.py"))
 
 
+def load_surfaces():
+    """Host-side RBF operator construction (ib.cu_amd/surfaces.py)."""
+    return _load("ibm_surfaces", os.path.join(ROOT, "ib.cu_amd", "surfaces.py"))
+
+
+def load_orc_rbf():
+    return _load("orc_rbf", os.path.join(ROOT, "oracle", "rbf.py"))
+
+
 def load_orc():
     """The CPU oracle binding (oracle/orc.py)."""
     return _load("orc", os.path.join(ROOT, "oracle", "orc.py"))
