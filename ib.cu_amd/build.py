@@ -62,7 +62,26 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_api_check(verbose)
     return LIB
+
+
+API_SRC = os.path.join(ROOT, "tests", "cpp", "api_check.cpp")
+API_BIN = os.path.join(ROOT, "tests", "cpp", "api_check")
+
+
+def build_api_check(verbose=True):
+    """The C++ API program the GPU tests run (links libibm.so)."""
+    cmd = [HIPCC, "-O2", "-std=c++20", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", os.path.join(ROOT, "include"), API_SRC, "-o", API_BIN + ".tmp",
+           "-L", HERE, "-l:libibm.so", "-Wl,-rpath,$ORIGIN/../../ib.cu_amd", "-Wno-unused-command-line-argument"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {API_SRC}:\n{r.stderr}")
+    os.replace(API_BIN + ".tmp", API_BIN)
+    if verbose:
+        print(f"built {API_BIN}")
 
 
 if __name__ == "__main__":

@@ -43,7 +43,7 @@ ibm_status scratch(ibm_ctx* ctx, int slot, size_t bytes, void** out)
     return IBM_OK;
 }
 
-prof_scope::prof_scope(ibm_ctx* c, const char* name) : ctx(c)
+prof_scope::prof_scope(ibm_ctx* c, const char* name, double work) : ctx(c)
 {
     if (!ctx->profiling) return;
     if (ctx->prof_used == ctx->prof.size()) {
@@ -55,6 +55,7 @@ prof_scope::prof_scope(ibm_ctx* c, const char* name) : ctx(c)
     }
     index = (int) ctx->prof_used++;
     ctx->prof[index].name = name;
+    ctx->prof[index].work = work;
     hipEventRecord(ctx->prof[index].start, ctx->stream);
 }
 
@@ -90,6 +91,16 @@ ibm_status ibm_profile_read(ibm_ctx* ctx, const char* name, double* total_ms, in
     }
     *total_ms = t;
     *launches = k;
+    return IBM_OK;
+}
+
+ibm_status ibm_profile_work(ibm_ctx* ctx, const char* name, double* total)
+{
+    if (!ctx || !name || !total) return IBM_ERR_ARG;
+    double w = 0;
+    for (size_t i = 0; i < ctx->prof_used; ++i)
+        if (std::string(ctx->prof[i].name) == name) w += ctx->prof[i].work;
+    *total = w;
     return IBM_OK;
 }
 

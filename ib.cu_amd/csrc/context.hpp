@@ -22,7 +22,7 @@ struct ibm_ctx {
 
     // Optional per-kernel timing: hipEvent pairs recorded around named
     // launches on ctx->stream while profiling is enabled.
-    struct prof_record { const char* name; hipEvent_t start, stop; };
+    struct prof_record { const char* name; hipEvent_t start, stop; double work; };
     bool profiling = false;
     std::vector<prof_record> prof;
     size_t prof_used = 0;
@@ -58,7 +58,9 @@ ibm_status scratch(ibm_ctx* ctx, int slot, size_t bytes, void** out);
 struct prof_scope {
     ibm_ctx* ctx;
     int index = -1;
-    prof_scope(ibm_ctx* c, const char* name);
+    // `work`: the launch's algorithmic bytes (or flops for matrix-core
+    // kernels), summed by ibm_profile_work.
+    prof_scope(ibm_ctx* c, const char* name, double work = 0);
     ~prof_scope();
 };
 

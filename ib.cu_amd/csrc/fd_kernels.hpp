@@ -35,6 +35,23 @@ ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const
 template <typename T>
 ibm_status launch_hadamard(ibm_ctx* ctx, size_t n, const T* a, const T* b, T* out);
 
+// *out = max |x_i|
+ibm_status launch_max_abs(ibm_ctx* ctx, size_t n, const double* x, double* partial, double* out);
+// *out = sqrt(*num / *den), 0 when *den is 0
+ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den, double* out);
+
+// fd::lower_boundary / fd::upper_boundary applied to face data
+// (fd/boundary_ops.h:21-71): the ghost-cell boundary coefficient of the face
+// row along `view`, scaled by resolution^order, times the (corrected)
+// identity of the other axes.
+struct boundary_op {
+    int n[3];
+    int view, row;
+    double scale;
+    double id_lo[3], id_hi[3];
+};
+ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, const double* g, double value, double* y);
+
 // The differential (kind 0) or average (kind 1) of grid `src` along `view`.
 ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst);
 

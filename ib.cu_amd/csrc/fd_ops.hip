@@ -314,6 +314,72 @@ __global__ void shift_kernel(size_t n, const double* __restrict__ sum, double sc
     for (; i < n; i += stride) y[i] -= m;
 }
 
+__global__ void __launch_bounds__(REDUCE_THREADS)
+max_abs_partial_kernel(size_t n, const double* __restrict__ x, double* __restrict__ partial)
+{
+    __shared__ double part[REDUCE_THREADS / 64];
+    double m = 0;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = fmax(r, part[w]);
+        partial[blockIdx.x] = r;
+    }
+}
+
+__global__ void __launch_bounds__(REDUCE_THREADS)
+finish_max_kernel(int nparts, const double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double part[REDUCE_THREADS / 64];
+    double m = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, partial[i]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = fmax(r, part[w]);
+        *out = r;
+    }
+}
+
+__global__ void sqrt_ratio_kernel(const double* __restrict__ num, const double* __restrict__ den,
+                                  double* __restrict__ out)
+{
+    double d = *den;
+    *out = d > 0 ? sqrt(*num / d) : 0.0;
+}
+
+// y[row] += alpha * scale * g[face] * prod_{d != view} id_d(i_d) on the face
+// rows (index `row` along `view`) of a grid; g indexed over the other axes,
+// x fastest (the columns of the kron-assembled boundary operator).
+__global__ void boundary_kernel(boundary_op B, double alpha, const double* __restrict__ g, double value,
+                                double* __restrict__ y)
+{
+    int a0 = B.view == 0 ? 1 : 0;
+    int a1 = B.view == 2 ? 1 : 2;
+    long nface = (long) B.n[a0] * B.n[a1];
+    for (long f = (long) blockIdx.x * blockDim.x + threadIdx.x; f < nface; f += (long) gridDim.x * blockDim.x) {
+        int i0 = (int) (f % B.n[a0]), i1 = (int) (f / B.n[a0]);
+        int idx[3];
+        idx[B.view] = B.row;
+        idx[a0] = i0;
+        idx[a1] = i1;
+        double w = 1.0;
+        if (i0 == 0) w *= B.id_lo[a0];
+        if (i0 == B.n[a0] - 1) w *= B.id_hi[a0];
+        if (i1 == 0) w *= B.id_lo[a1];
+        if (i1 == B.n[a1] - 1) w *= B.id_hi[a1];
+        double gv = g ? g[f] : value;
+        long at = idx[0] + (long) B.n[0] * (idx[1] + (long) B.n[1] * idx[2]);
+        y[at] += alpha * B.scale * gv * w;
+    }
+}
+
 inline int grid_for(size_t n, int threads = 256)
 {
     size_t b = (n + threads - 1) / threads;
@@ -335,7 +401,8 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
     dim3 block(STX, STY);
     dim3 grid((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
-    prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32");
+    const double streams = 2.0 + (y ? 1.0 : 0.0) + (z ? 1.0 : 0.0);   // x, out, optional y, z
+    prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32", streams * sizeof(T) * nx * ny * (double) nz);
     stencil_kernel<T><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
@@ -345,7 +412,8 @@ template <typename T>
 ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse)
 {
     size_t n = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
-    prof_scope prof(ctx, "restrict");
+    size_t nf = (size_t) R.a[0].nf * R.a[1].nf * R.a[2].nf;
+    prof_scope prof(ctx, "restrict", (double) sizeof(T) * (nf + n));
     restrict_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(R, fine, coarse);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
@@ -355,7 +423,8 @@ template <typename T>
 ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine)
 {
     size_t n = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
-    prof_scope prof(ctx, "prolong");
+    size_t nc = (size_t) P.a[0].nc * P.a[1].nc * P.a[2].nc;
+    prof_scope prof(ctx, "prolong", (double) sizeof(T) * (nc + 2 * n));
     prolong_add_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, coarse, fine);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
@@ -366,7 +435,8 @@ ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const
 {
     size_t n = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
     if (!n) return IBM_OK;
-    prof_scope prof(ctx, "twopoint");
+    size_t ns = (size_t) P.sn[0] * P.sn[1] * P.sn[2];
+    prof_scope prof(ctx, "twopoint", (double) sizeof(T) * (ns + n + (beta != T(0) ? n : 0)));
     twopoint_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, alpha, x, beta, out);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
@@ -429,6 +499,31 @@ ibm_status launch_remove_mean(ibm_ctx* ctx, size_t n, T* y, double* partial, dou
 {
     IBM_TRY(launch_sum<T>(ctx, n, y, partial, sum));
     shift_kernel<T><<<grid_for(n), 256, 0, ctx->stream>>>(n, sum, 1.0 / (double) n, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status launch_max_abs(ibm_ctx* ctx, size_t n, const double* x, double* partial, double* out)
+{
+    max_abs_partial_kernel<<<REDUCE_BLOCKS, REDUCE_THREADS, 0, ctx->stream>>>(n, x, partial);
+    IBM_LAUNCH_CHECK(ctx);
+    finish_max_kernel<<<1, REDUCE_THREADS, 0, ctx->stream>>>(REDUCE_BLOCKS, partial, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den, double* out)
+{
+    sqrt_ratio_kernel<<<1, 1, 0, ctx->stream>>>(num, den, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, const double* g, double value, double* y)
+{
+    int a0 = B.view == 0 ? 1 : 0, a1 = B.view == 2 ? 1 : 2;
+    size_t nface = (size_t) B.n[a0] * B.n[a1];
+    boundary_kernel<<<grid_for(nface), 256, 0, ctx->stream>>>(B, alpha, g, value, y);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }

@@ -87,6 +87,34 @@ ibm_status ibm_average(ibm_ctx* ctx, const ibm_grid* src, int view, double alpha
     return twopoint_apply(ctx, src, view, 1, alpha, x, beta, y);
 }
 
+// fd::lower_boundary / fd::upper_boundary (fd/boundary_ops.h:21-71) times
+// face data: y += alpha * B g.  B is (points x 1 along `view`, scaled by
+// resolution^order) kron the (order-corrected) identity of the other axes;
+// g holds one value per face point (other axes, x fastest) or, when null,
+// the constant `value` (the algo::fill vectors of test/ib-whole.cc:146-149).
+ibm_status ibm_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int view, int side, int order, double alpha,
+                              const double* values, double value, double* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, g, 1));
+    if (view < 0 || view > 2 || (side != 0 && side != 1) || (order != 0 && order != 2) || !y)
+        return fail(ctx, IBM_ERR_ARG, "bad boundary arguments (view %d side %d order %d)", view, side, order);
+    const ibm_axis& a = g->axis[view];
+    if (!a.solid) return IBM_OK;            // periodic: the boundary operator has no columns
+    boundary_op B;
+    for (int d = 0; d < 3; ++d) {
+        const ibm_axis& e = g->axis[d];
+        B.n[d] = e.points;
+        bool corr = order == 2 && e.solid;
+        B.id_lo[d] = corr ? 1.0 + e.laplacian[0] : 1.0;
+        B.id_hi[d] = corr ? 1.0 + e.laplacian[1] : 1.0;
+    }
+    B.view = view;
+    B.row = side ? a.points - 1 : 0;
+    B.scale = std::pow(a.resolution, order) * a.boundary[side];
+    return launch_boundary(ctx, B, alpha, values, value, y);
+}
+
 // ins::divergence (ins/differential.h:46-83): out = sum_i D_i u_i on the
 // grid reached by shifting u_i's grid along axis i (the cell centres for MAC).
 ibm_status ibm_divergence(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double* out)

@@ -18,6 +18,7 @@
 //   * mixed precision: FP32 V-cycles inside an FP64 defect correction
 //     x += V(b - A x), with the residual, its norm and the update in FP64.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <vector>
 #include "context.hpp"
@@ -178,8 +179,8 @@ static ibm_status norm2(ibm_ctx* ctx, ibm_mg* m, size_t n, const double* x, doub
 }
 
 template <typename T>
-static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, int maxit,
-                             int* iters, double* relres)
+static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, double atol,
+                             int maxit, int* iters, double* relres)
 {
     const mg_level& L0 = m->lv[0];
     const size_t n = L0.n;
@@ -198,7 +199,8 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
     IBM_TRY(launch_convert<double, double>(ctx, n, b, m->r));
     int it = 0;
     double nr = nb;
-    while (nb > 0 && it < maxit) {
+    const double stop = std::max(rtol * nb, atol);
+    while (nb > stop && it < maxit) {
         IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
         IBM_TRY(vcycle<T>(ctx, m, 0));
         IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
@@ -206,11 +208,43 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
         IBM_TRY(launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
         ++it;
         IBM_TRY(norm2(ctx, m, n, m->r, &nr));
-        if (nr <= rtol * nb) break;
+        if (nr <= stop) break;
     }
     if (iters) *iters = it;
     if (relres) *relres = nb > 0 ? nr / nb : 0.0;
-    if (nb > 0 && nr > rtol * nb) return fail(ctx, IBM_ERR_NOT_CONVERGED, "multigrid: relative residual %.3e after %d cycles", nr / nb, it);
+    if (nr > stop)
+        return fail(ctx, IBM_ERR_NOT_CONVERGED, "multigrid: relative residual %.3e after %d cycles", nr / nb, it);
+    return IBM_OK;
+}
+
+// The same defect correction with a fixed number of V-cycles and no host
+// synchronisation (so it can be recorded in a HIP graph); the final
+// relative residual ||b - A x|| / ||b|| is left in device memory.
+template <typename T>
+static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, double* relres_dev)
+{
+    const mg_level& L0 = m->lv[0];
+    const size_t n = L0.n;
+    level_bufs<T>& B0 = bufs<T>(m)[0];
+    if (m->singular) {
+        IBM_TRY(launch_convert<double, double>(ctx, n, b, m->rhs));
+        IBM_TRY(launch_remove_mean<double>(ctx, n, m->rhs, m->partial, m->scalars + 1));
+        b = m->rhs;
+    }
+    IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
+    IBM_TRY(launch_convert<double, double>(ctx, n, b, m->r));
+    for (int it = 0; it < iters; ++it) {
+        IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
+        IBM_TRY(vcycle<T>(ctx, m, 0));
+        IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
+        if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
+        IBM_TRY(launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
+    }
+    if (relres_dev) {
+        IBM_TRY(launch_dot<double, double>(ctx, n, b, b, m->partial, m->scalars + 2));
+        IBM_TRY(launch_dot<double, double>(ctx, n, m->r, m->r, m->partial, m->scalars + 3));
+        IBM_TRY(launch_sqrt_ratio(ctx, m->scalars + 3, m->scalars + 2, relres_dev));
+    }
     return IBM_OK;
 }
 
@@ -390,8 +424,37 @@ ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, dou
     if (!ctx || !m) return IBM_ERR_ARG;
     if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
     if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
-    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, maxit, iters, relres);
-    return solve_loop<float>(ctx, m, b, x, rtol, maxit, iters, relres);
+    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, 0.0, maxit, iters, relres);
+    return solve_loop<float>(ctx, m, b, x, rtol, 0.0, maxit, iters, relres);
+}
+
+ibm_status ibm_mg_solve_atol(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, double atol, int maxit,
+                             int* iters, double* relres)
+{
+    if (!ctx || !m) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
+    if (!(atol >= 0)) return fail(ctx, IBM_ERR_ARG, "negative absolute tolerance");
+    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, atol, maxit, iters, relres);
+    return solve_loop<float>(ctx, m, b, x, rtol, atol, maxit, iters, relres);
+}
+
+ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, double b, double* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (n < 0 || (n && (!x || !y))) return fail(ctx, IBM_ERR_ARG, "bad vector arguments");
+    if (!n) return IBM_OK;
+    return launch_axpby<double, double>(ctx, (size_t) n, a, x, b, y);
+}
+
+ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, double* relres)
+{
+    if (!ctx || !m) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
+    if (iters < 0) return fail(ctx, IBM_ERR_ARG, "negative iteration count");
+    if (m->precision == 64) return solve_fixed<double>(ctx, m, b, x, iters, relres);
+    return solve_fixed<float>(ctx, m, b, x, iters, relres);
 }
 
 // solvers::chebpcg (solvers/chebpcg.h:9-23 -> solvers/pcg.h:11-44): CG

@@ -424,7 +424,9 @@ ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int*
     G.total_tiles = tiles;
     int blocks = round_up(tiles, 8);
     G.xcd_chunk = blocks / 8;
-    prof_scope prof(ctx, "rbf_gemm");
+    double flops = 0;
+    for (int b = 0; b < count; ++b) flops += 2.0 * ops[b]->M * (double) N[b] * ops[b]->K * ops[b]->nops;
+    prof_scope prof(ctx, "rbf_gemm", flops);
     if (mode == MODE_F64) rbf_gemm_kernel<MODE_F64><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
     else if (mode == MODE_F32) rbf_gemm_kernel<MODE_F32><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
     else rbf_gemm_kernel<MODE_BF16X3><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
@@ -439,7 +441,8 @@ ibm_status ibm_surface_geometry(ibm_ctx* ctx, int ns, int nc, const double* y, c
     if (ns <= 0 || nc < 0) return fail(ctx, IBM_ERR_ARG, "bad sizes ns=%d nc=%d", ns, nc);
     if (!nc) return IBM_OK;
     if (!y || !t1 || !t2 || !w || !normal || !sigma || !area || !volume) return fail(ctx, IBM_ERR_ARG, "null array");
-    prof_scope prof(ctx, "geometry");
+    // y, t1, t2 and w read; normals, sigma, area, volume written
+    prof_scope prof(ctx, "geometry", 8.0 * ((double) ns * nc * 13 + ns + 2.0 * nc));
     geometry_kernel<<<nc, 256, 0, ctx->stream>>>(ns, nc, y, t1, t2, w, normal, sigma, area, volume);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;

@@ -1,0 +1,460 @@
+// Immersed-boundary timestep: the composer the reference calls as
+// ib::solver (test/ib-whole.cc:226,250-275; its header is not part of the
+// reference), built from the library's operators.  The scheme:
+//
+//   1. surface geometry of every cell type from its data sites X:
+//      [y, y_theta, y_phi] = E^T X (one RBF launch for all types), unit
+//      normals, patch areas and per-cell area / volume in FP64
+//      (bases/geometry.h:74-123);
+//   2. membrane forces at the sample sites: a tether f = -k (y - y_ref) to
+//      the reference sample positions (the spring of test/ib-whole.cc:237-245;
+//      the reference's Skalak / bending models are outside this library);
+//   3. F = spread(y, f) onto the MAC grid (ib/novel.h);
+//   4. H = ins::advection(u) (ins/advection.h:58-125);
+//   5. per component, ins::diffusion (ins/diffusion.h:46-56) with gamma = 1:
+//        rhs = dt nu (b + L u) + dt I~ (F / rho - H),
+//        (I~ - lambda L) du = rhs,  lambda = dt nu / 2,  u* = u + du,
+//      b the boundary data of moving walls (fd::upper_boundary x values,
+//      test/ib-whole.cc:129-152), solved by multigrid;
+//   6. projection: L_p phi = (D u*) / dt with L_p = D G the Neumann
+//      Laplacian of the cell centres (singular, mean-free solution),
+//      u = u* - dt G phi, p = phi;
+//   7. FP64 divergence check max |D u|;
+//   8. U = interpolate(u) at the data sites (ib/interpolate.h), X += dt U.
+//
+// All work is enqueued on the context stream.  ibm_solver_step chooses the
+// number of V-cycles of each solve adaptively (host reads of residual norms);
+// ibm_solver_step_fixed / ibm_solver_capture use fixed counts (the largest
+// the adaptive steps needed so far, plus one) and no host synchronisation,
+// so a whole step can be recorded once as a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "context.hpp"
+#include "fd_kernels.hpp"
+
+namespace ibm {
+namespace {
+
+constexpr int MAX_TYPES = IBM_MAX_CELL_TYPES;
+
+__global__ void tether_kernel(size_t n, double k, const double* __restrict__ y, const double* __restrict__ ref,
+                              double* __restrict__ f)
+{
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        f[i] = -k * (y[i] - ref[i]);
+}
+
+// h = F / rho - h
+__global__ void accel_kernel(size_t n, double inv_rho, const double* __restrict__ F, double* __restrict__ h)
+{
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        h[i] = F[i] * inv_rho - h[i];
+}
+
+// x *= a in place
+__global__ void scale_kernel(size_t n, double a, double* x)
+{
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        x[i] *= a;
+}
+
+int blocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 65536); }
+
+} // namespace
+} // namespace ibm
+
+using namespace ibm;
+
+struct ibm_solver {
+    ibm_solver_params par{};
+    ibm_grid g[3]{}, gp{};
+    size_t nu[3]{}, np = 0;
+    ibm_mg* helm[3]{};
+    ibm_mg* pois = nullptr;
+    struct type_t {
+        ibm_rbf* ops;
+        int nd, ns, nc;
+        double stiffness;
+        double *w, *ref, *Y, *f, *normal, *sigma, *area, *volume, *U;
+    };
+    std::vector<type_t> types;
+    double *F[3]{}, *H[3]{}, *rhs[3]{}, *du[3]{}, *div = nullptr;
+    double* info = nullptr;            // device: residual[4], divergence, scratch scalars
+    double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
+    int max_iters[4]{};                // largest adaptive iteration counts seen
+    int last_iters[4]{};
+    int adaptive_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ibm_state captured{};
+    std::vector<void*> owned;
+};
+
+namespace {
+
+ibm_status alloc(ibm_ctx* ctx, ibm_solver* s, size_t n, double** p)
+{
+    void* v = nullptr;
+    IBM_HIP(ctx, hipMalloc(&v, sizeof(double) * std::max<size_t>(n, 1)));
+    s->owned.push_back(v);
+    *p = (double*) v;
+    return IBM_OK;
+}
+
+void release(ibm_solver* s)
+{
+    if (s->exec) hipGraphExecDestroy(s->exec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    for (int i = 0; i < 3; ++i)
+        if (s->helm[i]) ibm_mg_destroy(s->helm[i]);
+    if (s->pois) ibm_mg_destroy(s->pois);
+    hipDeviceSynchronize();
+    for (void* p : s->owned) hipFree(p);
+    delete s;
+}
+
+ibm_status check_state(ibm_ctx* ctx, const ibm_solver* s, const ibm_state* st)
+{
+    if (!st) return fail(ctx, IBM_ERR_ARG, "null state");
+    for (int i = 0; i < 3; ++i)
+        if (!st->u[i]) return fail(ctx, IBM_ERR_ARG, "null velocity component %d", i);
+    if (!st->p) return fail(ctx, IBM_ERR_ARG, "null pressure");
+    for (size_t t = 0; t < s->types.size(); ++t)
+        if (!st->x[t] && s->types[t].nc) return fail(ctx, IBM_ERR_ARG, "null positions of cell type %zu", t);
+    return IBM_OK;
+}
+
+// One timestep; `fixed` selects fixed V-cycle counts and no host reads.
+ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* ub, bool fixed)
+{
+    const ibm_solver_params& P = s->par;
+    const double dt = P.timestep, nu = P.diffusivity;
+    const int nt = (int) s->types.size();
+
+    // 1. geometry (one launch for every cell type and operator)
+    {
+        ibm_rbf* ops[MAX_TYPES];
+        int N[MAX_TYPES];
+        const double* X[MAX_TYPES];
+        double* Y[MAX_TYPES];
+        int count = 0;
+        for (int t = 0; t < nt; ++t) {
+            auto& T = s->types[t];
+            if (!T.nc) continue;
+            ops[count] = T.ops;
+            N[count] = 3 * T.nc;
+            X[count] = st->x[t];
+            Y[count] = T.Y;
+            ++count;
+        }
+        if (count) IBM_TRY(ibm_rbf_eval(ctx, count, ops, N, X, Y));
+        for (int t = 0; t < nt; ++t) {
+            auto& T = s->types[t];
+            if (!T.nc) continue;
+            size_t blk = (size_t) T.ns * 3 * T.nc;
+            IBM_TRY(ibm_surface_geometry(ctx, T.ns, T.nc, T.Y, T.Y + blk, T.Y + 2 * blk, T.w, T.normal, T.sigma,
+                                         T.area, T.volume));
+            // 2. tether forces
+            tether_kernel<<<blocks(blk), 256, 0, ctx->stream>>>(blk, T.stiffness, T.Y, T.ref, T.f);
+            IBM_LAUNCH_CHECK(ctx);
+        }
+    }
+    // 3. spread all cell types onto the MAC grid
+    {
+        int counts[MAX_TYPES];
+        const double* xs[MAX_TYPES];
+        const double* fs[MAX_TYPES];
+        for (int t = 0; t < nt; ++t) {
+            counts[t] = s->types[t].ns * s->types[t].nc;
+            xs[t] = s->types[t].Y;
+            fs[t] = s->types[t].f;
+        }
+        IBM_TRY(ibm_spread_sets(ctx, s->g, (ibm_delta) P.delta, nt, counts, xs, fs, s->F));
+    }
+    // 4. advection
+    {
+        const double* u[3] = {st->u[0], st->u[1], st->u[2]};
+        IBM_TRY(ibm_advection(ctx, s->g, u, s->H));
+    }
+    // 5. diffusion, component by component
+    for (int i = 0; i < 3; ++i) {
+        const size_t n = s->nu[i];
+        accel_kernel<<<blocks(n), 256, 0, ctx->stream>>>(n, 1.0 / P.density, s->F[i], s->H[i]);
+        IBM_LAUNCH_CHECK(ctx);
+        op3<double> I = make_op<double>(s->g[i], 1.0, 0.0);
+        op3<double> L = make_op<double>(s->g[i], 0.0, 1.0);
+        // rhs = dt I~ (F/rho - H) + dt nu b
+        IBM_TRY(launch_stencil<double>(ctx, I, dt, s->H[i], ub && ub[i] ? dt * nu : 0.0, ub ? ub[i] : nullptr, 0.0,
+                                       nullptr, s->rhs[i]));
+        // rhs += dt nu L u
+        IBM_TRY(launch_stencil<double>(ctx, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr, s->rhs[i]));
+        if (fixed) {
+            IBM_TRY(ibm_mg_solve_fixed(ctx, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, s->info + i));
+        } else {
+            int it = 0;
+            double rr = 0;
+            IBM_TRY(ibm_mg_solve(ctx, s->helm[i], s->rhs[i], s->du[i], P.tolerance, P.max_iterations, &it, &rr));
+            s->last_iters[i] = it;
+            IBM_HIP(ctx, hipMemcpyAsync(s->info + i, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        IBM_TRY(launch_axpby<double, double>(ctx, n, 1.0, s->du[i], 1.0, st->u[i]));
+    }
+    // 6. projection
+    {
+        const double* u[3] = {st->u[0], st->u[1], st->u[2]};
+        IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
+        scale_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, 1.0 / dt, s->div);
+        IBM_LAUNCH_CHECK(ctx);
+        if (fixed) {
+            IBM_TRY(ibm_mg_solve_fixed(ctx, s->pois, s->div, st->p, s->max_iters[3] + 1, s->info + 3));
+        } else {
+            // Absolute floor: tolerance x the divergence scale of u*,
+            // ||D|| ||u*|| / dt with ||D|| <= 2 / h (the right-hand side of a
+            // nearly divergence-free u* is rounding noise).
+            double uu = 0;
+            for (int i = 0; i < 3; ++i) {
+                IBM_TRY(launch_dot<double, double>(ctx, s->nu[i], st->u[i], st->u[i], s->partial, s->info + 5));
+                double h = 0;
+                IBM_HIP(ctx, hipMemcpyAsync(&h, s->info + 5, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+                IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                uu += h;
+            }
+            double rmax = 0;
+            for (int d = 0; d < 3; ++d) rmax = std::max(rmax, s->gp.axis[d].resolution);
+            const double atol = P.tolerance * 2.0 * rmax * std::sqrt(uu) / dt;
+            int it = 0;
+            double rr = 0;
+            IBM_TRY(ibm_mg_solve_atol(ctx, s->pois, s->div, st->p, P.tolerance, atol, P.max_iterations, &it, &rr));
+            s->last_iters[3] = it;
+            IBM_HIP(ctx, hipMemcpyAsync(s->info + 3, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        double* out[3] = {st->u[0], st->u[1], st->u[2]};
+        IBM_TRY(ibm_gradient(ctx, &s->gp, -dt, st->p, 1.0, out));
+        // 7. FP64 divergence check
+        IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
+        IBM_TRY(launch_max_abs(ctx, s->np, s->div, s->partial, s->info + 4));
+    }
+    // 8. interpolate at the data sites and move them
+    {
+        int counts[MAX_TYPES];
+        const double* xs[MAX_TYPES];
+        double* outs[MAX_TYPES];
+        for (int t = 0; t < nt; ++t) {
+            counts[t] = s->types[t].nd * s->types[t].nc;
+            xs[t] = st->x[t];
+            outs[t] = s->types[t].U;
+        }
+        const double* ue[3] = {st->u[0], st->u[1], st->u[2]};
+        IBM_TRY(ibm_interpolate_sets(ctx, s->g, (ibm_delta) P.delta, nt, counts, xs, ue, outs));
+        for (int t = 0; t < nt; ++t)
+            if (counts[t])
+                IBM_TRY(launch_axpby<double, double>(ctx, (size_t) 3 * counts[t], dt, s->types[t].U, 1.0, st->x[t]));
+    }
+    if (!fixed) {
+        for (int i = 0; i < 4; ++i) s->max_iters[i] = std::max(s->max_iters[i], s->last_iters[i]);
+        ++s->adaptive_steps;
+    } else {
+        for (int i = 0; i < 4; ++i) s->last_iters[i] = s->max_iters[i] + 1;
+    }
+    return IBM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndims, int refinement,
+                             const ibm_solver_params* params, int ntypes, const ibm_cell_type* types,
+                             ibm_solver** out)
+{
+    if (!ctx || !out || !params) return IBM_ERR_ARG;
+    *out = nullptr;
+    if (ndims != 3) return fail(ctx, IBM_ERR_ARG, "the timestep needs a 3-dimensional domain");
+    if (ntypes < 0 || ntypes > IBM_MAX_CELL_TYPES || (ntypes && !types))
+        return fail(ctx, IBM_ERR_ARG, "between 0 and %d cell types", IBM_MAX_CELL_TYPES);
+    const ibm_solver_params& P = *params;
+    if (!(P.timestep > 0) || !(P.diffusivity >= 0) || !(P.density > 0) || !(P.tolerance > 0) || P.max_iterations < 1)
+        return fail(ctx, IBM_ERR_ARG, "bad solver parameters");
+    if (P.precision != 64 && P.precision != 32) return fail(ctx, IBM_ERR_ARG, "precision must be 64 or 32");
+    auto* s = new ibm_solver;
+    s->par = P;
+    ibm_status st = ibm_mac_grids(domain, ndims, refinement, s->g);
+    // Pressure lives on the cell centres with homogeneous Neumann walls:
+    // D (on the Dirichlet velocity grids) composed with G is that Laplacian.
+    ibm_dimension pd[3];
+    for (int d = 0; d < 3; ++d) {
+        pd[d] = domain[d];
+        if (pd[d].solid) {
+            pd[d].lower[0] = pd[d].upper[0] = 0.0;
+            pd[d].lower[1] = pd[d].upper[1] = 1.0;
+        }
+    }
+    if (st == IBM_OK) st = ibm_centered_grid(pd, 3, refinement, &s->gp);
+    if (st != IBM_OK) {
+        release(s);
+        return fail(ctx, st, "could not build the grids");
+    }
+    for (int i = 0; i < 3; ++i) s->nu[i] = (size_t) ibm_grid_size(&s->g[i]);
+    s->np = (size_t) ibm_grid_size(&s->gp);
+    const double lambda = P.timestep * P.diffusivity / 2;
+    for (int i = 0; i < 3 && st == IBM_OK; ++i) st = ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm[i]);
+    if (st == IBM_OK) st = ibm_mg_create(ctx, &s->gp, 0.0, 1.0, P.precision, &s->pois);
+    for (int i = 0; i < 3 && st == IBM_OK; ++i) {
+        if ((st = alloc(ctx, s, s->nu[i], &s->F[i]))) break;
+        if ((st = alloc(ctx, s, s->nu[i], &s->H[i]))) break;
+        if ((st = alloc(ctx, s, s->nu[i], &s->rhs[i]))) break;
+        st = alloc(ctx, s, s->nu[i], &s->du[i]);
+    }
+    if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->div);
+    if (st == IBM_OK) st = alloc(ctx, s, 8, &s->info);
+    if (st == IBM_OK) st = alloc(ctx, s, REDUCE_PARTS, &s->partial);
+    for (int t = 0; t < ntypes && st == IBM_OK; ++t) {
+        const ibm_cell_type& c = types[t];
+        if (!c.ops || c.data_sites <= 0 || c.sample_sites <= 0 || c.cells < 0 || !c.weights || !c.tether) {
+            st = fail(ctx, IBM_ERR_ARG, "bad cell type %d", t);
+            break;
+        }
+        ibm_solver::type_t T{};
+        T.ops = c.ops;
+        T.nd = c.data_sites;
+        T.ns = c.sample_sites;
+        T.nc = c.cells;
+        T.stiffness = c.stiffness;
+        size_t blk = (size_t) T.ns * 3 * T.nc;
+        if ((st = alloc(ctx, s, T.ns, &T.w)) || (st = alloc(ctx, s, blk, &T.ref)) || (st = alloc(ctx, s, 3 * blk, &T.Y)) ||
+            (st = alloc(ctx, s, blk, &T.f)) || (st = alloc(ctx, s, blk, &T.normal)) ||
+            (st = alloc(ctx, s, (size_t) T.ns * T.nc, &T.sigma)) || (st = alloc(ctx, s, T.nc, &T.area)) ||
+            (st = alloc(ctx, s, T.nc, &T.volume)) || (st = alloc(ctx, s, (size_t) 3 * T.nd * T.nc, &T.U)))
+            break;
+        if (hipMemcpyAsync(T.w, c.weights, sizeof(double) * T.ns, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(T.ref, c.tether, sizeof(double) * blk, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+            st = fail(ctx, IBM_ERR_HIP, "copying cell type %d", t);
+            break;
+        }
+        s->types.push_back(T);
+    }
+    if (st == IBM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(ctx, IBM_ERR_HIP, "solver setup");
+    if (st != IBM_OK) {
+        release(s);
+        return st;
+    }
+    *out = s;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_destroy(ibm_solver* s)
+{
+    if (s) release(s);
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_step(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* boundary)
+{
+    if (!ctx || !s) return IBM_ERR_ARG;
+    IBM_TRY(check_state(ctx, s, st));
+    IBM_TRY(step_impl(ctx, s, st, boundary, false));
+    st->t += s->par.timestep;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_step_fixed(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* boundary)
+{
+    if (!ctx || !s) return IBM_ERR_ARG;
+    IBM_TRY(check_state(ctx, s, st));
+    if (!s->adaptive_steps) return fail(ctx, IBM_ERR_ARG, "run an adaptive step first to calibrate the solves");
+    IBM_TRY(step_impl(ctx, s, st, boundary, true));
+    st->t += s->par.timestep;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_capture(ibm_ctx* ctx, ibm_solver* s, const ibm_state* st, const double* const* boundary)
+{
+    if (!ctx || !s) return IBM_ERR_ARG;
+    IBM_TRY(check_state(ctx, s, st));
+    if (!s->adaptive_steps) return fail(ctx, IBM_ERR_ARG, "run an adaptive step first to calibrate the solves");
+    if (!ctx->stream) return fail(ctx, IBM_ERR_ARG, "graph capture needs a created stream, not the null stream");
+    if (s->exec) {
+        hipGraphExecDestroy(s->exec);
+        s->exec = nullptr;
+    }
+    if (s->graph) {
+        hipGraphDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    const bool profiling = ctx->profiling;
+    ctx->profiling = false;
+    ibm_state copy = *st;
+    IBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ibm_status rc = step_impl(ctx, s, &copy, boundary, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->profiling = profiling;
+    if (rc != IBM_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    s->graph = g;
+    IBM_HIP(ctx, hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0));
+    s->captured = *st;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_replay(ibm_ctx* ctx, ibm_solver* s, ibm_state* st)
+{
+    if (!ctx || !s) return IBM_ERR_ARG;
+    if (!s->exec) return fail(ctx, IBM_ERR_ARG, "no captured step");
+    if (!st || std::memcmp(st->u, s->captured.u, sizeof(st->u)) || st->p != s->captured.p ||
+        std::memcmp(st->x, s->captured.x, sizeof(st->x)))
+        return fail(ctx, IBM_ERR_ARG, "the state's buffers differ from the captured ones");
+    IBM_HIP(ctx, hipGraphLaunch(s->exec, ctx->stream));
+    st->t += s->par.timestep;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_info(ibm_ctx* ctx, ibm_solver* s, ibm_step_info* out)
+{
+    if (!ctx || !s || !out) return IBM_ERR_ARG;
+    double h[5];
+    IBM_HIP(ctx, hipMemcpyAsync(h, s->info, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < 4; ++i) {
+        out->iterations[i] = s->last_iters[i];
+        out->residual[i] = h[i];
+    }
+    out->divergence = h[4];
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_cells(ibm_solver* s, int type, ibm_cell_view* out)
+{
+    if (!s || !out || type < 0 || type >= (int) s->types.size()) return IBM_ERR_ARG;
+    const auto& T = s->types[type];
+    size_t blk = (size_t) T.ns * 3 * T.nc;
+    out->positions = T.Y;
+    out->tangents[0] = T.Y + blk;
+    out->tangents[1] = T.Y + 2 * blk;
+    out->forces = T.f;
+    out->normals = T.normal;
+    out->sigma = T.sigma;
+    out->area = T.area;
+    out->volume = T.volume;
+    out->velocity = T.U;
+    return IBM_OK;
+}
+
+ibm_status ibm_solver_grids(ibm_solver* s, ibm_grid* velocity /* [3] */, ibm_grid* pressure)
+{
+    if (!s) return IBM_ERR_ARG;
+    if (velocity)
+        for (int i = 0; i < 3; ++i) velocity[i] = s->g[i];
+    if (pressure) *pressure = s->gp;
+    return IBM_OK;
+}
+
+} // extern "C"

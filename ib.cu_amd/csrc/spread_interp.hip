@@ -132,6 +132,14 @@ __device__ inline unsigned long long wave_max(unsigned long long v)
 // cell plus the integer part of its shift is q_j.  Every component's base
 // cell then lies in q_j + [lo_j, hi_j], which bounds the bins a brick reads.
 // ----------------------------------------------------------------------
+// Clears n 32-bit words (used in place of hipMemsetAsync so that a binning
+// pass can be recorded in a HIP graph).
+__global__ void zero_words_kernel(unsigned int* __restrict__ p, size_t n)
+{
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 __global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int with_forces,
                                  int* __restrict__ key, int* __restrict__ rank,
                                  int* __restrict__ code, int* __restrict__ count,
@@ -809,8 +817,14 @@ ibm_status bin_markers(ibm_ctx* ctx, const marker_sets& sets, const kgrids& kg, 
     IBM_TRY(scratch(ctx, SLOT_FLAGS, sizeof(bin_stats), &p)); stats = (bin_stats*) p;
 
     prof_scope prof(ctx, "bin");
-    IBM_HIP(ctx, hipMemsetAsync(count, 0, sizeof(int) * (size_t) (nbins + 1), ctx->stream));
-    IBM_HIP(ctx, hipMemsetAsync(stats, 0, sizeof(bin_stats), ctx->stream));
+    {
+        size_t words = (size_t) nbins + 1;
+        zero_words_kernel<<<blocks_for((long) std::min<size_t>(words, 1u << 22), 256), 256, 0, ctx->stream>>>(
+            (unsigned int*) count, words);
+        IBM_LAUNCH_CHECK(ctx);
+        zero_words_kernel<<<1, 256, 0, ctx->stream>>>((unsigned int*) stats, sizeof(bin_stats) / 4);
+        IBM_LAUNCH_CHECK(ctx);
+    }
     if (total > 0) {
         bin_count_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, with_forces, key, rank,
                                                                            code, count, stats);
@@ -880,7 +894,10 @@ ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
             for (int j = 0; j < 3; ++j)
                 if (kg.a[c][j].points > 1022)
                     return fail(ctx, IBM_ERR_SIZE, "small periodic axes with more than 1022 points on another axis");
-    prof_scope prof(ctx, "spread_brick");
+    double grid_points = 0;
+    for (int c = 0; c < 3; ++c) grid_points += (double) kg.points[c][0] * kg.points[c][1] * kg.points[c][2];
+    // positions + forces read once, three grids written once
+    prof_scope prof(ctx, "spread_brick", 48.0 * sets.total + 8.0 * grid_points);
     auto launch = [&](auto kernel) {
         kernel<<<(int) nbricks, SPREAD_THREADS, 0, ctx->stream>>>(
             kg, bd, delta, sets.total, start, sorted, stats, fe[0], fe[1], fe[2], bx, by, (int) nbricks, xcd_chunk);

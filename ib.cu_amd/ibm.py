@@ -97,6 +97,7 @@ _SIGS = {
     "ibm_last_error": (ctypes.c_char_p, [_VP]),
     "ibm_profile_enable": (_I, [_VP, _I]),
     "ibm_profile_read": (_I, [_VP, ctypes.c_char_p, _PD, ctypes.POINTER(_I)]),
+    "ibm_profile_work": (_I, [_VP, ctypes.c_char_p, _PD]),
     "ibm_spread": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
     "ibm_spread_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                              ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
@@ -127,6 +128,19 @@ _SIGS = {
     "ibm_rbf_eval": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_I), ctypes.POINTER(_VP),
                           ctypes.POINTER(_VP)]),
     "ibm_surface_geometry": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "ibm_axpby": (_I, [_VP, ctypes.c_int64, _D, _VP, _D, _VP]),
+    "ibm_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _I, _D, _VP, _D, _VP]),
+    "ibm_mg_solve_fixed": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
+    "ibm_mg_solve_atol": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, ctypes.POINTER(_I), _PD]),
+    "ibm_solver_create": (_I, [_VP, ctypes.POINTER(ibm_dimension), _I, _I, _VP, _I, _VP, ctypes.POINTER(_VP)]),
+    "ibm_solver_destroy": (_I, [_VP]),
+    "ibm_solver_step": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_solver_step_fixed": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_solver_capture": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_solver_replay": (_I, [_VP, _VP, _VP]),
+    "ibm_solver_info": (_I, [_VP, _VP, _VP]),
+    "ibm_solver_cells": (_I, [_VP, _I, _VP]),
+    "ibm_solver_grids": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(ibm_grid)]),
 }
 
 
@@ -262,6 +276,12 @@ class Context:
         k = ctypes.c_int()
         _check(lib().ibm_profile_read(self.handle, name.encode(), ctypes.byref(t), ctypes.byref(k)), self)
         return t.value, k.value
+
+    def profile_work(self, name):
+        """Summed algorithmic bytes (flops for rbf_gemm) of a kernel name's recorded launches."""
+        w = ctypes.c_double()
+        _check(lib().ibm_profile_work(self.handle, name.encode(), ctypes.byref(w)), self)
+        return w.value
 
     def close(self):
         if self.handle:
@@ -415,15 +435,29 @@ class Multigrid:
         _check(lib().ibm_mg_vcycle(self.ctx.handle, self.handle, _ptr(b), _ptr(x)), self.ctx)
         return x
 
-    def solve(self, b, rtol=1e-10, maxit=100, x=None):
+    def solve(self, b, rtol=1e-10, maxit=100, x=None, atol=0.0):
         import torch
         if x is None:
             x = torch.empty_like(b)
         it = ctypes.c_int()
         rr = ctypes.c_double()
-        _check(lib().ibm_mg_solve(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, maxit,
-                                  ctypes.byref(it), ctypes.byref(rr)), self.ctx)
+        if atol > 0:
+            _check(lib().ibm_mg_solve_atol(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, atol, maxit,
+                                           ctypes.byref(it), ctypes.byref(rr)), self.ctx)
+        else:
+            _check(lib().ibm_mg_solve(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, maxit,
+                                      ctypes.byref(it), ctypes.byref(rr)), self.ctx)
         return x, it.value, rr.value
+
+    def solve_fixed(self, b, iters, x=None):
+        """Exactly `iters` cycles, no host synchronisation; returns (x, relres
+        as a one-element device tensor)."""
+        import torch
+        if x is None:
+            x = torch.empty_like(b)
+        rr = torch.zeros(1, dtype=torch.float64, device=b.device)
+        _check(lib().ibm_mg_solve_fixed(self.ctx.handle, self.handle, _ptr(b), _ptr(x), iters, _ptr(rr)), self.ctx)
+        return x, rr
 
     def close(self):
         if self.handle:
@@ -562,3 +596,172 @@ def surface_geometry(ctx, ns, nc, y, t1, t2, w):
     _check(lib().ibm_surface_geometry(ctx.handle, ns, nc, _ptr(y), _ptr(t1), _ptr(t2), _ptr(w), _ptr(normal),
                                       _ptr(sigma), _ptr(area), _ptr(volume)), ctx)
     return normal, sigma, area, volume
+
+
+def boundary_apply(ctx, grid, view, side, y, value=0.0, values=None, alpha=1.0, order=2):
+    """y += alpha * fd::{lower,upper}_boundary(grid, view, order) g (side 0 / 1)."""
+    _check(lib().ibm_boundary_apply(ctx.handle, ctypes.byref(grid), view, side, order, alpha,
+                                    _ptr(values) if values is not None else None, value, _ptr(y)), ctx)
+    return y
+
+
+# ----------------------------------------------------------------------
+# Immersed-boundary timestep
+# ----------------------------------------------------------------------
+MAX_CELL_TYPES = 8
+
+
+class ibm_solver_params(ctypes.Structure):
+    _fields_ = [("timestep", _D), ("diffusivity", _D), ("density", _D), ("tolerance", _D),
+                ("delta", _I), ("precision", _I), ("max_iterations", _I), ("reserved", _I)]
+
+
+class ibm_cell_type(ctypes.Structure):
+    _fields_ = [("ops", _VP), ("data_sites", _I), ("sample_sites", _I), ("cells", _I), ("reserved", _I),
+                ("weights", _VP), ("tether", _VP), ("stiffness", _D)]
+
+
+class ibm_state(ctypes.Structure):
+    _fields_ = [("t", _D), ("u", _VP * 3), ("p", _VP), ("x", _VP * MAX_CELL_TYPES)]
+
+
+class ibm_step_info(ctypes.Structure):
+    _fields_ = [("iterations", _I * 4), ("residual", _D * 4), ("divergence", _D)]
+
+
+class ibm_cell_view(ctypes.Structure):
+    _fields_ = [("positions", _VP), ("tangents", _VP * 2), ("forces", _VP), ("normals", _VP), ("sigma", _VP),
+                ("area", _VP), ("volume", _VP), ("velocity", _VP)]
+
+
+_hip = None
+
+
+def _copy_device(dst, src_ptr):
+    """Device-to-device copy of dst.numel() elements from a raw pointer."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = _I
+        _hip.hipMemcpy.argtypes = [_VP, _VP, ctypes.c_size_t, _I]
+    import torch
+    torch.cuda.synchronize()
+    rc = _hip.hipMemcpy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src_ptr),
+                        dst.numel() * dst.element_size(), 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return dst
+
+
+class State:
+    """ib::state: time, MAC velocity, centred pressure and per-type data sites
+    (torch float64 CUDA tensors owned here)."""
+
+    def __init__(self, u, p, xs, t=0.0):
+        self.u, self.p, self.xs = list(u), p, list(xs)
+        self.c = ibm_state()
+        self.c.t = t
+        for i in range(3):
+            self.c.u[i] = self.u[i].data_ptr()
+        self.c.p = p.data_ptr()
+        for k, x in enumerate(self.xs):
+            self.c.x[k] = x.data_ptr()
+
+    @property
+    def t(self):
+        return self.c.t
+
+    def clone(self):
+        return State([a.clone() for a in self.u], self.p.clone(), [x.clone() for x in self.xs], self.c.t)
+
+
+class Solver:
+    """The immersed-boundary timestep (ibm_solver_*).
+
+    cell_types: list of dicts with keys ops (RbfOperator with 3 operators:
+    position, d/dtheta, d/dphi), cells, weights (CUDA tensor, ns),
+    tether (CUDA tensor, ns x 3 cells), stiffness."""
+
+    def __init__(self, ctx, domain, refinement, timestep, diffusivity, density=1.0, tolerance=1e-10,
+                 delta="cosine", precision=64, max_iterations=100, cell_types=()):
+        self.ctx = ctx
+        par = ibm_solver_params(timestep, diffusivity, density, tolerance, DELTA.get(delta, delta), precision,
+                                max_iterations, 0)
+        n = len(cell_types)
+        arr = (ibm_cell_type * max(n, 1))()
+        self._keep = []
+        for k, c in enumerate(cell_types):
+            op = c["ops"]
+            arr[k] = ibm_cell_type(op.handle.value, op.K, op.M, c["cells"], 0, c["weights"].data_ptr(),
+                                   c["tether"].data_ptr(), c["stiffness"])
+            self._keep.append(c)
+        dims, nd = _dims(domain)
+        h = _VP()
+        _check(lib().ibm_solver_create(ctx.handle, dims, nd, refinement, ctypes.byref(par), n, arr,
+                                       ctypes.byref(h)), ctx)
+        self.handle = h
+        self.types = list(cell_types)
+
+    def grids(self):
+        g = (ibm_grid * 3)()
+        pg = ibm_grid()
+        _check(lib().ibm_solver_grids(self.handle, g, ctypes.byref(pg)))
+        return g, pg
+
+    def zero_state(self, xs):
+        import torch
+        g, pg = self.grids()
+        u = [torch.zeros(g[i].size, dtype=torch.float64, device="cuda") for i in range(3)]
+        p = torch.zeros(pg.size, dtype=torch.float64, device="cuda")
+        return State(u, p, xs)
+
+    def _bnd(self, boundary):
+        return None if boundary is None else _ptrs(boundary)
+
+    def step(self, st, boundary=None):
+        _check(lib().ibm_solver_step(self.ctx.handle, self.handle, ctypes.byref(st.c), self._bnd(boundary)), self.ctx)
+
+    def step_fixed(self, st, boundary=None):
+        _check(lib().ibm_solver_step_fixed(self.ctx.handle, self.handle, ctypes.byref(st.c), self._bnd(boundary)),
+               self.ctx)
+
+    def capture(self, st, boundary=None):
+        _check(lib().ibm_solver_capture(self.ctx.handle, self.handle, ctypes.byref(st.c), self._bnd(boundary)),
+               self.ctx)
+
+    def replay(self, st):
+        _check(lib().ibm_solver_replay(self.ctx.handle, self.handle, ctypes.byref(st.c)), self.ctx)
+
+    def info(self):
+        i = ibm_step_info()
+        _check(lib().ibm_solver_info(self.ctx.handle, self.handle, ctypes.byref(i)), self.ctx)
+        return {"iterations": list(i.iterations), "residual": list(i.residual), "divergence": i.divergence}
+
+    def cells(self, k):
+        """Copies of a cell type's positions (3, nc, ns), tangents, forces,
+        normals, sigma (nc, ns), area, volume (nc) and data-site velocity."""
+        import torch
+        v = ibm_cell_view()
+        _check(lib().ibm_solver_cells(self.handle, k, ctypes.byref(v)))
+        c = self.types[k]
+        op = c["ops"]
+        nc, ns, nd = c["cells"], op.M, op.K
+
+        def get(ptr, shape):
+            return _copy_device(torch.empty(shape, dtype=torch.float64, device="cuda"), ptr)
+
+        return {"positions": get(v.positions, (3, nc, ns)), "tangents": [get(v.tangents[j], (3, nc, ns)) for j in range(2)],
+                "forces": get(v.forces, (3, nc, ns)), "normals": get(v.normals, (3, nc, ns)),
+                "sigma": get(v.sigma, (nc, ns)), "area": get(v.area, (nc,)), "volume": get(v.volume, (nc,)),
+                "velocity": get(v.velocity, (3, nc, nd))}
+
+    def close(self):
+        if self.handle:
+            lib().ibm_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

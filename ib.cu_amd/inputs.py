@@ -135,14 +135,13 @@ def platelet_reference(ns):
     return platelet_shape(th, ph)
 
 
-def layered_cells(Lx, Ly, Lz, n_rbc, n_plt, ns_rbc, ns_plt, seed):
-    """Seeded non-overlapping placement: RBC layers (discs normal to y) on a
-    4x4-per-layer lattice with small jitter/tilt, platelets on a 6x6 lattice
-    in the gaps between RBC layers.  Returns (rbc_positions, plt_positions,
-    poses) with positions as lists of 3 x ns arrays."""
+def cell_poses(Lx, Ly, Lz, n_rbc, n_plt, seed):
+    """Seeded non-overlapping poses: RBC layers (discs normal to y) on a
+    per-layer lattice of 8 um slots with small jitter and tilt, platelets on a
+    5 um lattice in the gaps between RBC layers.  Returns (rbc_poses,
+    plt_poses), each a list of (rotations, centre) with rotations a list of
+    (angle, axis) applied in order."""
     rng = SplitMix64(seed)
-    rbc_ref = rbc_reference(ns_rbc)
-    plt_ref = platelet_reference(ns_plt)
     per_layer_x = max(int(Lx / (8.0 * UM)), 1)
     per_layer_z = max(int(Lz / (8.0 * UM)), 1)
     per_layer = per_layer_x * per_layer_z
@@ -158,9 +157,7 @@ def layered_cells(Lx, Ly, Lz, n_rbc, n_plt, ns_rbc, ns_plt, seed):
         cy = (layer + 0.5) * dy + (u[2] - 0.5) * 0.1 * UM
         spin = 2 * math.pi * u[3]
         tilt = math.radians(2.0) * (u[4] - 0.5)
-        p = rotate(rbc_ref, spin, (0.0, 1.0, 0.0))
-        p = rotate(p, tilt, (1.0, 0.0, 0.0))
-        rbcs.append(p + np.array([cx, cy, cz])[:, None])
+        rbcs.append(([(spin, (0.0, 1.0, 0.0)), (tilt, (1.0, 0.0, 0.0))], np.array([cx, cy, cz])))
     plts = []
     gaps = max(layers - 1, 1)
     nx = max(int(Lx / (5.0 * UM)), 1)
@@ -173,9 +170,26 @@ def layered_cells(Lx, Ly, Lz, n_rbc, n_plt, ns_rbc, ns_plt, seed):
         cx = (ix + 0.5) * Lx / nx + (u[0] - 0.5) * 0.5 * UM
         cz = (iz + 0.5) * Lz / nz + (u[1] - 0.5) * 0.5 * UM
         cy = (gap + 1) * dy if layers > 1 else 0.5 * Ly
-        p = rotate(plt_ref, 2 * math.pi * u[2], (0.0, 1.0, 0.0))
-        plts.append(p + np.array([cx, cy, cz])[:, None])
+        plts.append(([(2 * math.pi * u[2], (0.0, 1.0, 0.0))], np.array([cx, cy, cz])))
     return rbcs, plts
+
+
+def posed(points, pose, scale=1.0):
+    """Applies a pose (rotations, centre) to 3 x n reference points, after
+    scaling them about the origin."""
+    p = scale * points
+    for angle, axis in pose[0]:
+        p = rotate(p, angle, axis)
+    return p + pose[1][:, None]
+
+
+def layered_cells(Lx, Ly, Lz, n_rbc, n_plt, ns_rbc, ns_plt, seed):
+    """Sample-site positions (lists of 3 x ns arrays) of the cell_poses
+    placement."""
+    rp, pp = cell_poses(Lx, Ly, Lz, n_rbc, n_plt, seed)
+    rbc_ref = rbc_reference(ns_rbc)
+    plt_ref = platelet_reference(ns_plt)
+    return [posed(rbc_ref, q) for q in rp], [posed(plt_ref, q) for q in pp]
 
 
 def config_c4(seed=4, ns_rbc=8832, ns_plt=1552):

@@ -26,8 +26,8 @@ extern "C" {
 #endif
 
 /* ------------------------------------------------------------------ */
-/* Status codes.  The C++ mirror headers map these to the reference's   */
-/* exception types (fd::bad_grid_points, linalg::mismatch, ...).        */
+/* Status codes.  include/ibm.hpp turns them into exceptions           */
+/* (ibm::error, ins::too_small).                                        */
 /* ------------------------------------------------------------------ */
 typedef enum ibm_status {
     IBM_OK = 0,
@@ -139,6 +139,10 @@ const char* ibm_last_error(const ibm_ctx* ctx);
  * event time and launch count of one kernel name. */
 ibm_status ibm_profile_enable(ibm_ctx* ctx, int on);
 ibm_status ibm_profile_read(ibm_ctx* ctx, const char* name, double* total_ms, int* launches);
+/* Summed algorithmic work of the recorded launches of one kernel name:
+ * compulsory HBM bytes (operands read once, results written once) for the
+ * bandwidth-bound kernels, flops (2 M N K) for "rbf_gemm". */
+ibm_status ibm_profile_work(ibm_ctx* ctx, const char* name, double* total);
 
 /* ------------------------------------------------------------------ */
 /* Immersed-boundary coupling (FP64)                                    */
@@ -311,6 +315,109 @@ ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int*
  * per cell the area sum(sigma) and volume sum(x . n sigma) / 3. */
 ibm_status ibm_surface_geometry(ibm_ctx* ctx, int ns, int nc, const double* y, const double* t1, const double* t2,
                                 const double* w, double* normal, double* sigma, double* area, double* volume);
+
+/* y = a x + b y over n doubles (linalg::axpy / scal, linalg/blas.h). */
+ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, double b, double* y);
+
+/* ---------------------------------------------------------------------
+ * Boundary data and fixed-count solves
+ * ------------------------------------------------------------------- */
+/* y += alpha * B g with B = fd::lower_boundary (side 0) / fd::upper_boundary
+ * (side 1) of grid g along `view` with correction order 0 or 2
+ * (fd/boundary_ops.h:21-71): the face row's ghost-cell boundary coefficient
+ * times resolution^order, kron the order-corrected identity of the other
+ * axes.  `values` holds one value per face point (other axes, x fastest),
+ * or is null for the constant `value`.  A periodic `view` adds nothing. */
+ibm_status ibm_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int view, int side, int order, double alpha,
+                              const double* values, double value, double* y);
+
+/* ibm_mg_solve stopping at ||b - A x|| <= max(rtol ||b||, atol): for right-hand
+ * sides whose norm is itself rounding noise (the divergence of a field that
+ * is already divergence-free), atol carries the scale of the problem. */
+ibm_status ibm_mg_solve_atol(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, double rtol, double atol,
+                             int maxit, int* iters, double* relres);
+
+/* ibm_mg_solve with exactly `iters` defect-correction cycles and no host
+ * synchronisation (recordable in a HIP graph).  When `relres` (a device
+ * double) is given, ||b - A x|| / ||b|| is written there. */
+ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, int iters, double* relres);
+
+/* ---------------------------------------------------------------------
+ * Immersed-boundary timestep (the ib::solver of test/ib-whole.cc:226-275;
+ * the reference does not ship its implementation, so the scheme is this
+ * library's: geometry by RBF evaluation, tether forces, spread, advection,
+ * ins::diffusion-form Helmholtz solves, projection onto divergence-free
+ * fields with a Neumann pressure Poisson solve, interpolation at the data
+ * sites and X += dt U)
+ * ------------------------------------------------------------------- */
+#define IBM_MAX_CELL_TYPES 8
+
+typedef struct ibm_solver ibm_solver;
+
+typedef struct {
+    double timestep;     /* dt (library time units) */
+    double diffusivity;  /* nu = mu / rho (ins/simulation.h) */
+    double density;      /* rho: spread forces are divided by it */
+    double tolerance;    /* relative residual of every Helmholtz / Poisson solve */
+    int delta;           /* ibm_delta of spread and interpolate */
+    int precision;       /* multigrid V-cycles in 64 or 32 bit; defect correction is FP64 */
+    int max_iterations;  /* per solve */
+    int reserved;
+} ibm_solver_params;
+
+typedef struct {
+    ibm_rbf* ops;            /* 3 operators data -> sample: position, d/dtheta, d/dphi */
+    int data_sites, sample_sites, cells;
+    int reserved;
+    const double* weights;   /* sample-site quadrature weights (device; copied) */
+    const double* tether;    /* reference sample positions, ns x 3 cells (device; copied) */
+    double stiffness;        /* tether force f = -stiffness (y - tether) */
+} ibm_cell_type;
+
+/* ib::state (test/ib-whole.cc:159,180-181): caller-owned device buffers. */
+typedef struct {
+    double t;
+    double* u[3];                    /* MAC velocity components */
+    double* p;                       /* pressure on the cell centres */
+    double* x[IBM_MAX_CELL_TYPES];   /* data sites per cell type: nd x 3 cells, column comp * cells + cell */
+} ibm_state;
+
+typedef struct {
+    int iterations[4];    /* V-cycles of the u, v, w Helmholtz solves and the pressure solve */
+    double residual[4];   /* their final relative residuals (FP64) */
+    double divergence;    /* max |D u| after projection (FP64) */
+} ibm_step_info;
+
+/* Device views of a cell type's per-step data (owned by the solver). */
+typedef struct {
+    const double* positions;     /* ns x 3 cells */
+    const double* tangents[2];
+    const double* forces;
+    const double* normals;
+    const double* sigma;         /* ns x cells */
+    const double* area;          /* cells */
+    const double* volume;        /* cells */
+    const double* velocity;      /* interpolated at the data sites: nd x 3 cells */
+} ibm_cell_view;
+
+ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndims, int refinement,
+                             const ibm_solver_params* params, int ntypes, const ibm_cell_type* types,
+                             ibm_solver** out);
+ibm_status ibm_solver_destroy(ibm_solver* s);
+/* One step; V-cycle counts chosen by the residual (host reads). `boundary`
+ * is null or 3 device vectors of wall data on the velocity grids (e.g. from
+ * ibm_boundary_apply), the `ub` of test/ib-whole.cc:250,275. */
+ibm_status ibm_solver_step(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* boundary);
+/* One step with the V-cycle counts fixed (the most an adaptive step has
+ * needed, plus one) and no host synchronisation. */
+ibm_status ibm_solver_step_fixed(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* boundary);
+/* Records ibm_solver_step_fixed on the state's buffers as a HIP graph;
+ * ibm_solver_replay launches it (the state must hold the same buffers). */
+ibm_status ibm_solver_capture(ibm_ctx* ctx, ibm_solver* s, const ibm_state* st, const double* const* boundary);
+ibm_status ibm_solver_replay(ibm_ctx* ctx, ibm_solver* s, ibm_state* st);
+ibm_status ibm_solver_info(ibm_ctx* ctx, ibm_solver* s, ibm_step_info* out);
+ibm_status ibm_solver_cells(ibm_solver* s, int type, ibm_cell_view* out);
+ibm_status ibm_solver_grids(ibm_solver* s, ibm_grid* velocity /* [3] */, ibm_grid* pressure);
 
 #ifdef __cplusplus
 } /* extern "C" */

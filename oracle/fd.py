@@ -71,6 +71,23 @@ def id1(axis, order):
     return sp.diags(d, 0, format="csr")
 
 
+def boundary1(axis, lower):
+    """fd::__1::boundary (fd/boundary_ops.h:16-27): points x solid, one entry
+    (the ghost-cell boundary coefficient) in the first / last row."""
+    rows = axis.points
+    if not axis.solid:
+        return sp.csr_matrix((rows, 0))
+    return single_entry(rows, 1, 1.0, not lower, not lower, axis.boundary[0 if lower else 1])
+
+
+def boundary_op(axes, view, lower, order):
+    """fd::lower_boundary / fd::upper_boundary (fd/boundary_ops.h:29-71)."""
+    mats = []
+    for j, a in enumerate(axes):
+        mats.append(a.resolution ** order * boundary1(a, lower) if j == view else id1(a, order))
+    return kron_product(mats)
+
+
 def shifted_axis(axis):
     """The discretization with alignment shifted by 1/2 (fd/differential.h:49-52)."""
     class A:

@@ -74,3 +74,12 @@ def test_grid_coarsen_halves_refinement(ibm):
     assert c.points == (64, 128, 64)
     assert c.refinement == 64
     assert list(c.axis[1].interior) == list(g.axis[1].interior)
+
+
+def test_cpp_header_is_plain_cpp():
+    """include/ibm.hpp compiles as host C++17 with no HIP headers."""
+    import subprocess
+    src = '#include "ibm.hpp"\nint main() { fd::mac m{32}; return m.refinement() == 32 ? 0 : 1; }\n'
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

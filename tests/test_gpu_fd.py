@@ -168,3 +168,25 @@ def test_c3_helmholtz_mixed_precision_full_size(ibm, ctx):
     assert float(r.norm()) <= 1.01e-10 * float(b.norm())
     assert it <= 20
     mg.close()
+
+
+def test_fixed_and_floor_solves(ibm, orc, ctx):
+    """ibm_mg_solve_fixed reproduces the adaptive solve with the same cycle
+    count bitwise; ibm_mg_solve_atol accepts a right-hand side that is pure
+    rounding noise (the adaptive relative test alone cannot)."""
+    import torch
+    dom, g, af = _mg_case(ibm, orc, "p", "neumann")
+    mg = ibm.Multigrid(ctx, g, ibm.LAPLACIAN, precision=32)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(g.size)
+    b -= b.mean()
+    x, it, rr = mg.solve(_t(b), rtol=1e-10, maxit=80)
+    xf, rrf = mg.solve_fixed(_t(b), it)
+    assert torch.equal(x, xf)
+    assert abs(float(rrf) - rr) <= 1e-12 * rr + 1e-300
+    noise = _t(1e-17 * rng.standard_normal(g.size))
+    with pytest.raises(ibm.IbmError):
+        mg.solve(noise, rtol=1e-12, maxit=5)
+    x, it, rr = mg.solve(noise, rtol=1e-12, maxit=5, atol=1e-12)
+    assert it <= 5
+    mg.close()

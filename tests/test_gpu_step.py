@@ -1,0 +1,172 @@
+"""GPU parity of the immersed-boundary timestep (ibm_solver_*) against the
+oracle's composition of the same scheme (oracle/step.py), plus the
+boundary-data operator against its assembled CSR form.
+
+Bars: after three FP64 steps, velocity, pressure and data-site positions
+within 1e-8 relative (max norm) of the oracle (solves to rtol 1e-12 on both
+sides; sparse direct solves in the oracle); the mixed form (FP32 V-cycles
+inside FP64 defect correction) within 1e-8 of the FP64 form; a replayed HIP
+graph bitwise equal to the same fixed-count step run eagerly."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, _load, load_orc_rbf
+import step_cases as C
+
+pytestmark = pytest.mark.gpu
+
+fd = _load("orc_fd", os.path.join(ROOT, "oracle", "fd.py"))
+step_mod = _load("orc_step", os.path.join(ROOT, "oracle", "step.py"))
+O = load_orc_rbf()
+REF = 16
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def types():
+    return C.cell_types()
+
+
+def _solver(ibm, ctx, types, walls, precision=64, rbf="f64", tol=1e-12):
+    ops = [ibm.RbfOperator(ctx, rbf, [_t(T["E"]), _t(T["Et"]), _t(T["Ep"])], T["nd"]) for T in types]
+    ct = [{"ops": op, "cells": T["nc"], "weights": _t(T["w"]), "tether": _t(T["ref"].reshape(-1)),
+           "stiffness": T["k"]} for op, T in zip(ops, types)]
+    s = ibm.Solver(ctx, C.domain(ibm, walls), REF, C.DT, C.NU, density=C.RHO, tolerance=tol, precision=precision,
+                   max_iterations=60, cell_types=ct)
+    st = s.zero_state([_t(T["X"]) for T in types])
+    return s, st, ops
+
+
+def _oracle(orc, types, walls):
+    return step_mod.Step(orc, fd, O, C.domain_spec(walls), REF, C.DT, C.NU, C.RHO, "cosine", types)
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_three_steps_match_oracle(ibm, orc, ctx, types, walls):
+    s, st, ops = _solver(ibm, ctx, types, walls)
+    ost = _oracle(orc, types, walls)
+    u = [np.zeros(st.u[i].numel()) for i in range(3)]
+    p = np.zeros(st.p.numel())
+    xs = [T["X"].copy() for T in types]
+    for k in range(3):
+        s.step(st)
+        u, p, xs, geo = ost.step(u, p, xs)
+        info = s.info()
+        print("step", k, info)
+        # Helmholtz solves to rtol; the projection may stop at its absolute
+        # floor tol * 2 r ||u*|| / dt, a little above rtol ||b||.
+        assert max(info["residual"][:3]) <= 1e-12 and info["residual"][3] <= 1e-11
+        umax = max(np.abs(c).max() for c in u)
+        assert info["divergence"] <= 1e-9 * umax * 20
+    errs = [_rel(st.u[i].cpu().numpy(), u[i]) for i in range(3)]
+    errs.append(_rel(st.p.cpu().numpy(), p))
+    errs += [_rel(st.xs[j].cpu().numpy(), xs[j]) for j in range(len(types))]
+    print("relative errors u, v, w, p, x...", errs)
+    assert max(errs) < 1e-8
+    # Geometry of the last step against the oracle's.
+    for j, (y, t1, t2) in enumerate(geo):
+        c = s.cells(j)
+        assert _rel(c["positions"].cpu().numpy(), y) < 1e-12
+        rn, rs, ra, rv = O.geometry(y, t1, t2, types[j]["w"])
+        assert _rel(c["area"].cpu().numpy(), ra) < 1e-12
+        assert _rel(c["volume"].cpu().numpy(), rv) < 1e-12
+    s.close()
+
+
+def test_mixed_precision_matches_fp64(ibm, ctx, types):
+    a, sa, _ = _solver(ibm, ctx, types, True, precision=64, tol=1e-11)
+    b, sb, _ = _solver(ibm, ctx, types, True, precision=32, tol=1e-11)
+    for _ in range(3):
+        a.step(sa)
+        b.step(sb)
+    errs = [_rel(sb.u[i].cpu().numpy(), sa.u[i].cpu().numpy()) for i in range(3)]
+    errs.append(_rel(sb.p.cpu().numpy(), sa.p.cpu().numpy()))
+    errs += [_rel(x.cpu().numpy(), y.cpu().numpy()) for x, y in zip(sb.xs, sa.xs)]
+    print("mixed vs FP64 relative errors", errs, b.info())
+    assert max(errs) < 1e-8
+    a.close()
+    b.close()
+
+
+def test_graph_replay_matches_fixed_steps(ibm, ctx, types):
+    import torch
+    side = torch.cuda.Stream()                   # capture needs a created stream
+    gctx = ibm.Context(0, stream=side)
+    s, st, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10)
+    torch.cuda.synchronize()
+    s.step(st)                                   # calibrates the V-cycle counts
+    torch.cuda.synchronize()
+    eager = st.clone()
+    torch.cuda.synchronize()
+    s.capture(st)
+    for k in range(3):
+        s.replay(st)
+        torch.cuda.synchronize()
+        s.step_fixed(eager)
+        torch.cuda.synchronize()
+        # unrelated eager work between replays
+        junk = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        for i in range(3):
+            assert torch.equal(st.u[i], eager.u[i]), (k, i)
+        assert torch.equal(st.p, eager.p)
+        for x, y in zip(st.xs, eager.xs):
+            assert torch.equal(x, y)
+        info = s.info()
+        assert max(info["residual"]) < 1e-9, info
+    assert abs(st.t - eager.t) == 0
+    s.close()
+    gctx.close()
+
+
+@pytest.mark.parametrize("order", [0, 2])
+def test_boundary_apply_matches_csr(ibm, orc, ctx, order):
+    dom = C.domain(ibm, True)
+    spec = C.domain_spec(True)
+    rng = np.random.default_rng(5)
+    for comp, align in enumerate(step_mod.MAC):
+        g = ibm.grid_make(dom, REF, align)
+        axes = [orc.grid_axes(spec, REF, align)[j] for j in range(3)]
+        for side in (0, 1):
+            B = fd.boundary_op(axes, 1, side == 0, order)
+            vals = rng.standard_normal(B.shape[1])
+            y0 = rng.standard_normal(g.size)
+            y = ibm.boundary_apply(ctx, g, 1, side, _t(y0.copy()), values=_t(vals), alpha=0.5, order=order)
+            assert _rel(y.cpu().numpy(), y0 + 0.5 * (B @ vals)) < 1e-14
+            y = ibm.boundary_apply(ctx, g, 1, side, _t(y0.copy()), value=2.0, order=order)
+            assert _rel(y.cpu().numpy(), y0 + B @ np.full(B.shape[1], 2.0)) < 1e-14
+        # periodic view: no columns, nothing added
+        y = ibm.boundary_apply(ctx, g, 0, 1, _t(np.zeros(g.size)), value=1.0, order=order)
+        assert float(y.abs().max()) == 0
+
+
+def test_sheared_channel_matches_oracle(ibm, orc, ctx):
+    """Moving top wall (the boundary data of test/ib-whole.cc:129-152), no
+    cells: three steps against the oracle."""
+    import torch
+    s = ibm.Solver(ctx, C.domain(ibm, True), REF, C.DT, C.NU, density=C.RHO, tolerance=1e-12, precision=32)
+    g, pg = s.grids()
+    st = s.zero_state([])
+    wall = 0.1
+    ub = [torch.zeros(g[i].size, dtype=torch.float64, device="cuda") for i in range(3)]
+    ibm.boundary_apply(ctx, g[0], 1, 1, ub[0], value=wall, order=2)
+    ost = _oracle(orc, [], True)
+    ubn = [c.cpu().numpy() for c in ub]
+    u = [np.zeros(g[i].size) for i in range(3)]
+    p = np.zeros(pg.size)
+    for _ in range(3):
+        s.step(st, boundary=ub)
+        u, p, _, _ = ost.step(u, p, [], boundary=ubn)
+    assert _rel(st.u[0].cpu().numpy(), u[0]) < 1e-8
+    assert float(st.u[0].max()) > 0
+    s.close()
