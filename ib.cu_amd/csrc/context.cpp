@@ -68,6 +68,8 @@ prof_scope::~prof_scope()
 
 extern "C" {
 
+int ibm_abi_version(void) { return IBM_ABI_VERSION; }
+
 ibm_status ibm_profile_enable(ibm_ctx* ctx, int on)
 {
     if (!ctx) return IBM_ERR_ARG;

@@ -178,9 +178,23 @@ static ibm_status norm2(ibm_ctx* ctx, ibm_mg* m, size_t n, const double* x, doub
     return IBM_OK;
 }
 
+// Initial residual: r = b from x = 0, or r = b - A x from the caller's x
+// (warm start; on a singular operator x is made mean-free first).
+static ibm_status start(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, bool warm)
+{
+    const mg_level& L0 = m->lv[0];
+    const size_t n = L0.n;
+    if (!warm) {
+        IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
+        return launch_convert<double, double>(ctx, n, b, m->r);
+    }
+    if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
+    return launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r);
+}
+
 template <typename T>
 static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, double atol,
-                             int maxit, int* iters, double* relres)
+                             int maxit, bool warm, int* iters, double* relres)
 {
     const mg_level& L0 = m->lv[0];
     const size_t n = L0.n;
@@ -195,12 +209,12 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
     }
     double nb;
     IBM_TRY(norm2(ctx, m, n, b, &nb));
-    IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
-    IBM_TRY(launch_convert<double, double>(ctx, n, b, m->r));
+    IBM_TRY(start(ctx, m, b, x, warm));
     int it = 0;
     double nr = nb;
+    if (warm) IBM_TRY(norm2(ctx, m, n, m->r, &nr));
     const double stop = std::max(rtol * nb, atol);
-    while (nb > stop && it < maxit) {
+    while (nr > stop && it < maxit) {
         IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
         IBM_TRY(vcycle<T>(ctx, m, 0));
         IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
@@ -221,7 +235,8 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
 // synchronisation (so it can be recorded in a HIP graph); the final
 // relative residual ||b - A x|| / ||b|| is left in device memory.
 template <typename T>
-static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, double* relres_dev)
+static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, bool warm,
+                              double* relres_dev)
 {
     const mg_level& L0 = m->lv[0];
     const size_t n = L0.n;
@@ -231,8 +246,7 @@ static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* 
         IBM_TRY(launch_remove_mean<double>(ctx, n, m->rhs, m->partial, m->scalars + 1));
         b = m->rhs;
     }
-    IBM_TRY(launch_fill<double>(ctx, n, 0.0, x));
-    IBM_TRY(launch_convert<double, double>(ctx, n, b, m->r));
+    IBM_TRY(start(ctx, m, b, x, warm));
     for (int it = 0; it < iters; ++it) {
         IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
         IBM_TRY(vcycle<T>(ctx, m, 0));
@@ -424,19 +438,19 @@ ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, dou
     if (!ctx || !m) return IBM_ERR_ARG;
     if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
     if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
-    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, 0.0, maxit, iters, relres);
-    return solve_loop<float>(ctx, m, b, x, rtol, 0.0, maxit, iters, relres);
+    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, 0.0, maxit, false, iters, relres);
+    return solve_loop<float>(ctx, m, b, x, rtol, 0.0, maxit, false, iters, relres);
 }
 
 ibm_status ibm_mg_solve_atol(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, double rtol, double atol, int maxit,
-                             int* iters, double* relres)
+                             int warm, int* iters, double* relres)
 {
     if (!ctx || !m) return IBM_ERR_ARG;
     if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
     if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
     if (!(atol >= 0)) return fail(ctx, IBM_ERR_ARG, "negative absolute tolerance");
-    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, atol, maxit, iters, relres);
-    return solve_loop<float>(ctx, m, b, x, rtol, atol, maxit, iters, relres);
+    if (m->precision == 64) return solve_loop<double>(ctx, m, b, x, rtol, atol, maxit, warm != 0, iters, relres);
+    return solve_loop<float>(ctx, m, b, x, rtol, atol, maxit, warm != 0, iters, relres);
 }
 
 ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, double b, double* y)
@@ -447,14 +461,15 @@ ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, doubl
     return launch_axpby<double, double>(ctx, (size_t) n, a, x, b, y);
 }
 
-ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, double* relres)
+ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, int iters, int warm,
+                              double* relres)
 {
     if (!ctx || !m) return IBM_ERR_ARG;
     if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
     if (b == x) return fail(ctx, IBM_ERR_ARG, "b and x must not alias");
     if (iters < 0) return fail(ctx, IBM_ERR_ARG, "negative iteration count");
-    if (m->precision == 64) return solve_fixed<double>(ctx, m, b, x, iters, relres);
-    return solve_fixed<float>(ctx, m, b, x, iters, relres);
+    if (m->precision == 64) return solve_fixed<double>(ctx, m, b, x, iters, warm != 0, relres);
+    return solve_fixed<float>(ctx, m, b, x, iters, warm != 0, relres);
 }
 
 // solvers::chebpcg (solvers/chebpcg.h:9-23 -> solvers/pcg.h:11-44): CG

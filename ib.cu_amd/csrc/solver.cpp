@@ -22,6 +22,14 @@
 //   7. FP64 divergence check max |D u|;
 //   8. U = interpolate(u) at the data sites (ib/interpolate.h), X += dt U.
 //
+// With a force scale (params.fscale > 0) the step size follows ins::tracker
+// (ins/tracker.h:13-100): e = floor(log2(k_max / k_min)) with
+// k_max = sqrt(fscale / (|f| + 1e-3)) / 4 and |f| = sqrt(sum over cell types
+// of max|f|^2), dt = k_min 2^min(4, e over the last 16 steps); e < 0 is
+// IBM_ERR_DT_TOO_SMALL.  (The reference's retry loop compares the newest
+// exponent with a minimum that already includes it, so it never runs.)
+// One set of Helmholtz hierarchies is kept per step size in use.
+//
 // All work is enqueued on the context stream.  ibm_solver_step chooses the
 // number of V-cycles of each solve adaptively (host reads of residual norms);
 // ibm_solver_step_fixed / ibm_solver_capture use fixed counts (the largest
@@ -72,7 +80,11 @@ struct ibm_solver {
     ibm_solver_params par{};
     ibm_grid g[3]{}, gp{};
     size_t nu[3]{}, np = 0;
-    ibm_mg* helm[3]{};
+    ibm_mg* helm[3]{};                 // Helmholtz hierarchies of the current step size
+    ibm_mg* helm_e[5][3]{};            // per tracker exponent (adaptive step size)
+    double dt = 0;                     // current step size
+    int hist[16]{};                    // tracker exponents of the last steps
+    int hist_len = 0, hist_next = 0;
     ibm_mg* pois = nullptr;
     struct type_t {
         ibm_rbf* ops;
@@ -87,9 +99,11 @@ struct ibm_solver {
     int max_iters[4]{};                // largest adaptive iteration counts seen
     int last_iters[4]{};
     int adaptive_steps = 0;
+    int last_exponent = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ibm_state captured{};
+    double captured_dt = 0;
     std::vector<void*> owned;
 };
 
@@ -108,8 +122,14 @@ void release(ibm_solver* s)
 {
     if (s->exec) hipGraphExecDestroy(s->exec);
     if (s->graph) hipGraphDestroy(s->graph);
-    for (int i = 0; i < 3; ++i)
-        if (s->helm[i]) ibm_mg_destroy(s->helm[i]);
+    if (s->par.fscale > 0) {
+        for (int e = 0; e < 5; ++e)
+            for (int i = 0; i < 3; ++i)
+                if (s->helm_e[e][i]) ibm_mg_destroy(s->helm_e[e][i]);
+    } else {
+        for (int i = 0; i < 3; ++i)
+            if (s->helm[i]) ibm_mg_destroy(s->helm[i]);
+    }
     if (s->pois) ibm_mg_destroy(s->pois);
     hipDeviceSynchronize();
     for (void* p : s->owned) hipFree(p);
@@ -127,11 +147,44 @@ ibm_status check_state(ibm_ctx* ctx, const ibm_solver* s, const ibm_state* st)
     return IBM_OK;
 }
 
+// ins::tracker: pushes this step's exponent, selects dt and its Helmholtz
+// hierarchies (created on first use).
+ibm_status track(ibm_ctx* ctx, ibm_solver* s)
+{
+    const ibm_solver_params& P = s->par;
+    double f2 = 0;
+    for (auto& T : s->types) {
+        if (!T.nc) continue;
+        IBM_TRY(launch_max_abs(ctx, (size_t) T.ns * 3 * T.nc, T.f, s->partial, s->info + 6));
+        double m = 0;
+        IBM_HIP(ctx, hipMemcpyAsync(&m, s->info + 6, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        f2 += m * m;
+    }
+    const double fmax = std::sqrt(f2);
+    const double kmax = 0.25 * std::sqrt(P.fscale / (fmax + 1e-3));
+    const int e_new = (int) std::floor(std::log2(kmax / P.timestep));
+    s->hist[s->hist_next] = e_new;
+    s->hist_next = (s->hist_next + 1) % 16;
+    s->hist_len = std::min(s->hist_len + 1, 16);
+    int e = 4;
+    for (int i = 0; i < s->hist_len; ++i) e = std::min(e, s->hist[i]);
+    s->last_exponent = e;
+    if (e < 0) return fail(ctx, IBM_ERR_DT_TOO_SMALL, "time step bound %.3e below kmin %.3e", kmax, P.timestep);
+    s->dt = P.timestep * (double) (1 << e);
+    if (!s->helm_e[e][0]) {
+        const double lambda = s->dt * P.diffusivity / 2;
+        for (int i = 0; i < 3; ++i) IBM_TRY(ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm_e[e][i]));
+    }
+    for (int i = 0; i < 3; ++i) s->helm[i] = s->helm_e[e][i];
+    return IBM_OK;
+}
+
 // One timestep; `fixed` selects fixed V-cycle counts and no host reads.
 ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* const* ub, bool fixed)
 {
     const ibm_solver_params& P = s->par;
-    const double dt = P.timestep, nu = P.diffusivity;
+    const double nu = P.diffusivity;
     const int nt = (int) s->types.size();
 
     // 1. geometry (one launch for every cell type and operator)
@@ -162,6 +215,9 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             IBM_LAUNCH_CHECK(ctx);
         }
     }
+    // 2b. step size (ins::tracker) from the force bound
+    if (P.fscale > 0 && !fixed) IBM_TRY(track(ctx, s));
+    const double dt = s->dt;
     // 3. spread all cell types onto the MAC grid
     {
         int counts[MAX_TYPES];
@@ -192,11 +248,12 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         // rhs += dt nu L u
         IBM_TRY(launch_stencil<double>(ctx, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr, s->rhs[i]));
         if (fixed) {
-            IBM_TRY(ibm_mg_solve_fixed(ctx, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, s->info + i));
+            IBM_TRY(ibm_mg_solve_fixed(ctx, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, 1, s->info + i));
         } else {
             int it = 0;
             double rr = 0;
-            IBM_TRY(ibm_mg_solve(ctx, s->helm[i], s->rhs[i], s->du[i], P.tolerance, P.max_iterations, &it, &rr));
+            IBM_TRY(ibm_mg_solve_atol(ctx, s->helm[i], s->rhs[i], s->du[i], P.tolerance, 0.0, P.max_iterations, 1,
+                                      &it, &rr));
             s->last_iters[i] = it;
             IBM_HIP(ctx, hipMemcpyAsync(s->info + i, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -210,7 +267,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         scale_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, 1.0 / dt, s->div);
         IBM_LAUNCH_CHECK(ctx);
         if (fixed) {
-            IBM_TRY(ibm_mg_solve_fixed(ctx, s->pois, s->div, st->p, s->max_iters[3] + 1, s->info + 3));
+            IBM_TRY(ibm_mg_solve_fixed(ctx, s->pois, s->div, st->p, s->max_iters[3] + 1, 1, s->info + 3));
         } else {
             // Absolute floor: tolerance x the divergence scale of u*,
             // ||D|| ||u*|| / dt with ||D|| <= 2 / h (the right-hand side of a
@@ -228,7 +285,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             const double atol = P.tolerance * 2.0 * rmax * std::sqrt(uu) / dt;
             int it = 0;
             double rr = 0;
-            IBM_TRY(ibm_mg_solve_atol(ctx, s->pois, s->div, st->p, P.tolerance, atol, P.max_iterations, &it, &rr));
+            IBM_TRY(ibm_mg_solve_atol(ctx, s->pois, s->div, st->p, P.tolerance, atol, P.max_iterations, 1, &it, &rr));
             s->last_iters[3] = it;
             IBM_HIP(ctx, hipMemcpyAsync(s->info + 3, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -278,7 +335,8 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     if (ntypes < 0 || ntypes > IBM_MAX_CELL_TYPES || (ntypes && !types))
         return fail(ctx, IBM_ERR_ARG, "between 0 and %d cell types", IBM_MAX_CELL_TYPES);
     const ibm_solver_params& P = *params;
-    if (!(P.timestep > 0) || !(P.diffusivity >= 0) || !(P.density > 0) || !(P.tolerance > 0) || P.max_iterations < 1)
+    if (!(P.timestep > 0) || !(P.diffusivity >= 0) || !(P.density > 0) || !(P.tolerance > 0) || P.max_iterations < 1 ||
+        !(P.fscale >= 0))
         return fail(ctx, IBM_ERR_ARG, "bad solver parameters");
     if (P.precision != 64 && P.precision != 32) return fail(ctx, IBM_ERR_ARG, "precision must be 64 or 32");
     auto* s = new ibm_solver;
@@ -301,8 +359,12 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     }
     for (int i = 0; i < 3; ++i) s->nu[i] = (size_t) ibm_grid_size(&s->g[i]);
     s->np = (size_t) ibm_grid_size(&s->gp);
-    const double lambda = P.timestep * P.diffusivity / 2;
-    for (int i = 0; i < 3 && st == IBM_OK; ++i) st = ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm[i]);
+    s->dt = P.timestep;
+    if (!(P.fscale > 0)) {
+        const double lambda = P.timestep * P.diffusivity / 2;
+        for (int i = 0; i < 3 && st == IBM_OK; ++i)
+            st = ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm[i]);
+    }
     if (st == IBM_OK) st = ibm_mg_create(ctx, &s->gp, 0.0, 1.0, P.precision, &s->pois);
     for (int i = 0; i < 3 && st == IBM_OK; ++i) {
         if ((st = alloc(ctx, s, s->nu[i], &s->F[i]))) break;
@@ -358,7 +420,7 @@ ibm_status ibm_solver_step(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const dou
     if (!ctx || !s) return IBM_ERR_ARG;
     IBM_TRY(check_state(ctx, s, st));
     IBM_TRY(step_impl(ctx, s, st, boundary, false));
-    st->t += s->par.timestep;
+    st->t += s->dt;
     return IBM_OK;
 }
 
@@ -368,7 +430,7 @@ ibm_status ibm_solver_step_fixed(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, con
     IBM_TRY(check_state(ctx, s, st));
     if (!s->adaptive_steps) return fail(ctx, IBM_ERR_ARG, "run an adaptive step first to calibrate the solves");
     IBM_TRY(step_impl(ctx, s, st, boundary, true));
-    st->t += s->par.timestep;
+    st->t += s->dt;
     return IBM_OK;
 }
 
@@ -402,6 +464,7 @@ ibm_status ibm_solver_capture(ibm_ctx* ctx, ibm_solver* s, const ibm_state* st, 
     s->graph = g;
     IBM_HIP(ctx, hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0));
     s->captured = *st;
+    s->captured_dt = s->dt;
     return IBM_OK;
 }
 
@@ -413,7 +476,7 @@ ibm_status ibm_solver_replay(ibm_ctx* ctx, ibm_solver* s, ibm_state* st)
         std::memcmp(st->x, s->captured.x, sizeof(st->x)))
         return fail(ctx, IBM_ERR_ARG, "the state's buffers differ from the captured ones");
     IBM_HIP(ctx, hipGraphLaunch(s->exec, ctx->stream));
-    st->t += s->par.timestep;
+    st->t += s->captured_dt;
     return IBM_OK;
 }
 
@@ -428,6 +491,8 @@ ibm_status ibm_solver_info(ibm_ctx* ctx, ibm_solver* s, ibm_step_info* out)
         out->residual[i] = h[i];
     }
     out->divergence = h[4];
+    out->timestep = s->dt;
+    out->exponent = s->last_exponent;
     return IBM_OK;
 }
 

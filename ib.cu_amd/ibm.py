@@ -83,7 +83,10 @@ _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _PD = ctypes.POINTER(ctypes.c_double)
+ABI_VERSION = 3          # IBM_ABI_VERSION of include/ibm.h this binding matches
+
 _SIGS = {
+    "ibm_abi_version": (_I, []),
     "ibm_domain_unit": (_I, [ctypes.POINTER(ibm_dimension), _I, _PD]),
     "ibm_grid_make": (_I, [ctypes.POINTER(ibm_dimension), _I, _I, _PD, ctypes.POINTER(ibm_grid)]),
     "ibm_mac_grids": (_I, [ctypes.POINTER(ibm_dimension), _I, _I, ctypes.POINTER(ibm_grid)]),
@@ -130,8 +133,8 @@ _SIGS = {
     "ibm_surface_geometry": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "ibm_axpby": (_I, [_VP, ctypes.c_int64, _D, _VP, _D, _VP]),
     "ibm_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _I, _D, _VP, _D, _VP]),
-    "ibm_mg_solve_fixed": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
-    "ibm_mg_solve_atol": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, ctypes.POINTER(_I), _PD]),
+    "ibm_mg_solve_fixed": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "ibm_mg_solve_atol": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(_I), _PD]),
     "ibm_solver_create": (_I, [_VP, ctypes.POINTER(ibm_dimension), _I, _I, _VP, _I, _VP, ctypes.POINTER(_VP)]),
     "ibm_solver_destroy": (_I, [_VP]),
     "ibm_solver_step": (_I, [_VP, _VP, _VP, _VP]),
@@ -155,6 +158,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.ibm_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.ibm_abi_version()}, this binding expects "
+                               f"{ABI_VERSION}: rebuild with __graft_entry__.build()")
         _lib = L
     return _lib
 
@@ -435,28 +441,30 @@ class Multigrid:
         _check(lib().ibm_mg_vcycle(self.ctx.handle, self.handle, _ptr(b), _ptr(x)), self.ctx)
         return x
 
-    def solve(self, b, rtol=1e-10, maxit=100, x=None, atol=0.0):
+    def solve(self, b, rtol=1e-10, maxit=100, x=None, atol=0.0, warm=False):
+        """warm=True starts from the given x."""
         import torch
         if x is None:
-            x = torch.empty_like(b)
+            x = torch.zeros_like(b)
         it = ctypes.c_int()
         rr = ctypes.c_double()
-        if atol > 0:
+        if atol > 0 or warm:
             _check(lib().ibm_mg_solve_atol(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, atol, maxit,
-                                           ctypes.byref(it), ctypes.byref(rr)), self.ctx)
+                                           1 if warm else 0, ctypes.byref(it), ctypes.byref(rr)), self.ctx)
         else:
             _check(lib().ibm_mg_solve(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, maxit,
                                       ctypes.byref(it), ctypes.byref(rr)), self.ctx)
         return x, it.value, rr.value
 
-    def solve_fixed(self, b, iters, x=None):
+    def solve_fixed(self, b, iters, x=None, warm=False):
         """Exactly `iters` cycles, no host synchronisation; returns (x, relres
         as a one-element device tensor)."""
         import torch
         if x is None:
             x = torch.empty_like(b)
         rr = torch.zeros(1, dtype=torch.float64, device=b.device)
-        _check(lib().ibm_mg_solve_fixed(self.ctx.handle, self.handle, _ptr(b), _ptr(x), iters, _ptr(rr)), self.ctx)
+        _check(lib().ibm_mg_solve_fixed(self.ctx.handle, self.handle, _ptr(b), _ptr(x), iters, 1 if warm else 0,
+                                        _ptr(rr)), self.ctx)
         return x, rr
 
     def close(self):
@@ -613,7 +621,7 @@ MAX_CELL_TYPES = 8
 
 class ibm_solver_params(ctypes.Structure):
     _fields_ = [("timestep", _D), ("diffusivity", _D), ("density", _D), ("tolerance", _D),
-                ("delta", _I), ("precision", _I), ("max_iterations", _I), ("reserved", _I)]
+                ("delta", _I), ("precision", _I), ("max_iterations", _I), ("reserved", _I), ("fscale", _D)]
 
 
 class ibm_cell_type(ctypes.Structure):
@@ -626,7 +634,8 @@ class ibm_state(ctypes.Structure):
 
 
 class ibm_step_info(ctypes.Structure):
-    _fields_ = [("iterations", _I * 4), ("residual", _D * 4), ("divergence", _D)]
+    _fields_ = [("iterations", _I * 4), ("residual", _D * 4), ("divergence", _D), ("timestep", _D),
+                ("exponent", _I), ("reserved", _I)]
 
 
 class ibm_cell_view(ctypes.Structure):
@@ -683,10 +692,11 @@ class Solver:
     tether (CUDA tensor, ns x 3 cells), stiffness."""
 
     def __init__(self, ctx, domain, refinement, timestep, diffusivity, density=1.0, tolerance=1e-10,
-                 delta="cosine", precision=64, max_iterations=100, cell_types=()):
+                 delta="cosine", precision=64, max_iterations=100, cell_types=(), fscale=0.0):
+        """With fscale > 0, `timestep` is the tracker's smallest step k_min."""
         self.ctx = ctx
         par = ibm_solver_params(timestep, diffusivity, density, tolerance, DELTA.get(delta, delta), precision,
-                                max_iterations, 0)
+                                max_iterations, 0, fscale)
         n = len(cell_types)
         arr = (ibm_cell_type * max(n, 1))()
         self._keep = []
@@ -735,7 +745,8 @@ class Solver:
     def info(self):
         i = ibm_step_info()
         _check(lib().ibm_solver_info(self.ctx.handle, self.handle, ctypes.byref(i)), self.ctx)
-        return {"iterations": list(i.iterations), "residual": list(i.residual), "divergence": i.divergence}
+        return {"iterations": list(i.iterations), "residual": list(i.residual), "divergence": i.divergence,
+                "timestep": i.timestep, "exponent": i.exponent}
 
     def cells(self, k):
         """Copies of a cell type's positions (3, nc, ns), tangents, forces,

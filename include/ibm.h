@@ -25,6 +25,11 @@
 extern "C" {
 #endif
 
+/* Revision of the entry points' signatures and structures; bindings check it
+ * against ibm_abi_version() before calling anything else. */
+#define IBM_ABI_VERSION 3
+int ibm_abi_version(void);
+
 /* ------------------------------------------------------------------ */
 /* Status codes.  include/ibm.hpp turns them into exceptions           */
 /* (ibm::error, ins::too_small).                                        */
@@ -333,14 +338,16 @@ ibm_status ibm_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int view, int sid
 
 /* ibm_mg_solve stopping at ||b - A x|| <= max(rtol ||b||, atol): for right-hand
  * sides whose norm is itself rounding noise (the divergence of a field that
- * is already divergence-free), atol carries the scale of the problem. */
+ * is already divergence-free), atol carries the scale of the problem.  With
+ * warm != 0 the iteration starts from the x passed in instead of zero. */
 ibm_status ibm_mg_solve_atol(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, double rtol, double atol,
-                             int maxit, int* iters, double* relres);
+                             int maxit, int warm, int* iters, double* relres);
 
 /* ibm_mg_solve with exactly `iters` defect-correction cycles and no host
- * synchronisation (recordable in a HIP graph).  When `relres` (a device
+ * synchronisation (recordable in a HIP graph); warm as for ibm_mg_solve_atol.  When `relres` (a device
  * double) is given, ||b - A x|| / ||b|| is written there. */
-ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, int iters, double* relres);
+ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, int iters, int warm,
+                              double* relres);
 
 /* ---------------------------------------------------------------------
  * Immersed-boundary timestep (the ib::solver of test/ib-whole.cc:226-275;
@@ -355,7 +362,7 @@ ibm_status ibm_mg_solve_fixed(ibm_ctx* ctx, ibm_mg* mg, const double* b, double*
 typedef struct ibm_solver ibm_solver;
 
 typedef struct {
-    double timestep;     /* dt (library time units) */
+    double timestep;     /* dt; with fscale > 0 the smallest step k_min (library time units) */
     double diffusivity;  /* nu = mu / rho (ins/simulation.h) */
     double density;      /* rho: spread forces are divided by it */
     double tolerance;    /* relative residual of every Helmholtz / Poisson solve */
@@ -363,6 +370,7 @@ typedef struct {
     int precision;       /* multigrid V-cycles in 64 or 32 bit; defect correction is FP64 */
     int max_iterations;  /* per solve */
     int reserved;
+    double fscale;       /* > 0: step size from ins::tracker's force bound (ins/tracker.h:62-75) */
 } ibm_solver_params;
 
 typedef struct {
@@ -386,6 +394,9 @@ typedef struct {
     int iterations[4];    /* V-cycles of the u, v, w Helmholtz solves and the pressure solve */
     double residual[4];   /* their final relative residuals (FP64) */
     double divergence;    /* max |D u| after projection (FP64) */
+    double timestep;      /* step size of the last step */
+    int exponent;         /* its tracker exponent e (dt = k_min 2^e), 0 without tracking */
+    int reserved;
 } ibm_step_info;
 
 /* Device views of a cell type's per-step data (owned by the solver). */

@@ -362,6 +362,7 @@ struct advection {
 
 struct parameters {
     double timestep, diffusivity, density, tolerance;
+    double fscale = 0;   // > 0: adaptive step size (ins::tracker), timestep = k_min
 };
 
 // ins::diffusion(grid, params)(gamma, u, rhs, f) (ins/diffusion.h:46-56):
@@ -438,7 +439,8 @@ public:
     solver(const fd::mac& m, const fd::domain& dom, Delta, const ins::parameters& p, int precision,
            const std::vector<ibm_cell_type>& types, ibm::context& ctx = ibm::context::current())
     {
-        ibm_solver_params par{p.timestep, p.diffusivity, p.density, p.tolerance, (int) Delta::kind, precision, 100, 0};
+        ibm_solver_params par{p.timestep, p.diffusivity, p.density, p.tolerance, (int) Delta::kind, precision, 100, 0,
+                              p.fscale};
         ibm_solver* s = nullptr;
         ctx.check(ibm_solver_create(ctx.get(), dom.data(), 3, m.refinement(), &par, (int) types.size(), types.data(),
                                     &s));

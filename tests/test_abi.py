@@ -83,3 +83,9 @@ def test_cpp_header_is_plain_cpp():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_abi_version_matches_header(ibm):
+    text = open(os.path.join(ROOT, "include", "ibm.h")).read()
+    declared = int(re.search(r"#define IBM_ABI_VERSION (\d+)", text).group(1))
+    assert ibm.lib().ibm_abi_version() == declared == ibm.ABI_VERSION

@@ -170,3 +170,41 @@ def test_sheared_channel_matches_oracle(ibm, orc, ctx):
     assert _rel(st.u[0].cpu().numpy(), u[0]) < 1e-8
     assert float(st.u[0].max()) > 0
     s.close()
+
+
+def test_tracker_step_sizes(ibm, orc, ctx, types):
+    """ins::tracker on the device forces: the step sizes follow the oracle's
+    restatement (oracle/tracker.py) fed with the same forces; too_small is
+    IBM_ERR_DT_TOO_SMALL."""
+    tracker = _load("orc_tracker", os.path.join(ROOT, "oracle", "tracker.py"))
+    f0 = []
+    for T in types:
+        y = O.evaluate(T["E"], T["nd"], T["X"].reshape(3 * T["nc"], T["nd"]))
+        f0.append(-T["k"] * (y.reshape(3, T["nc"], -1) - T["ref"]))
+    fmax = np.sqrt(sum(float(np.abs(f).max()) ** 2 for f in f0))
+    fscale = (4 * 2 ** 2.5 * C.DT) ** 2 * (fmax + 1e-3)          # k_max = 2^2.5 k_min at the start
+    ops = [ibm.RbfOperator(ctx, "f64", [_t(T["E"]), _t(T["Et"]), _t(T["Ep"])], T["nd"]) for T in types]
+    ct = [{"ops": op, "cells": T["nc"], "weights": _t(T["w"]), "tether": _t(T["ref"].reshape(-1)),
+           "stiffness": T["k"]} for op, T in zip(ops, types)]
+    s = ibm.Solver(ctx, C.domain(ibm, True), REF, C.DT, C.NU, density=C.RHO, tolerance=1e-10, precision=32,
+                   cell_types=ct, fscale=fscale)
+    st = s.zero_state([_t(T["X"]) for T in types])
+    ref = tracker.Tracker(C.DT, fscale)
+    t = 0.0
+    for _ in range(3):
+        s.step(st)
+        forces = [s.cells(j)["forces"].cpu().numpy() for j in range(len(types))]
+        k, e = ref(forces)
+        info = s.info()
+        assert info["exponent"] == e and info["timestep"] == k, (info, k, e)
+        t += k
+    assert e == 2
+    assert abs(st.t - t) <= 1e-15 * t
+    s.close()
+    tiny = ibm.Solver(ctx, C.domain(ibm, True), REF, C.DT, C.NU, density=C.RHO, tolerance=1e-10, precision=32,
+                      cell_types=ct, fscale=fscale * 1e-4)
+    st = tiny.zero_state([_t(T["X"]) for T in types])
+    with pytest.raises(ibm.IbmError) as err:
+        tiny.step(st)
+    assert err.value.code == 5
+    tiny.close()
