@@ -30,6 +30,21 @@ ibm_status launch_sum(ibm_ctx* ctx, size_t n, const T* x, double* partial, doubl
 template <typename T>
 ibm_status launch_remove_mean(ibm_ctx* ctx, size_t n, T* y, double* partial, double* sum);
 
+// The coarse levels of a V-cycle, finest first, run by one workgroup.
+constexpr int MAX_COARSE_LEVELS = 6;
+constexpr size_t COARSE_POINTS = 16384;   // levels with at most this many points
+template <typename T>
+struct coarse_levels {
+    int count;
+    op3<T> A[MAX_COARSE_LEVELS];
+    transfer3<T> R[MAX_COARSE_LEVELS];     // level l -> l + 1
+    T w[MAX_COARSE_LEVELS][4];             // Chebyshev weights
+    T *x[MAX_COARSE_LEVELS], *b[MAX_COARSE_LEVELS], *r[MAX_COARSE_LEVELS];
+    T *s1[MAX_COARSE_LEVELS], *s2[MAX_COARSE_LEVELS];
+};
+template <typename T>
+ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L);
+
 template <typename T>
 ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const T* x, T beta, T* out);
 template <typename T>

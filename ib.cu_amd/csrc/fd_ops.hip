@@ -43,6 +43,34 @@ __device__ inline T id1(const op_axis<T>& a, int i)
 
 constexpr int STX = 64, STY = 4, KZ = 8;
 
+// One point of out = alpha * A x + beta * y + gamma * z (the expression of
+// stencil_kernel, reading the neighbours directly).
+template <typename T>
+__device__ inline T stencil_at(const op3<T>& A, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
+                               T gamma, const T* __restrict__ z, int i, int j, int k)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const size_t sxy = (size_t) nx * ny;
+    const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
+    const int ip = A.a[0].periodic ? (i == nx - 1 ? 0 : i + 1) : (i < nx - 1 ? i + 1 : i);
+    const int jm = A.a[1].periodic ? (j == 0 ? ny - 1 : j - 1) : (j > 0 ? j - 1 : j);
+    const int jp = A.a[1].periodic ? (j == ny - 1 ? 0 : j + 1) : (j < ny - 1 ? j + 1 : j);
+    const int km = A.a[2].periodic ? (k == 0 ? nz - 1 : k - 1) : (k > 0 ? k - 1 : k);
+    const int kp = A.a[2].periodic ? (k == nz - 1 ? 0 : k + 1) : (k < nz - 1 ? k + 1 : k);
+    const size_t row = (size_t) nx * j + sxy * (size_t) k;
+    const size_t p = (size_t) i + row;
+    const T x0 = x[p];
+    const T lx = lap1(A.a[0], i, x[row + im], x0, x[row + ip]);
+    const T ly = lap1(A.a[1], j, x[(size_t) i + (size_t) nx * jm + sxy * k], x0, x[(size_t) i + (size_t) nx * jp + sxy * k]);
+    const T lz = lap1(A.a[2], k, x[(size_t) i + (size_t) nx * j + sxy * km], x0, x[(size_t) i + (size_t) nx * j + sxy * kp]);
+    const T Ix = id1(A.a[0], i), Iy = id1(A.a[1], j), Iz = id1(A.a[2], k);
+    const T ax = A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
+    T v = alpha * ax;
+    if (y) v += beta * y[p];
+    if (z) v += gamma * z[p];
+    return v;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(STX * STY)
 stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
@@ -110,11 +138,8 @@ __device__ inline int fold(const transfer_axis<T>& a, int idx, int n, T& w)
 }
 
 template <typename T>
-__global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* __restrict__ coarse)
+__device__ inline T restrict_at(const transfer3<T>& R, const T* __restrict__ fine, size_t t)
 {
-    const size_t total = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
-    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
     int c[3];
     c[0] = (int) (t % R.a[0].nc);
     c[1] = (int) ((t / R.a[0].nc) % R.a[1].nc);
@@ -143,15 +168,22 @@ __global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* _
         }
         acc += wt[2][sz] * az;
     }
-    coarse[t] = acc;
+    return acc;
 }
 
 template <typename T>
-__global__ void prolong_add_kernel(transfer3<T> P, const T* __restrict__ coarse, T* __restrict__ fine)
+__global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* __restrict__ coarse)
 {
-    const size_t total = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    const size_t total = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
     size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
+    coarse[t] = restrict_at(R, fine, t);
+}
+
+// The coarse-to-fine interpolation of one fine point.
+template <typename T>
+__device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coarse, size_t t)
+{
     int f[3];
     f[0] = (int) (t % P.a[0].nf);
     f[1] = (int) ((t / P.a[0].nf) % P.a[1].nf);
@@ -183,7 +215,69 @@ __global__ void prolong_add_kernel(transfer3<T> P, const T* __restrict__ coarse,
         }
         acc += wt[2][b] * az;
     }
-    fine[t] += acc;
+    return acc;
+}
+
+template <typename T>
+__global__ void prolong_add_kernel(transfer3<T> P, const T* __restrict__ coarse, T* __restrict__ fine)
+{
+    const size_t total = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    fine[t] += prolong_at(P, coarse, t);
+}
+
+// ---------------------------------------------------------------------------
+// The coarse end of a V-cycle in one workgroup: every level of `L` (finest
+// first) is smoothed, restricted, solved, prolonged and smoothed again with
+// the same per-point arithmetic as the separate kernels, one
+// __syncthreads() between passes.  Replaces ~10 launches per coarse level.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void coarse_stencil_pass(const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                                           const T* z, T* out)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int n = nx * ny * nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int i = t % nx, j = (t / nx) % ny, k = t / (nx * ny);
+        out[t] = stencil_at(A, alpha, x, beta, y, gamma, z, i, j, k);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline void coarse_cheb(const coarse_levels<T>& L, int l, const T* v, T* out, bool add)
+{
+    const op3<T>& A = L.A[l];
+    const T* w = L.w[l];
+    coarse_stencil_pass(A, -w[0], v, T(0), (const T*) nullptr, w[1], v, L.s1[l]);
+    coarse_stencil_pass(A, T(-1), L.s1[l], T(0), (const T*) nullptr, w[2], v, L.s2[l]);
+    if (add) coarse_stencil_pass(A, T(-1), L.s2[l], T(1), (const T*) out, w[3], v, out);
+    else coarse_stencil_pass(A, T(-1), L.s2[l], T(0), (const T*) nullptr, w[3], v, out);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) coarse_vcycle_kernel(coarse_levels<T> L)
+{
+    const int last = L.count - 1;
+    for (int l = 0; l <= last; ++l) {
+        coarse_cheb(L, l, L.b[l], L.x[l], false);
+        if (l == last) break;
+        coarse_stencil_pass(L.A[l], T(-1), L.x[l], T(0), (const T*) nullptr, T(1), L.b[l], L.r[l]);
+        const transfer3<T>& R = L.R[l];
+        const int nc = R.a[0].nc * R.a[1].nc * R.a[2].nc;
+        for (int t = threadIdx.x; t < nc; t += blockDim.x) L.b[l + 1][t] = restrict_at(R, L.r[l], (size_t) t);
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= 0; --l) {
+        const transfer3<T>& R = L.R[l];
+        const int nf = R.a[0].nf * R.a[1].nf * R.a[2].nf;
+        for (int t = threadIdx.x; t < nf; t += blockDim.x) L.x[l][t] += prolong_at(R, L.x[l + 1], (size_t) t);
+        __syncthreads();
+        coarse_stencil_pass(L.A[l], T(-1), L.x[l], T(0), (const T*) nullptr, T(1), L.b[l], L.r[l]);
+        coarse_cheb(L, l, L.r[l], L.x[l], true);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -528,6 +622,15 @@ ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, con
     return IBM_OK;
 }
 
+template <typename T>
+ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L)
+{
+    prof_scope prof(ctx, "coarse_vcycle");
+    coarse_vcycle_kernel<T><<<1, 1024, 0, ctx->stream>>>(L);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
 #define IBM_INSTANTIATE(T)                                                                              \
     template ibm_status launch_stencil<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, T*); \
     template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
@@ -536,7 +639,8 @@ ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, con
     template ibm_status launch_sum<T>(ibm_ctx*, size_t, const T*, double*, double*);                   \
     template ibm_status launch_remove_mean<T>(ibm_ctx*, size_t, T*, double*, double*);                 \
     template ibm_status launch_twopoint<T>(ibm_ctx*, const twopoint_op<T>&, T, const T*, T, T*);      \
-    template ibm_status launch_hadamard<T>(ibm_ctx*, size_t, const T*, const T*, T*);
+    template ibm_status launch_hadamard<T>(ibm_ctx*, size_t, const T*, const T*, T*);                 \
+    template ibm_status launch_coarse_vcycle<T>(ibm_ctx*, const coarse_levels<T>&);
 IBM_INSTANTIATE(double)
 IBM_INSTANTIATE(float)
 template ibm_status launch_axpby<double, double>(ibm_ctx*, size_t, double, const double*, double, double*);

@@ -107,6 +107,9 @@ struct ibm_mg {
     double* partial = nullptr;  // reduction scratch
     double* scalars = nullptr;  // device scalars
     std::vector<void*> owned;
+    size_t coarse_start = 0;    // first level handled by the one-workgroup coarse kernel (lv.size(): none)
+    ibm::coarse_levels<double> coarse64{};
+    ibm::coarse_levels<float> coarse32{};
 };
 
 namespace ibm {
@@ -146,8 +149,35 @@ static ibm_status cheb_apply(ibm_ctx* ctx, const op3<T>& A, const double w[4], c
 // solvers/multigrid.h:120-128 (iterative levels) and :46-63 (coarsest):
 //   x = S b; r = b - A x; x += P vcycle(R r); x += S(b - A x)
 template <typename T>
+static const coarse_levels<T>& coarse_desc(const ibm_mg* m);
+template <>
+const coarse_levels<double>& coarse_desc<double>(const ibm_mg* m) { return m->coarse64; }
+template <>
+const coarse_levels<float>& coarse_desc<float>(const ibm_mg* m) { return m->coarse32; }
+
+// The levels from coarse_start down, for one precision.
+template <typename T>
+static void fill_coarse(ibm_mg* m, coarse_levels<T>& C)
+{
+    C.count = (int) (m->lv.size() - m->coarse_start);
+    for (int i = 0; i < C.count; ++i) {
+        const mg_level& L = m->lv[m->coarse_start + i];
+        level_bufs<T>& B = bufs<T>(m)[m->coarse_start + i];
+        C.A[i] = level_op<T>(L);
+        if (m->coarse_start + i + 1 < m->lv.size()) C.R[i] = level_tr<T>(L);
+        for (int k = 0; k < 4; ++k) C.w[i][k] = (T) L.w[k];
+        C.x[i] = B.x;
+        C.b[i] = B.b;
+        C.r[i] = B.r;
+        C.s1[i] = B.s1;
+        C.s2[i] = B.s2;
+    }
+}
+
+template <typename T>
 static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l)
 {
+    if (l == m->coarse_start) return launch_coarse_vcycle<T>(ctx, coarse_desc<T>(m));
     const mg_level& L = m->lv[l];
     level_bufs<T>& B = bufs<T>(m)[l];
     op3<T> A = level_op<T>(L);
@@ -386,6 +416,18 @@ ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* g, double sigma, double t
                 break;
             m->b32.push_back(B);
         }
+    }
+    // Levels with at most COARSE_POINTS points (and at most MAX_COARSE_LEVELS
+    // of them) run in the one-workgroup kernel.
+    m->coarse_start = m->lv.size();
+    for (size_t l = 0; l < m->lv.size(); ++l)
+        if (m->lv[l].n <= COARSE_POINTS && m->lv.size() - l <= (size_t) MAX_COARSE_LEVELS) {
+            m->coarse_start = l;
+            break;
+        }
+    if (st == IBM_OK && m->coarse_start < m->lv.size()) {
+        if (precision == 64) fill_coarse<double>(m, m->coarse64);
+        else fill_coarse<float>(m, m->coarse32);
     }
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->r);
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->rhs);
