@@ -42,7 +42,6 @@ enum scratch_slot {
     SLOT_SCAN_TMP,
     SLOT_FLAGS,
     SLOT_REDUCE,        // reduction partials of the solvers (REDUCE_PARTS + 8 doubles)
-    SLOT_INS_TEMP,      // advection temporaries
     SLOT_RBF_X,         // packed data-site coordinates of an RBF evaluation
     SLOT_STEP_BASE,     // timestep workspaces; slots above are the driver's
     SLOT_COUNT = 64

@@ -47,8 +47,15 @@ ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L);
 
 template <typename T>
 ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const T* x, T beta, T* out);
+// ins::advection of one component: A[j] = A_i on u^j's grid, B[j] = A_j on
+// u^i's grid, D[j] = D_j from their common grid back to u^i's grid.
 template <typename T>
-ibm_status launch_hadamard(ibm_ctx* ctx, size_t n, const T* a, const T* b, T* out);
+struct advection_op {
+    twopoint_op<T> A[3], B[3], D[3];
+};
+template <typename T>
+ibm_status launch_advection(ibm_ctx* ctx, const advection_op<T>& Op, const T* const* u, int comp, T* H);
+
 
 // *out = max |x_i|
 ibm_status launch_max_abs(ibm_ctx* ctx, size_t n, const double* x, double* partial, double* out);

@@ -313,13 +313,68 @@ __global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__
     out[t] = o;
 }
 
-// out_c = a * in1_c * in2_c (elementwise product, products of averages in ins::advection)
-template <typename T>
-__global__ void hadamard_kernel(size_t n, const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out)
+// One point of a two-point operator whose source values come from a functor
+// src(x, y, z) of source-grid coordinates (same fetch rules as
+// twopoint_kernel: periodic wrap, or the ghost coefficient times the end
+// value).
+template <typename T, typename F>
+__device__ inline T twopoint_at(const twopoint_op<T>& P, int x, int y, int z, const F& src)
 {
-    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t) gridDim.x * blockDim.x;
-    for (; i < n; i += stride) out[i] = a[i] * b[i];
+    int c[3] = {x, y, z};
+    const int v = P.view, cols = P.sn[v], r = c[v];
+    auto fetch = [&](int idx) -> T {
+        if (idx < 0 || idx >= cols) {
+            if (P.periodic) {
+                idx = idx < 0 ? idx + cols : idx - cols;
+            } else {
+                T g = idx < 0 ? P.int_lo : P.int_hi;
+                c[v] = idx < 0 ? 0 : cols - 1;
+                return g * src(c[0], c[1], c[2]);
+            }
+        }
+        c[v] = idx;
+        return src(c[0], c[1], c[2]);
+    };
+    T f0 = fetch(r - P.s1);
+    T f1 = fetch(r + 1 - P.s1);
+    return P.c0 * f0 + P.c1 * f1;
+}
+
+// ins::advection for one component in one pass:
+//   H^i = sum_j D_j [ (A_i u^j) o (A_j u^i) ]
+// evaluated point by point on u^i's grid (j summed in order 0, 1, 2).
+template <typename T>
+__global__ void advection_kernel(advection_op<T> Op, const T* __restrict__ u0, const T* __restrict__ u1,
+                                 const T* __restrict__ u2, int comp, T* __restrict__ H)
+{
+    const int* dn = Op.D[0].dn;
+    const size_t total = (size_t) dn[0] * dn[1] * dn[2];
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int cx = (int) (t % dn[0]);
+    const int cy = (int) ((t / dn[0]) % dn[1]);
+    const int cz = (int) (t / ((size_t) dn[0] * dn[1]));
+    const T* u[3] = {u0, u1, u2};
+    const T* ui = u[comp];
+    T acc = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const twopoint_op<T>& A = Op.A[j];
+        const twopoint_op<T>& B = Op.B[j];
+        const T* uj = u[j];
+        auto prod = [&](int x, int y, int z) -> T {
+            T a = twopoint_at(A, x, y, z, [&](int p, int q, int r) -> T {
+                return uj[(size_t) p + (size_t) A.sn[0] * ((size_t) q + (size_t) A.sn[1] * r)];
+            });
+            T b = twopoint_at(B, x, y, z, [&](int p, int q, int r) -> T {
+                return ui[(size_t) p + (size_t) B.sn[0] * ((size_t) q + (size_t) B.sn[1] * r)];
+            });
+            return a * b;
+        };
+        T d = twopoint_at(Op.D[j], cx, cy, cz, prod);
+        acc = j == 0 ? d : acc + d;
+    }
+    H[t] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -537,9 +592,15 @@ ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const
 }
 
 template <typename T>
-ibm_status launch_hadamard(ibm_ctx* ctx, size_t n, const T* a, const T* b, T* out)
+ibm_status launch_advection(ibm_ctx* ctx, const advection_op<T>& Op, const T* const* u, int comp, T* H)
 {
-    hadamard_kernel<T><<<grid_for(n), 256, 0, ctx->stream>>>(n, a, b, out);
+    const int* dn = Op.D[0].dn;
+    size_t n = (size_t) dn[0] * dn[1] * dn[2];
+    if (!n) return IBM_OK;
+    double nsrc = 0;
+    for (int d = 0; d < 3; ++d) nsrc += (double) Op.A[d].sn[0] * Op.A[d].sn[1] * Op.A[d].sn[2];
+    prof_scope prof(ctx, "advection", (double) sizeof(T) * (nsrc + n));   // u^0..2 read once, H^i written
+    advection_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], comp, H);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -639,8 +700,8 @@ ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L)
     template ibm_status launch_sum<T>(ibm_ctx*, size_t, const T*, double*, double*);                   \
     template ibm_status launch_remove_mean<T>(ibm_ctx*, size_t, T*, double*, double*);                 \
     template ibm_status launch_twopoint<T>(ibm_ctx*, const twopoint_op<T>&, T, const T*, T, T*);      \
-    template ibm_status launch_hadamard<T>(ibm_ctx*, size_t, const T*, const T*, T*);                 \
-    template ibm_status launch_coarse_vcycle<T>(ibm_ctx*, const coarse_levels<T>&);
+    template ibm_status launch_coarse_vcycle<T>(ibm_ctx*, const coarse_levels<T>&);                   \
+    template ibm_status launch_advection<T>(ibm_ctx*, const advection_op<T>&, const T* const*, int, T*);
 IBM_INSTANTIATE(double)
 IBM_INSTANTIATE(float)
 template ibm_status launch_axpby<double, double>(ibm_ctx*, size_t, double, const double*, double, double*);

@@ -162,34 +162,24 @@ ibm_status ibm_advection(ibm_ctx* ctx, const ibm_grid* u_grids, const double* co
     if (!ctx) return IBM_ERR_ARG;
     IBM_TRY(check3(ctx, u_grids, 3));
     if (!u || !H) return fail(ctx, IBM_ERR_ARG, "null vector");
-    size_t maxn = 0;
+    for (int i = 0; i < 3; ++i)
+        if (!u[i] || !H[i]) return fail(ctx, IBM_ERR_ARG, "null component");
     ibm_grid ga[3][3];
     twopoint_op<double> avg_ij[3][3];      // A_i applied to u^j
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            IBM_TRY(make_twopoint(u_grids[j], i, 1, &avg_ij[i][j], &ga[i][j]));
-            maxn = std::max(maxn, (size_t) ibm_grid_size(&ga[i][j]));
-        }
-    void* p;
-    IBM_TRY(scratch(ctx, SLOT_INS_TEMP, sizeof(double) * 3 * maxn, &p));
-    double* a = (double*) p;
-    double* b = a + maxn;
-    double* prod = b + maxn;
+        for (int j = 0; j < 3; ++j) IBM_TRY(make_twopoint(u_grids[j], i, 1, &avg_ij[i][j], &ga[i][j]));
     for (int i = 0; i < 3; ++i) {
-        if (!u[i] || !H[i]) return fail(ctx, IBM_ERR_ARG, "null component");
+        advection_op<double> Op;
         for (int j = 0; j < 3; ++j) {
             const ibm_grid& g = ga[i][j];                        // grid of A_i u^j
             if (!same_points(g, ga[j][i])) return fail(ctx, IBM_ERR_SIZE, "advection products on different grids");
-            size_t n = (size_t) ibm_grid_size(&g);
-            IBM_TRY(launch_twopoint<double>(ctx, avg_ij[i][j], 1.0, u[j], 0.0, a));
-            IBM_TRY(launch_twopoint<double>(ctx, avg_ij[j][i], 1.0, u[i], 0.0, b));
-            IBM_TRY(launch_hadamard<double>(ctx, n, a, b, prod));
-            twopoint_op<double> dj;
             ibm_grid back;
-            IBM_TRY(make_twopoint(g, j, 0, &dj, &back));
+            IBM_TRY(make_twopoint(g, j, 0, &Op.D[j], &back));
             if (!same_points(back, u_grids[i])) return fail(ctx, IBM_ERR_SIZE, "advection result off the component grid");
-            IBM_TRY(launch_twopoint<double>(ctx, dj, 1.0, prod, j ? 1.0 : 0.0, H[i]));
+            Op.A[j] = avg_ij[i][j];
+            Op.B[j] = avg_ij[j][i];
         }
+        IBM_TRY(launch_advection<double>(ctx, Op, u, i, H[i]));
     }
     return IBM_OK;
 }

@@ -238,10 +238,11 @@ __global__ void bin_canon_kernel(marker_sets sets, int with_forces,
 // walked in canonical order in chunks of 64; every wavefront owns a stream
 // of chunks.  For a chunk, lanes are markers: each computes its separable
 // weights once and, if its stencil meets the brick, stores them in the
-// wavefront's LDS slice (compacted with a ballot).  Then lanes are the 64
-// stencil points (sx, sy, sz): the wavefront walks the compacted markers,
-// sums the contributions of consecutive markers of one cell in a register
-// and adds the cell's 64 partial sums to the brick tile.
+// wavefront's LDS slice (compacted with a ballot).  Then each quarter of the
+// wavefront walks a quarter of the compacted markers with lanes = the 16
+// (sy, sz) rows of the stencil, four x points per lane: it sums the
+// contributions of consecutive markers of one cell in registers and adds
+// them to the brick tile when the cell changes.
 //
 // The tile holds 64-bit fixed-point integers (two's complement): integer
 // addition is associative, so the four wavefronts may add in any order and
@@ -308,7 +309,10 @@ constexpr int MAX_SEG2 = 2 * MAX_ROWS;
 // start in distinct LDS banks, and the 8 extra entries absorb the reads of a
 // final partial batch.
 constexpr int WROW = CHUNK + 9;
-constexpr int VISIT_BATCH = 8;
+constexpr int VISIT_BATCH = 4;
+// Wx row of 6 doubles (48 bytes): the 16-byte stores of 16 consecutive lanes
+// start 12 banks apart and cover distinct banks.
+constexpr int WX_STRIDE = 6;
 constexpr int POS_LIST = 2048;      // region markers listed in LDS; larger regions search the segments
 // FAST: every periodic axis has at least BX + 8 cells, so a stencil that
 // reaches the brick has a unique brick-relative origin in [-3, B).
@@ -350,7 +354,8 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
                     int bricks_x, int bricks_y, int nbricks, int xcd_chunk)
 {
     __shared__ unsigned long long acc[3][BRICK_PTS];
-    __shared__ double W[WAVES][20][WROW];     // rows 0-3: wx[sx]; rows 4-19: wz[a] * wy[b] * f at 4 + 4a + b
+    __shared__ double W[WAVES][16][WROW];     // wz[a] * wy[b] * f at row 4a + b
+    __shared__ __attribute__((aligned(16))) double Wx[WAVES][CHUNK + VISIT_BATCH][WX_STRIDE];   // wx[0..3]
     __shared__ int org[WAVES][CHUNK + VISIT_BATCH];
     __shared__ int seg_begin[MAX_SEG2];
     __shared__ int seg_pref[MAX_SEG2 + 1];
@@ -434,7 +439,6 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
 
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int sx = lane & 3, sy = (lane >> 2) & 3, sz = lane >> 4;
     const unsigned long long lanemask_lt = (1ull << lane) - 1;
 
     // Sorted position of region marker idx (binary search over the segments).
@@ -516,24 +520,17 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
                              && axis_hit<FAST, BZ>(g.a[c][2], bb2, P0z, ow2);
             const unsigned long long hits = __ballot(hit);
             const int nh = __popcll(hits);
-            unsigned long long below = hits & lanemask_lt;
-            int prev = below ? 63 - __clzll((long long) below) : lane;
-            int pb0 = __shfl(bb0, prev, 64), pb1 = __shfl(bb1, prev, 64), pb2 = __shfl(bb2, prev, 64);
-            bool head = hit && (!below || pb0 != bb0 || pb1 != bb1 || pb2 != bb2);
-            const int slot = __popcll(below);
-            // Head flags in compacted order: OR-reduce (head << slot).
-            unsigned long long hm = head ? (1ull << slot) : 0ull;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) hm |= __shfl_xor(hm, off, 64);
+            const int slot = __popcll(hits & lanemask_lt);
             if (hit) {
                 double fc = fv[c] * g.res3[c];
-#pragma unroll
-                for (int o = 0; o < 4; ++o) W[wave][o][slot] = w[0][o];
+                double2* wxp = reinterpret_cast<double2*>(&Wx[wave][slot][0]);
+                wxp[0] = make_double2(w[0][0], w[0][1]);
+                wxp[1] = make_double2(w[0][2], w[0][3]);
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     double wyf = w[1][b] * fc;
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) W[wave][4 + 4 * a + b][slot] = w[2][a] * wyf;
+                    for (int a = 0; a < 4; ++a) W[wave][4 * a + b][slot] = w[2][a] * wyf;
                 }
                 if (FAST) {
                     // flat brick index of stencil point 0 (biased to stay positive) and,
@@ -553,53 +550,77 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
                     org[wave][slot] = (bb0 + 1) | ((bb1 + 1) << 10) | ((bb2 + 1) << 20);
                 }
             }
-            const unsigned long long headmask =
-                ((unsigned long long) __builtin_amdgcn_readfirstlane((unsigned) (hm >> 32)) << 32) |
-                (unsigned) __builtin_amdgcn_readfirstlane((unsigned) hm);
             wave_sync();
-            // ---- lanes are stencil points ----
+            // ---- lanes are (quarter q, sy, sz), each for the four x points ----
+            // Quarter q walks compacted markers [q per, (q+1) per) and sums
+            // the contributions of consecutive markers of one cell in four
+            // registers; a change of cell (stencil origin) adds them to the tile.
             const kaxis& ax = g.a[c][0];
             const kaxis& ay = g.a[c][1];
             const kaxis& az = g.a[c][2];
-            const double* wa = &W[wave][sx][0];
-            const double* wb = &W[wave][4 + 4 * sz + sy][0];
-            const int lane_off = sx + BX * (sy + BY * sz) - ORG_BIAS;
-            double P = 0.0;
-            int target = -1;
-            for (int k0 = 0; k0 < nh; k0 += VISIT_BATCH) {
-                // all loads of the batch first (entries past nh are never used)
-                double av[VISIT_BATCH], bv[VISIT_BATCH];
+            const int qy = lane & 3, qz = (lane >> 2) & 3, quarter = lane >> 4;
+            const double* wrow = &W[wave][4 * qz + qy][0];
+            const int per = (nh + 3) >> 2;
+            const int kbeg = quarter * per;
+            const int kend = min(kbeg + per, nh);
+            const int lane_off = BX * (qy + BY * qz) - ORG_BIAS;
+            double P[4] = {0.0, 0.0, 0.0, 0.0};
+            int cur = -1;
+            auto flush = [&]() {
+                if (cur < 0) return;
+                if (FAST) {
+                    const int o = cur;
+                    if ((o >> (16 + qy)) & (o >> (20 + qz)) & 1) {
+                        const int t0 = (o & 4095) + lane_off;
+#pragma unroll
+                        for (int sx = 0; sx < 4; ++sx)
+                            if ((o >> (12 + sx)) & 1)
+                                atomicAdd(&acc[c][t0 + sx], (unsigned long long) __double2ll_rn(P[sx] * inv_q));
+                    }
+                } else {
+                    const int o = cur;
+                    int oy = brick_offset<BY>(ay, ((o >> 10) & 1023) - 1, qy, P0y);
+                    int oz = brick_offset<BZ>(az, ((o >> 20) & 1023) - 1, qz, P0z);
+                    if (oy >= 0 && oz >= 0) {
+#pragma unroll
+                        for (int sx = 0; sx < 4; ++sx) {
+                            int ox = brick_offset<BX>(ax, (o & 1023) - 1, sx, P0x);
+                            if (ox >= 0)
+                                atomicAdd(&acc[c][ox + BX * (oy + BY * oz)],
+                                          (unsigned long long) __double2ll_rn(P[sx] * inv_q));
+                        }
+                    }
+                }
+            };
+            for (int k0 = kbeg; k0 < kend; k0 += VISIT_BATCH) {
+                // all loads of the batch first (entries past kend are never used)
                 int ov[VISIT_BATCH];
+                double pv[VISIT_BATCH];
+                double2 xa[VISIT_BATCH], xb[VISIT_BATCH];
 #pragma unroll
                 for (int i = 0; i < VISIT_BATCH; ++i) {
-                    av[i] = wa[k0 + i];
-                    bv[i] = wb[k0 + i];
                     ov[i] = org[wave][k0 + i];
+                    pv[i] = wrow[k0 + i];
+                    const double2* wxp = reinterpret_cast<const double2*>(&Wx[wave][k0 + i][0]);
+                    xa[i] = wxp[0];
+                    xb[i] = wxp[1];
                 }
 #pragma unroll
                 for (int i = 0; i < VISIT_BATCH; ++i) {
-                    if (k0 + i < nh) {
-                        if ((headmask >> (k0 + i)) & 1) {
-                            if (target >= 0)
-                                atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
-                            P = 0.0;
-                            int o = ov[i];
-                            if (FAST) {
-                                int ok = (o >> (12 + sx)) & (o >> (16 + sy)) & (o >> (20 + sz)) & 1;
-                                target = ok ? (o & 4095) + lane_off : -1;
-                            } else {
-                                int ox = brick_offset<BX>(ax, (o & 1023) - 1, sx, P0x);
-                                int oy = brick_offset<BY>(ay, ((o >> 10) & 1023) - 1, sy, P0y);
-                                int oz = brick_offset<BZ>(az, ((o >> 20) & 1023) - 1, sz, P0z);
-                                target = (ox >= 0 && oy >= 0 && oz >= 0) ? ox + BX * (oy + BY * oz) : -1;
-                            }
+                    if (k0 + i < kend) {
+                        if (ov[i] != cur) {
+                            flush();
+                            P[0] = P[1] = P[2] = P[3] = 0.0;
+                            cur = ov[i];
                         }
-                        P = fma(av[i], bv[i], P);
+                        P[0] = fma(xa[i].x, pv[i], P[0]);
+                        P[1] = fma(xa[i].y, pv[i], P[1]);
+                        P[2] = fma(xb[i].x, pv[i], P[2]);
+                        P[3] = fma(xb[i].y, pv[i], P[3]);
                     }
                 }
             }
-            if (target >= 0)
-                atomicAdd(&acc[c][target], (unsigned long long) __double2ll_rn(P * inv_q));
+            flush();
             wave_sync();
         }
     }

@@ -99,20 +99,25 @@ def test_mixed_precision_matches_fp64(ibm, ctx, types):
 
 
 def test_graph_replay_matches_fixed_steps(ibm, ctx, types):
+    """A replayed HIP graph equals the same fixed-count step run eagerly by a
+    second, identically calibrated solver (each solver keeps its own
+    warm-start state), bitwise, with unrelated eager work between replays."""
     import torch
     side = torch.cuda.Stream()                   # capture needs a created stream
     gctx = ibm.Context(0, stream=side)
     s, st, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10)
+    e, eager, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10)
     torch.cuda.synchronize()
     s.step(st)                                   # calibrates the V-cycle counts
+    e.step(eager)
     torch.cuda.synchronize()
-    eager = st.clone()
-    torch.cuda.synchronize()
+    for i in range(3):
+        assert torch.equal(st.u[i], eager.u[i])
     s.capture(st)
     for k in range(3):
         s.replay(st)
         torch.cuda.synchronize()
-        s.step_fixed(eager)
+        e.step_fixed(eager)
         torch.cuda.synchronize()
         # unrelated eager work between replays
         junk = torch.randn(1 << 20, device="cuda").sum()
@@ -126,6 +131,7 @@ def test_graph_replay_matches_fixed_steps(ibm, ctx, types):
         assert max(info["residual"]) < 1e-9, info
     assert abs(st.t - eager.t) == 0
     s.close()
+    e.close()
     gctx.close()
 
 
