@@ -372,8 +372,11 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         if ((st = alloc(ctx, s, s->nu[i], &s->rhs[i]))) break;
         st = alloc(ctx, s, s->nu[i], &s->du[i]);
     }
+    // The velocity increments are the warm start of the first Helmholtz solves.
+    for (int i = 0; i < 3 && st == IBM_OK; ++i) st = launch_fill<double>(ctx, s->nu[i], 0.0, s->du[i]);
     if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->div);
     if (st == IBM_OK) st = alloc(ctx, s, 8, &s->info);
+    if (st == IBM_OK) st = launch_fill<double>(ctx, 8, 0.0, s->info);
     if (st == IBM_OK) st = alloc(ctx, s, REDUCE_PARTS, &s->partial);
     for (int t = 0; t < ntypes && st == IBM_OK; ++t) {
         const ibm_cell_type& c = types[t];
