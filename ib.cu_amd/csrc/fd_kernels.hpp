@@ -10,6 +10,13 @@ namespace ibm {
 template <typename T>
 ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
                           const T* z, T* out);
+// out = alpha A x + beta y + gamma z, also stored as floats in out32
+ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
+                                const double* y, double gamma, const double* z, double* out, float* out32);
+// acc += alpha A x + beta y + gamma z (acc in FP64)
+template <typename T>
+ibm_status launch_stencil_acc(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                              const T* z, double* acc);
 template <typename T>
 ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse);
 template <typename T>
@@ -30,20 +37,35 @@ ibm_status launch_sum(ibm_ctx* ctx, size_t n, const T* x, double* partial, doubl
 template <typename T>
 ibm_status launch_remove_mean(ibm_ctx* ctx, size_t n, T* y, double* partial, double* sum);
 
-// The coarse levels of a V-cycle, finest first, run by one workgroup.
+// The coarse levels of a V-cycle, finest first, run by one workgroup with
+// all their vectors (x, b and two temporaries per level) in LDS; only the
+// first level's right-hand side is read from and its solution written to
+// device memory.
 constexpr int MAX_COARSE_LEVELS = 6;
-constexpr size_t COARSE_POINTS = 16384;   // levels with at most this many points
+constexpr int COARSE_VECTORS = 4;
+// Larger levels run faster as separate grid-wide kernels than in one
+// workgroup (measured at 8K points: ~130 us in one workgroup vs ~85 us).
+constexpr size_t COARSE_MAX_POINTS = 2048;
+constexpr int COARSE_THREADS = 1024;
 template <typename T>
 struct coarse_levels {
     int count;
+    int n[MAX_COARSE_LEVELS];              // points per level
+    int off[MAX_COARSE_LEVELS];            // first LDS element of each level's five vectors
     op3<T> A[MAX_COARSE_LEVELS];
     transfer3<T> R[MAX_COARSE_LEVELS];     // level l -> l + 1
     T w[MAX_COARSE_LEVELS][4];             // Chebyshev weights
-    T *x[MAX_COARSE_LEVELS], *b[MAX_COARSE_LEVELS], *r[MAX_COARSE_LEVELS];
-    T *s1[MAX_COARSE_LEVELS], *s2[MAX_COARSE_LEVELS];
+    const T* b0;                           // first level's right-hand side (device)
+    T* x0;                                 // first level's solution (device)
+    size_t lds_bytes;
 };
 template <typename T>
 ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L);
+// Largest dynamic LDS of one workgroup on the device, and the kernel
+// attribute that allows a launch to use `lds_bytes` of it.
+size_t coarse_lds_limit(int device);
+template <typename T>
+ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes);
 
 template <typename T>
 ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const T* x, T beta, T* out);

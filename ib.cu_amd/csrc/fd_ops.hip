@@ -71,46 +71,80 @@ __device__ inline T stencil_at(const op3<T>& A, T alpha, const T* __restrict__ x
     return v;
 }
 
-template <typename T>
+// Thread (i, j) computes KZ consecutive planes.  Every load of the KZ planes
+// (the centre column with one plane of halo on each side, the x and y
+// neighbours, y and z) is issued before any arithmetic, so a wavefront has
+// ~7 KZ independent loads in flight; out-of-range planes of the last chunk
+// load clamped (in-bounds) addresses and are not stored.
+// OUT selects what is done with each value v: 0 stores out = v; 1 also
+// stores a float copy (the FP32 right-hand side of the next V-cycle); 2 adds
+// it to the FP64 vector acc instead of storing it (the last smoothing pass of
+// a V-cycle folded into the defect correction x += e).
+template <typename T, bool HY, bool HZ, int OUT>
 __global__ void __launch_bounds__(STX * STY)
 stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
-               T gamma, const T* __restrict__ z, T* __restrict__ out)
+               T gamma, const T* __restrict__ z, T* __restrict__ out, float* __restrict__ out32,
+               double* __restrict__ acc)
 {
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     const int i = blockIdx.x * STX + threadIdx.x;
     const int j = blockIdx.y * STY + threadIdx.y;
     const int k0 = blockIdx.z * KZ;
     if (i >= nx || j >= ny || k0 >= nz) return;
-    const int k1 = min(k0 + KZ, nz);
     const size_t sxy = (size_t) nx * ny;
     const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
     const int ip = A.a[0].periodic ? (i == nx - 1 ? 0 : i + 1) : (i < nx - 1 ? i + 1 : i);
     const int jm = A.a[1].periodic ? (j == 0 ? ny - 1 : j - 1) : (j > 0 ? j - 1 : j);
     const int jp = A.a[1].periodic ? (j == ny - 1 ? 0 : j + 1) : (j < ny - 1 ? j + 1 : j);
-    const T Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
     const size_t col = (size_t) i + (size_t) nx * j;
-    auto at = [&](int k) -> size_t { return col + sxy * (size_t) k; };
-    int km = k0 - 1;
-    if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
-    T xm = x[at(km)];
-    T x0 = x[at(k0)];
-    for (int k = k0; k < k1; ++k) {
-        int kp = k + 1;
-        if (kp >= nz) kp = A.a[2].periodic ? 0 : nz - 1;
-        T xp = x[at(kp)];
-        const size_t p = at(k);
-        const size_t row = (size_t) nx * j + sxy * (size_t) k;
-        T lx = lap1(A.a[0], i, x[row + im], x0, x[row + ip]);
-        T ly = lap1(A.a[1], j, x[(size_t) i + (size_t) nx * jm + sxy * k], x0, x[(size_t) i + (size_t) nx * jp + sxy * k]);
-        T lz = lap1(A.a[2], k, xm, x0, xp);
-        T Iz = id1(A.a[2], k);
-        T ax = A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
+    const ptrdiff_t dxm = (ptrdiff_t) im - i, dxp = (ptrdiff_t) ip - i;
+    const ptrdiff_t dym = ((ptrdiff_t) jm - j) * nx, dyp = ((ptrdiff_t) jp - j) * nx;
+
+    T c[KZ + 2], xl[KZ], xr[KZ], yl[KZ], yr[KZ], yv[KZ], zv[KZ];
+    {
+        int km = k0 - 1;
+        if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
+        c[0] = x[col + sxy * (size_t) km];
+    }
+#pragma unroll
+    for (int q = 0; q <= KZ; ++q) {
+        int k = k0 + q;
+        if (k >= nz) k = A.a[2].periodic ? k % nz : nz - 1;      // wrap (or clamp; lap1 ignores it)
+        c[q + 1] = x[col + sxy * (size_t) k];
+    }
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) {
+        const int k = min(k0 + q, nz - 1);
+        const size_t p = col + sxy * (size_t) k;
+        xl[q] = x[p + dxm];
+        xr[q] = x[p + dxp];
+        yl[q] = x[p + dym];
+        yr[q] = x[p + dyp];
+        if (HY) yv[q] = y[p];
+        if (HZ) zv[q] = z[p];
+    }
+    const T Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) {
+        const int k = k0 + q;
+        if (k >= nz) break;
+        const T x0 = c[q + 1];
+        const T xm = c[q], xp = c[q + 2];
+        const T lx = lap1(A.a[0], i, xl[q], x0, xr[q]);
+        const T ly = lap1(A.a[1], j, yl[q], x0, yr[q]);
+        const T lz = lap1(A.a[2], k, xm, x0, xp);
+        const T Iz = id1(A.a[2], k);
+        const T ax = A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
         T v = alpha * ax;
-        if (y) v += beta * y[p];
-        if (z) v += gamma * z[p];
-        out[p] = v;
-        xm = x0;
-        x0 = xp;
+        if (HY) v += beta * yv[q];
+        if (HZ) v += gamma * zv[q];
+        const size_t p = col + sxy * (size_t) k;
+        if (OUT == 2) {
+            acc[p] += (double) v;
+        } else {
+            out[p] = v;
+            if (OUT == 1) out32[p] = (float) v;
+        }
     }
 }
 
@@ -138,12 +172,13 @@ __device__ inline int fold(const transfer_axis<T>& a, int idx, int n, T& w)
 }
 
 template <typename T>
-__device__ inline T restrict_at(const transfer3<T>& R, const T* __restrict__ fine, size_t t)
+__device__ inline T restrict_at(const transfer3<T>& R, const T* __restrict__ fine, unsigned t)
 {
     int c[3];
-    c[0] = (int) (t % R.a[0].nc);
-    c[1] = (int) ((t / R.a[0].nc) % R.a[1].nc);
-    c[2] = (int) (t / ((size_t) R.a[0].nc * R.a[1].nc));
+    const unsigned q = t / (unsigned) R.a[0].nc;
+    c[0] = (int) (t - q * (unsigned) R.a[0].nc);
+    c[1] = (int) (q % (unsigned) R.a[1].nc);
+    c[2] = (int) (q / (unsigned) R.a[1].nc);
     int idx[3][4];
     T wt[3][4];
 #pragma unroll
@@ -177,17 +212,18 @@ __global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* _
     const size_t total = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
     size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    coarse[t] = restrict_at(R, fine, t);
+    coarse[t] = restrict_at(R, fine, (unsigned) t);
 }
 
 // The coarse-to-fine interpolation of one fine point.
 template <typename T>
-__device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coarse, size_t t)
+__device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coarse, unsigned t)
 {
     int f[3];
-    f[0] = (int) (t % P.a[0].nf);
-    f[1] = (int) ((t / P.a[0].nf) % P.a[1].nf);
-    f[2] = (int) (t / ((size_t) P.a[0].nf * P.a[1].nf));
+    const unsigned q = t / (unsigned) P.a[0].nf;
+    f[0] = (int) (t - q * (unsigned) P.a[0].nf);
+    f[1] = (int) (q % (unsigned) P.a[1].nf);
+    f[2] = (int) (q / (unsigned) P.a[1].nf);
     int idx[3][2];
     T wt[3][2];
 #pragma unroll
@@ -224,7 +260,7 @@ __global__ void prolong_add_kernel(transfer3<T> P, const T* __restrict__ coarse,
     const size_t total = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
     size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    fine[t] += prolong_at(P, coarse, t);
+    fine[t] += prolong_at(P, coarse, (unsigned) t);
 }
 
 // ---------------------------------------------------------------------------
@@ -247,36 +283,69 @@ __device__ inline void coarse_stencil_pass(const op3<T>& A, T alpha, const T* x,
 }
 
 template <typename T>
-__device__ inline void coarse_cheb(const coarse_levels<T>& L, int l, const T* v, T* out, bool add)
+struct coarse_bufs {
+    T *x, *b, *t1, *t2;
+};
+
+// out (+)= p(A) v with temporaries s1, s2 (all distinct from v and out).
+template <typename T>
+__device__ inline void coarse_cheb(const coarse_levels<T>& L, int l, T* s1, T* s2, const T* v, T* out, bool add)
 {
     const op3<T>& A = L.A[l];
     const T* w = L.w[l];
-    coarse_stencil_pass(A, -w[0], v, T(0), (const T*) nullptr, w[1], v, L.s1[l]);
-    coarse_stencil_pass(A, T(-1), L.s1[l], T(0), (const T*) nullptr, w[2], v, L.s2[l]);
-    if (add) coarse_stencil_pass(A, T(-1), L.s2[l], T(1), (const T*) out, w[3], v, out);
-    else coarse_stencil_pass(A, T(-1), L.s2[l], T(0), (const T*) nullptr, w[3], v, out);
+    coarse_stencil_pass(A, -w[0], v, T(0), (const T*) nullptr, w[1], v, s1);
+    coarse_stencil_pass(A, T(-1), (const T*) s1, T(0), (const T*) nullptr, w[2], v, s2);
+    if (add) coarse_stencil_pass(A, T(-1), (const T*) s2, T(1), (const T*) out, w[3], v, out);
+    else coarse_stencil_pass(A, T(-1), (const T*) s2, T(0), (const T*) nullptr, w[3], v, out);
 }
 
+// Four LDS vectors per level suffice: going down, x = S b with temporaries
+// t1, t2, then the residual in t1 is restricted into the next level's b;
+// going up, the residual b - A x goes to t1 and the post-smoothing
+// x += S t1 uses t2 and the (no longer needed) b as temporaries.
 template <typename T>
-__global__ void __launch_bounds__(1024) coarse_vcycle_kernel(coarse_levels<T> L)
+__global__ void __launch_bounds__(COARSE_THREADS) coarse_vcycle_kernel(coarse_levels<T> L)
 {
+    extern __shared__ __attribute__((aligned(16))) unsigned char coarse_smem[];
+    T* base = reinterpret_cast<T*>(coarse_smem);
+    auto bufs = [&](int l) {
+        coarse_bufs<T> c;
+        T* p = base + L.off[l];
+        const int n = L.n[l];
+        c.x = p;
+        c.b = p + n;
+        c.t1 = p + 2 * n;
+        c.t2 = p + 3 * n;
+        return c;
+    };
+    {
+        const coarse_bufs<T> B0 = bufs(0);
+        for (int t = threadIdx.x; t < L.n[0]; t += blockDim.x) B0.b[t] = L.b0[t];
+    }
+    __syncthreads();
     const int last = L.count - 1;
     for (int l = 0; l <= last; ++l) {
-        coarse_cheb(L, l, L.b[l], L.x[l], false);
+        const coarse_bufs<T> B = bufs(l);
+        coarse_cheb(L, l, B.t1, B.t2, (const T*) B.b, B.x, false);
         if (l == last) break;
-        coarse_stencil_pass(L.A[l], T(-1), L.x[l], T(0), (const T*) nullptr, T(1), L.b[l], L.r[l]);
+        coarse_stencil_pass(L.A[l], T(-1), (const T*) B.x, T(0), (const T*) nullptr, T(1), (const T*) B.b, B.t1);
         const transfer3<T>& R = L.R[l];
-        const int nc = R.a[0].nc * R.a[1].nc * R.a[2].nc;
-        for (int t = threadIdx.x; t < nc; t += blockDim.x) L.b[l + 1][t] = restrict_at(R, L.r[l], (size_t) t);
+        T* nb = bufs(l + 1).b;
+        for (int t = threadIdx.x; t < L.n[l + 1]; t += blockDim.x) nb[t] = restrict_at(R, (const T*) B.t1, (unsigned) t);
         __syncthreads();
     }
     for (int l = last - 1; l >= 0; --l) {
+        const coarse_bufs<T> B = bufs(l);
+        const T* cx = bufs(l + 1).x;
         const transfer3<T>& R = L.R[l];
-        const int nf = R.a[0].nf * R.a[1].nf * R.a[2].nf;
-        for (int t = threadIdx.x; t < nf; t += blockDim.x) L.x[l][t] += prolong_at(R, L.x[l + 1], (size_t) t);
+        for (int t = threadIdx.x; t < L.n[l]; t += blockDim.x) B.x[t] += prolong_at(R, cx, (unsigned) t);
         __syncthreads();
-        coarse_stencil_pass(L.A[l], T(-1), L.x[l], T(0), (const T*) nullptr, T(1), L.b[l], L.r[l]);
-        coarse_cheb(L, l, L.r[l], L.x[l], true);
+        coarse_stencil_pass(L.A[l], T(-1), (const T*) B.x, T(0), (const T*) nullptr, T(1), (const T*) B.b, B.t1);
+        coarse_cheb(L, l, B.t2, B.b, (const T*) B.t1, B.x, true);
+    }
+    {
+        const coarse_bufs<T> B0 = bufs(0);
+        for (int t = threadIdx.x; t < L.n[0]; t += blockDim.x) L.x0[t] = B0.x[t];
     }
 }
 
@@ -291,9 +360,12 @@ __global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__
     size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int c[3];
-    c[0] = (int) (t % P.dn[0]);
-    c[1] = (int) ((t / P.dn[0]) % P.dn[1]);
-    c[2] = (int) (t / ((size_t) P.dn[0] * P.dn[1]));
+    {
+        const unsigned u = (unsigned) t, q = u / (unsigned) P.dn[0];
+        c[0] = (int) (u - q * (unsigned) P.dn[0]);
+        c[1] = (int) (q % (unsigned) P.dn[1]);
+        c[2] = (int) (q / (unsigned) P.dn[1]);
+    }
     const int v = P.view;
     const int cols = P.sn[v];
     size_t stride = 1;
@@ -351,9 +423,10 @@ __global__ void advection_kernel(advection_op<T> Op, const T* __restrict__ u0, c
     const size_t total = (size_t) dn[0] * dn[1] * dn[2];
     size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int cx = (int) (t % dn[0]);
-    const int cy = (int) ((t / dn[0]) % dn[1]);
-    const int cz = (int) (t / ((size_t) dn[0] * dn[1]));
+    const unsigned tt = (unsigned) t, q = tt / (unsigned) dn[0];
+    const int cx = (int) (tt - q * (unsigned) dn[0]);
+    const int cy = (int) (q % (unsigned) dn[1]);
+    const int cz = (int) (q / (unsigned) dn[1]);
     const T* u[3] = {u0, u1, u2};
     const T* ui = u[comp];
     T acc = 0;
@@ -542,25 +615,56 @@ inline int grid_for(size_t n, int threads = 256)
 // ---------------------------------------------------------------------------
 namespace ibm {
 
-template <typename T>
-ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
-                          const T* z, T* out)
+template <typename T, int OUT>
+static ibm_status stencil_dispatch(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                                   const T* z, T* out, float* out32, double* acc)
 {
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
     dim3 block(STX, STY);
     dim3 grid((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
-    const double streams = 2.0 + (y ? 1.0 : 0.0) + (z ? 1.0 : 0.0);   // x, out, optional y, z
-    prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32", streams * sizeof(T) * nx * ny * (double) nz);
-    stencil_kernel<T><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out);
+    // x, y, z read and out written; the float copy adds 4 bytes, the FP64
+    // read-modify-write of acc replaces the store of out by 16 bytes
+    const double streams = (OUT == 2 ? 1.0 : 2.0) + (y ? 1.0 : 0.0) + (z ? 1.0 : 0.0);
+    const double extra = OUT == 1 ? 4.0 : OUT == 2 ? 16.0 : 0.0;
+    prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32",
+                    (streams * sizeof(T) + extra) * nx * ny * (double) nz);
+    if (y && z) stencil_kernel<T, true, true, OUT><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out, out32, acc);
+    else if (y) stencil_kernel<T, true, false, OUT><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out, out32, acc);
+    else if (z) stencil_kernel<T, false, true, OUT><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out, out32, acc);
+    else stencil_kernel<T, false, false, OUT><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out, out32, acc);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
 
 template <typename T>
+ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                          const T* z, T* out)
+{
+    return stencil_dispatch<T, 0>(ctx, A, alpha, x, beta, y, gamma, z, out, nullptr, nullptr);
+}
+
+ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
+                                const double* y, double gamma, const double* z, double* out, float* out32)
+{
+    return stencil_dispatch<double, 1>(ctx, A, alpha, x, beta, y, gamma, z, out, out32, nullptr);
+}
+
+template <typename T>
+ibm_status launch_stencil_acc(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
+                              const T* z, double* acc)
+{
+    return stencil_dispatch<T, 2>(ctx, A, alpha, x, beta, y, gamma, z, nullptr, nullptr, acc);
+}
+
+// Per-point kernels index grids with 32-bit unsigned arithmetic.
+constexpr size_t MAX_GRID_POINTS = 0xffffffffull;
+
+template <typename T>
 ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse)
 {
     size_t n = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
+    if ((size_t) R.a[0].nf * R.a[1].nf * R.a[2].nf > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
     size_t nf = (size_t) R.a[0].nf * R.a[1].nf * R.a[2].nf;
     prof_scope prof(ctx, "restrict", (double) sizeof(T) * (nf + n));
     restrict_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(R, fine, coarse);
@@ -572,6 +676,7 @@ template <typename T>
 ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine)
 {
     size_t n = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    if (n > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
     size_t nc = (size_t) P.a[0].nc * P.a[1].nc * P.a[2].nc;
     prof_scope prof(ctx, "prolong", (double) sizeof(T) * (nc + 2 * n));
     prolong_add_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, coarse, fine);
@@ -584,6 +689,8 @@ ibm_status launch_twopoint(ibm_ctx* ctx, const twopoint_op<T>& P, T alpha, const
 {
     size_t n = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
     if (!n) return IBM_OK;
+    if (n > MAX_GRID_POINTS || (size_t) P.sn[0] * P.sn[1] * P.sn[2] > MAX_GRID_POINTS)
+        return fail(ctx, IBM_ERR_SIZE, "grid too large");
     size_t ns = (size_t) P.sn[0] * P.sn[1] * P.sn[2];
     prof_scope prof(ctx, "twopoint", (double) sizeof(T) * (ns + n + (beta != T(0) ? n : 0)));
     twopoint_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(P, alpha, x, beta, out);
@@ -597,6 +704,7 @@ ibm_status launch_advection(ibm_ctx* ctx, const advection_op<T>& Op, const T* co
     const int* dn = Op.D[0].dn;
     size_t n = (size_t) dn[0] * dn[1] * dn[2];
     if (!n) return IBM_OK;
+    if (n > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
     double nsrc = 0;
     for (int d = 0; d < 3; ++d) nsrc += (double) Op.A[d].sn[0] * Op.A[d].sn[1] * Op.A[d].sn[2];
     prof_scope prof(ctx, "advection", (double) sizeof(T) * (nsrc + n));   // u^0..2 read once, H^i written
@@ -687,13 +795,31 @@ template <typename T>
 ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L)
 {
     prof_scope prof(ctx, "coarse_vcycle");
-    coarse_vcycle_kernel<T><<<1, 1024, 0, ctx->stream>>>(L);
+    coarse_vcycle_kernel<T><<<1, COARSE_THREADS, L.lds_bytes, ctx->stream>>>(L);
     IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+size_t coarse_lds_limit(int device)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return (size_t) v;
+}
+
+template <typename T>
+ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void*) coarse_vcycle_kernel<T>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);
+    if (e != hipSuccess) return hip_fail(ctx, e, "coarse multigrid kernel LDS size");
     return IBM_OK;
 }
 
 #define IBM_INSTANTIATE(T)                                                                              \
     template ibm_status launch_stencil<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, T*); \
+    template ibm_status launch_stencil_acc<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, double*); \
     template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
     template ibm_status launch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);            \
     template ibm_status launch_fill<T>(ibm_ctx*, size_t, T, T*);                                       \
@@ -701,6 +827,7 @@ ibm_status launch_coarse_vcycle(ibm_ctx* ctx, const coarse_levels<T>& L)
     template ibm_status launch_remove_mean<T>(ibm_ctx*, size_t, T*, double*, double*);                 \
     template ibm_status launch_twopoint<T>(ibm_ctx*, const twopoint_op<T>&, T, const T*, T, T*);      \
     template ibm_status launch_coarse_vcycle<T>(ibm_ctx*, const coarse_levels<T>&);                   \
+    template ibm_status prepare_coarse_vcycle<T>(ibm_ctx*, size_t);                   \
     template ibm_status launch_advection<T>(ibm_ctx*, const advection_op<T>&, const T* const*, int, T*);
 IBM_INSTANTIATE(double)
 IBM_INSTANTIATE(float)

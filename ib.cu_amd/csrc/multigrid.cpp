@@ -136,12 +136,15 @@ std::vector<level_bufs<float>>& bufs<float>(ibm_mg* m) { return m->b32; }
 // out = p(A) v, the degree-4 Chebyshev polynomial by Horner's rule
 // (algo/chebyshev.h:149-163): s0 = w0 v, s_k = w_k v - A s_{k-1}.
 // With add = true, out += p(A) v instead (out must not alias v, s1, s2).
+// With acc (add mode only), the final out + p(A) v is added to the FP64
+// vector acc instead of being stored in out.
 template <typename T>
 static ibm_status cheb_apply(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* s1, T* s2,
-                             T* out, bool add)
+                             T* out, bool add, double* acc = nullptr)
 {
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -w[0], v, 0, nullptr, (T) w[1], v, s1));
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, s1, 0, nullptr, (T) w[2], v, s2));
+    if (add && acc) return launch_stencil_acc<T>(ctx, A, (T) -1, s2, (T) 1, out, (T) w[3], v, acc);
     if (add) return launch_stencil<T>(ctx, A, (T) -1, s2, (T) 1, out, (T) w[3], v, out);
     return launch_stencil<T>(ctx, A, (T) -1, s2, 0, nullptr, (T) w[3], v, out);
 }
@@ -160,22 +163,25 @@ template <typename T>
 static void fill_coarse(ibm_mg* m, coarse_levels<T>& C)
 {
     C.count = (int) (m->lv.size() - m->coarse_start);
+    C.lds_bytes = 0;
     for (int i = 0; i < C.count; ++i) {
         const mg_level& L = m->lv[m->coarse_start + i];
-        level_bufs<T>& B = bufs<T>(m)[m->coarse_start + i];
+        C.n[i] = (int) L.n;
+        C.off[i] = (int) (C.lds_bytes / sizeof(T));
         C.A[i] = level_op<T>(L);
         if (m->coarse_start + i + 1 < m->lv.size()) C.R[i] = level_tr<T>(L);
         for (int k = 0; k < 4; ++k) C.w[i][k] = (T) L.w[k];
-        C.x[i] = B.x;
-        C.b[i] = B.b;
-        C.r[i] = B.r;
-        C.s1[i] = B.s1;
-        C.s2[i] = B.s2;
+        C.lds_bytes += COARSE_VECTORS * sizeof(T) * L.n;
     }
+    level_bufs<T>& B0 = bufs<T>(m)[m->coarse_start];
+    C.b0 = B0.b;
+    C.x0 = B0.x;
 }
 
+// With acc, the level's final correction is added to the FP64 vector acc
+// (used on level 0 when fused(), see below) instead of being stored in x.
 template <typename T>
-static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l)
+static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l, double* acc = nullptr)
 {
     if (l == m->coarse_start) return launch_coarse_vcycle<T>(ctx, coarse_desc<T>(m));
     const mg_level& L = m->lv[l];
@@ -189,7 +195,35 @@ static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l)
     IBM_TRY(vcycle<T>(ctx, m, l + 1));
     IBM_TRY(launch_prolong_add<T>(ctx, level_tr<T>(L), C.x, B.x));
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
-    return cheb_apply<T>(ctx, A, L.w, B.r, B.s1, B.s2, B.x, true);
+    return cheb_apply<T>(ctx, A, L.w, B.r, B.s1, B.s2, B.x, true, acc);
+}
+
+// FP32 V-cycles whose level 0 runs the separate kernels (two levels or more,
+// the coarse end not starting at level 0) fold the defect-correction update
+// x += e into the last smoothing pass and produce the next FP32 right-hand
+// side in the FP64 residual pass.
+template <typename T>
+static bool fused(const ibm_mg* m)
+{
+    return sizeof(T) == 4 && m->lv.size() > 1 && m->coarse_start > 0;
+}
+
+// One defect-correction iteration: x += V(r), r = b - A x; with fused(),
+// B0.b already holds r in FP32 and is refreshed by the residual pass.
+template <typename T>
+static ibm_status correct(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x)
+{
+    const mg_level& L0 = m->lv[0];
+    const size_t n = L0.n;
+    level_bufs<T>& B0 = bufs<T>(m)[0];
+    if (fused<T>(m)) {
+        IBM_TRY(vcycle<T>(ctx, m, 0, x));
+        return launch_stencil_split(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r, (float*) B0.b);
+    }
+    IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
+    IBM_TRY(vcycle<T>(ctx, m, 0));
+    IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
+    return launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r);
 }
 
 static ibm_status read_scalar(ibm_ctx* ctx, const double* d, double* h)
@@ -244,16 +278,16 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
     double nr = nb;
     if (warm) IBM_TRY(norm2(ctx, m, n, m->r, &nr));
     const double stop = std::max(rtol * nb, atol);
+    if (fused<T>(m)) IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
     while (nr > stop && it < maxit) {
-        IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
-        IBM_TRY(vcycle<T>(ctx, m, 0));
-        IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
-        if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
-        IBM_TRY(launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
+        IBM_TRY(correct<T>(ctx, m, b, x));
         ++it;
         IBM_TRY(norm2(ctx, m, n, m->r, &nr));
         if (nr <= stop) break;
     }
+    // A 1 = 0 on the singular operators, so the constant component of x does
+    // not change the residuals above: it is projected out once, here.
+    if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
     if (iters) *iters = it;
     if (relres) *relres = nb > 0 ? nr / nb : 0.0;
     if (nr > stop)
@@ -277,13 +311,9 @@ static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* 
         b = m->rhs;
     }
     IBM_TRY(start(ctx, m, b, x, warm));
-    for (int it = 0; it < iters; ++it) {
-        IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
-        IBM_TRY(vcycle<T>(ctx, m, 0));
-        IBM_TRY(launch_axpby<double, T>(ctx, n, 1.0, B0.x, 1.0, x));
-        if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
-        IBM_TRY(launch_stencil<double>(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
-    }
+    if (fused<T>(m) && iters > 0) IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
+    for (int it = 0; it < iters; ++it) IBM_TRY(correct<T>(ctx, m, b, x));
+    if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
     if (relres_dev) {
         IBM_TRY(launch_dot<double, double>(ctx, n, b, b, m->partial, m->scalars + 2));
         IBM_TRY(launch_dot<double, double>(ctx, n, m->r, m->r, m->partial, m->scalars + 3));
@@ -417,17 +447,27 @@ ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* g, double sigma, double t
             m->b32.push_back(B);
         }
     }
-    // Levels with at most COARSE_POINTS points (and at most MAX_COARSE_LEVELS
-    // of them) run in the one-workgroup kernel.
+    // The coarsest levels whose vectors fit in one workgroup's LDS (at most
+    // MAX_COARSE_LEVELS of them) run in the one-workgroup kernel.
     m->coarse_start = m->lv.size();
-    for (size_t l = 0; l < m->lv.size(); ++l)
-        if (m->lv[l].n <= COARSE_POINTS && m->lv.size() - l <= (size_t) MAX_COARSE_LEVELS) {
+    {
+        const size_t elem = precision == 64 ? sizeof(double) : sizeof(float);
+        const size_t limit = coarse_lds_limit(ctx->device);
+        size_t bytes = 0;
+        for (size_t l = m->lv.size(); l-- > 0;) {
+            bytes += COARSE_VECTORS * elem * m->lv[l].n;
+            if (bytes > limit || m->lv[l].n > COARSE_MAX_POINTS || m->lv.size() - l > (size_t) MAX_COARSE_LEVELS) break;
             m->coarse_start = l;
-            break;
         }
+    }
     if (st == IBM_OK && m->coarse_start < m->lv.size()) {
-        if (precision == 64) fill_coarse<double>(m, m->coarse64);
-        else fill_coarse<float>(m, m->coarse32);
+        if (precision == 64) {
+            fill_coarse<double>(m, m->coarse64);
+            st = prepare_coarse_vcycle<double>(ctx, m->coarse64.lds_bytes);
+        } else {
+            fill_coarse<float>(m, m->coarse32);
+            st = prepare_coarse_vcycle<float>(ctx, m->coarse32.lds_bytes);
+        }
     }
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->r);
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->rhs);
