@@ -89,7 +89,7 @@ def make_solver(ibm, ctx, L, types, precision):
 
 
 KERNELS = ("spread_brick", "bin", "interp", "rbf_gemm", "geometry", "stencil_f64", "stencil_f32", "restrict",
-           "prolong", "twopoint", "coarse_vcycle", "advection")
+           "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64")
 
 
 def run_form(ibm, ctx, stream, L, types, precision, steps, warmup):

@@ -48,6 +48,17 @@ def _compile(src):
     return obj
 
 
+def device_asm(src, out):
+    """gfx950 assembly of one source, with the flags of the library build."""
+    cmd = [HIPCC, *COMMON, "--cuda-device-only", "-S", src, "-o", out]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-x", "hip", *COMMON, "--cuda-device-only", "-S", src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc -S failed on {src}:\n{r.stderr}")
+    return out
+
+
 def build(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = sources()

@@ -10,6 +10,11 @@ namespace ibm {
 template <typename T>
 ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
                           const T* z, T* out);
+// out (+)= p(A) v, the degree-4 Chebyshev smoothing of weights w in one
+// pass; with acc, acc += out + p(A) v instead (add must be true).
+template <typename T>
+ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
+                             double* acc);
 // out = alpha A x + beta y + gamma z, also stored as floats in out32
 ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
                                 const double* y, double gamma, const double* z, double* out, float* out32);

@@ -149,6 +149,145 @@ stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __re
 }
 
 // ---------------------------------------------------------------------------
+// Degree-4 Chebyshev smoothing in one pass (temporal blocking):
+//   s1 = w1 v - w0 A v,  s2 = w2 v - A s1,  out = w3 v - A s2 (+ out)
+// A block owns a TX x TY x TZ tile of the output; it loads v on the tile
+// grown by 3 points per side, computes s1 on the tile grown by 2 and s2 on
+// the tile grown by 1 in LDS (s2 over v: each point reads v only where it
+// writes), then the output on the tile, re-reading v there from memory.  Same per-point
+// arithmetic as stencil_kernel; region points outside a walled axis are not
+// computed (lap1 never reads them).  With ACC the result is added to the
+// FP64 vector acc instead of being stored.
+// The axes' periodicity is a template parameter (bit d of PER: axis d
+// periodic), so no branch on it is left in the kernel.  With it as a run-time
+// flag the compiler built the branch masks once, inside the first region
+// loop, and reused them in the later loops: lanes that had already left the
+// first loop held 0 there, and waves whose active lanes were all such lanes
+// took the periodic branch on a walled axis (missing wall terms).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct cheb_tile;
+template <>
+struct cheb_tile<float> {
+    static constexpr int X = 32, Y = 8, Z = 8;
+};
+template <>
+struct cheb_tile<double> {
+    static constexpr int X = 8, Y = 8, Z = 8;
+};
+constexpr int CHEB_THREADS = 256;
+
+// lap1 with the axis' periodicity fixed at compile time.
+template <bool P, typename T>
+__device__ inline T lap1_fixed(const op_axis<T>& a, int i, T xm, T x0, T xp)
+{
+    if constexpr (P) {
+        return a.r2 * (xm + xp - 2 * x0);
+    } else {
+        T d = -2;
+        if (i == 0) d += a.lap_lo;
+        if (i == a.n - 1) d += a.lap_hi;
+        T l = i > 0 ? xm : T(0);
+        T r = i < a.n - 1 ? xp : T(0);
+        return a.r2 * (l + r + d * x0);
+    }
+}
+
+// One point of sigma I~ + tau L from a region array S (strides 1, sy, sz)
+// at local offset l, global coordinates (gi, gj, gk).
+template <int PER, typename T>
+__device__ inline T region_op(const op3<T>& A, const T* S, int l, int sy, int sz, int gi, int gj, int gk)
+{
+    const T x0 = S[l];
+    const T lx = lap1_fixed<(PER & 1) != 0>(A.a[0], gi, S[l - 1], x0, S[l + 1]);
+    const T ly = lap1_fixed<(PER & 2) != 0>(A.a[1], gj, S[l - sy], x0, S[l + sy]);
+    const T lz = lap1_fixed<(PER & 4) != 0>(A.a[2], gk, S[l - sz], x0, S[l + sz]);
+    const T Ix = id1(A.a[0], gi), Iy = id1(A.a[1], gj), Iz = id1(A.a[2], gk);
+    return A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
+}
+
+// Global coordinate of region coordinate c along an axis (tile origin o,
+// halo h): wrapped on a periodic axis, -1 outside a walled one.
+template <bool P, typename T>
+__device__ inline int region_global(const op_axis<T>& a, int o, int h, int c)
+{
+    int g = o - h + c;
+    if constexpr (P) {
+        g %= a.n;
+        if (g < 0) g += a.n;
+        return g;
+    } else {
+        return (g < 0 || g >= a.n) ? -1 : g;
+    }
+}
+
+template <typename T, bool ADD, bool ACC, int PER>
+__global__ void __launch_bounds__(CHEB_THREADS)
+cheb_fused_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* __restrict__ out,
+                  double* __restrict__ acc, int tiles_x, int tiles_y, int ntiles, int xcd_chunk)
+{
+    constexpr int TX = cheb_tile<T>::X, TY = cheb_tile<T>::Y, TZ = cheb_tile<T>::Z;
+    constexpr int VX = TX + 6, VY = TY + 6, VZ = TZ + 6;
+    __shared__ T Sv[VX * VY * VZ];        // v, then s2 (on the region grown by 1)
+    __shared__ T S1[VX * VY * VZ];
+    int b = blockIdx.x;
+    if (xcd_chunk) b = (b & 7) * xcd_chunk + (b >> 3);
+    if (b >= ntiles) return;
+    const int ox = (b % tiles_x) * TX;
+    const int oy = ((b / tiles_x) % tiles_y) * TY;
+    const int oz = (b / (tiles_x * tiles_y)) * TZ;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    constexpr int sy = VX, sz = VX * VY;
+
+    // v on the region grown by 3
+    for (int t = threadIdx.x; t < VX * VY * VZ; t += CHEB_THREADS) {
+        const int cx = t % VX, cy = (t / VX) % VY, cz = t / (VX * VY);
+        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
+        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
+        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
+        T val = T(0);
+        if (gi >= 0 && gj >= 0 && gk >= 0) val = v[(size_t) gi + (size_t) nx * ((size_t) gj + (size_t) ny * gk)];
+        Sv[t] = val;
+    }
+    __syncthreads();
+    // s1 = w1 v - w0 A v on the region grown by 2 (region offsets 1 .. V-2)
+    for (int t = threadIdx.x; t < (VX - 2) * (VY - 2) * (VZ - 2); t += CHEB_THREADS) {
+        const int cx = 1 + t % (VX - 2), cy = 1 + (t / (VX - 2)) % (VY - 2), cz = 1 + t / ((VX - 2) * (VY - 2));
+        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
+        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
+        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
+        if (gi < 0 || gj < 0 || gk < 0) continue;
+        const int l = cx + sy * cy + sz * cz;
+        S1[l] = -w0 * region_op<PER>(A, Sv, l, sy, sz, gi, gj, gk) + w1 * Sv[l];
+    }
+    __syncthreads();
+    // s2 = w2 v - A s1 on the region grown by 1
+    for (int t = threadIdx.x; t < (VX - 4) * (VY - 4) * (VZ - 4); t += CHEB_THREADS) {
+        const int cx = 2 + t % (VX - 4), cy = 2 + (t / (VX - 4)) % (VY - 4), cz = 2 + t / ((VX - 4) * (VY - 4));
+        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
+        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
+        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
+        if (gi < 0 || gj < 0 || gk < 0) continue;
+        const int l = cx + sy * cy + sz * cz;
+        Sv[l] = T(-1) * region_op<PER>(A, S1, l, sy, sz, gi, gj, gk) + w2 * Sv[l];
+    }
+    __syncthreads();
+    // out = w3 v - A s2 (+ out) on the tile
+    for (int t = threadIdx.x; t < TX * TY * TZ; t += CHEB_THREADS) {
+        const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+        const int gi = ox + tx, gj = oy + ty, gk = oz + tz;
+        if (gi >= nx || gj >= ny || gk >= nz) continue;
+        const int l = (tx + 3) + sy * (ty + 3) + sz * (tz + 3);
+        const size_t p = (size_t) gi + (size_t) nx * ((size_t) gj + (size_t) ny * gk);
+        T val = T(-1) * region_op<PER>(A, Sv, l, sy, sz, gi, gj, gk);
+        if (ADD) val += T(1) * out[p];
+        val += w3 * v[p];
+        if (ACC) acc[p] += (double) val;
+        else out[p] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Transfers.  Along one axis the fine->coarse weights are w[t]/2 at fine
 // index 2c + t - 1 and the coarse->fine weights w[j - 2c + 1]; an index one
 // past either end wraps (periodic) or folds onto the end point times the fine
@@ -644,6 +783,56 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
     return stencil_dispatch<T, 0>(ctx, A, alpha, x, beta, y, gamma, z, out, nullptr, nullptr);
 }
 
+// One instantiation per periodicity pattern of the three axes.
+template <typename T, bool ADD, bool ACC>
+static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A, T w0, T w1, T w2, T w3, const T* v,
+                        T* out, double* acc, int tx, int ty, int ntiles, int xcd_chunk)
+{
+#define IBM_CHEB_CASE(P)                                                                                            \
+    case P:                                                                                                         \
+        cheb_fused_kernel<T, ADD, ACC, P><<<blocks, CHEB_THREADS, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, tx, ty, \
+                                                                              ntiles, xcd_chunk);                   \
+        break;
+    switch (per) {
+        IBM_CHEB_CASE(0)
+        IBM_CHEB_CASE(1)
+        IBM_CHEB_CASE(2)
+        IBM_CHEB_CASE(3)
+        IBM_CHEB_CASE(4)
+        IBM_CHEB_CASE(5)
+        IBM_CHEB_CASE(6)
+        IBM_CHEB_CASE(7)
+    }
+#undef IBM_CHEB_CASE
+}
+
+template <typename T>
+ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
+                             double* acc)
+{
+    constexpr int TX = cheb_tile<T>::X, TY = cheb_tile<T>::Y, TZ = cheb_tile<T>::Z;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
+    const int tx = (nx + TX - 1) / TX, ty = (ny + TY - 1) / TY, tz = (nz + TZ - 1) / TZ;
+    const int ntiles = tx * ty * tz;
+    const int blocks = (ntiles + 7) / 8 * 8;
+    const int xcd_chunk = blocks / 8;
+    const double n = (double) nx * ny * nz;
+    // v read, out written (read too when adding; FP64 read-modify-write with acc)
+    const double bytes = n * (sizeof(T) + (acc ? 16.0 + sizeof(T) : (add ? 2.0 : 1.0) * sizeof(T)));
+    prof_scope prof(ctx, sizeof(T) == 8 ? "chebyshev_f64" : "chebyshev_f32", bytes);
+    const T w0 = (T) w[0], w1 = (T) w[1], w2 = (T) w[2], w3 = (T) w[3];
+    const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
+    if (acc)
+        cheb_launch<T, true, true>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+    else if (add)
+        cheb_launch<T, true, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+    else
+        cheb_launch<T, false, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
 ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
                                 const double* y, double gamma, const double* z, double* out, float* out32)
 {
@@ -820,6 +1009,7 @@ ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes)
 #define IBM_INSTANTIATE(T)                                                                              \
     template ibm_status launch_stencil<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, T*); \
     template ibm_status launch_stencil_acc<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, double*); \
+    template ibm_status launch_cheb_fused<T>(ibm_ctx*, const op3<T>&, const double*, const T*, T*, bool, double*); \
     template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
     template ibm_status launch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);            \
     template ibm_status launch_fill<T>(ibm_ctx*, size_t, T, T*);                                       \
