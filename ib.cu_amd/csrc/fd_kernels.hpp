@@ -10,6 +10,8 @@ namespace ibm {
 template <typename T>
 ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T beta, const T* y, T gamma,
                           const T* z, T* out);
+// Planes the one-pass smoother handles (x and y extents below 32768).
+bool cheb_fused_fits(int nx, int ny);
 // out (+)= p(A) v, the degree-4 Chebyshev smoothing of weights w in one
 // pass; with acc, acc += out + p(A) v instead (add must be true).
 template <typename T>

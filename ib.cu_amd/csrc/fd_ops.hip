@@ -7,6 +7,7 @@
 // stencil pass moves only the vectors (16 B per point in FP64, 8 in FP32,
 // +8/+4 per extra input vector) instead of ~100 B of CSR data per row.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <vector>
 #include "context.hpp"
@@ -149,67 +150,49 @@ stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __re
 }
 
 // ---------------------------------------------------------------------------
-// Degree-4 Chebyshev smoothing in one pass (temporal blocking):
+// Degree-4 Chebyshev smoothing in one pass:
 //   s1 = w1 v - w0 A v,  s2 = w2 v - A s1,  out = w3 v - A s2 (+ out)
-// A block owns a TX x TY x TZ tile of the output; it loads v on the tile
-// grown by 3 points per side, computes s1 on the tile grown by 2 and s2 on
-// the tile grown by 1 in LDS (s2 over v: each point reads v only where it
-// writes), then the output on the tile, re-reading v there from memory.  Same per-point
-// arithmetic as stencil_kernel; region points outside a walled axis are not
-// computed (lap1 never reads them).  With ACC the result is added to the
-// FP64 vector acc instead of being stored.
+// A block owns a TX x TY column tile of the output and a chunk of z planes
+// [z0, z1), and marches through z with rings of x-y planes in LDS: v on the
+// tile grown by 3 points per side, s1 (grown by 2) and s2 (grown by 1), all
+// stored in the grown-by-3 plane layout.  Phase q computes s1 on plane
+// z0-5+q, s2 on plane z0-7+q and the output on plane z0-9+q (whose out / acc
+// operands were loaded two phases earlier), then stores v plane z0-3+q
+// (loaded two phases earlier) and starts loading plane z0-1+q; every
+// stage reads only planes written in earlier phases, so one barrier per
+// phase suffices.  A thread's x-y positions (and their wall coefficients)
+// are the same on every plane.  Same per-point arithmetic as
+// stencil_kernel; points outside a walled axis are not computed (lap1 never
+// reads them).  With ACC the result is added to the FP64 vector acc instead
+// of being stored.
 // The axes' periodicity is a template parameter (bit d of PER: axis d
 // periodic), so no branch on it is left in the kernel.  With it as a run-time
-// flag the compiler built the branch masks once, inside the first region
-// loop, and reused them in the later loops: lanes that had already left the
-// first loop held 0 there, and waves whose active lanes were all such lanes
-// took the periodic branch on a walled axis (missing wall terms).
+// flag the compiler built the branch masks once, inside the first of several
+// loops over positions, and reused them in the later loops: lanes that had
+// left the first loop early held 0 there, and waves whose active lanes were
+// all such lanes took the periodic branch on a walled axis.
 // ---------------------------------------------------------------------------
 template <typename T>
-struct cheb_tile;
+struct march_tile;
 template <>
-struct cheb_tile<float> {
-    static constexpr int X = 32, Y = 8, Z = 8;
+struct march_tile<float> {
+    static constexpr int X = 64, Y = 8;
 };
 template <>
-struct cheb_tile<double> {
-    static constexpr int X = 8, Y = 8, Z = 8;
+struct march_tile<double> {
+    static constexpr int X = 32, Y = 8;
 };
-constexpr int CHEB_THREADS = 256;
+constexpr int MARCH_THREADS = 512;
+constexpr int MARCH_NV = 7;      // v planes alive in a phase
+constexpr int MARCH_NS = 4;      // s1 (and s2) planes alive in a phase
+constexpr int MARCH_LEAD = 9;    // phases between loading a v plane and storing its output
+constexpr int MARCH_TARGET_BLOCKS = 512;
+constexpr int MARCH_MIN_Z = 4;
 
-// lap1 with the axis' periodicity fixed at compile time.
+// Global index of coordinate o - h + c along an axis: wrapped on a periodic
+// axis, -1 outside a walled one.
 template <bool P, typename T>
-__device__ inline T lap1_fixed(const op_axis<T>& a, int i, T xm, T x0, T xp)
-{
-    if constexpr (P) {
-        return a.r2 * (xm + xp - 2 * x0);
-    } else {
-        T d = -2;
-        if (i == 0) d += a.lap_lo;
-        if (i == a.n - 1) d += a.lap_hi;
-        T l = i > 0 ? xm : T(0);
-        T r = i < a.n - 1 ? xp : T(0);
-        return a.r2 * (l + r + d * x0);
-    }
-}
-
-// One point of sigma I~ + tau L from a region array S (strides 1, sy, sz)
-// at local offset l, global coordinates (gi, gj, gk).
-template <int PER, typename T>
-__device__ inline T region_op(const op3<T>& A, const T* S, int l, int sy, int sz, int gi, int gj, int gk)
-{
-    const T x0 = S[l];
-    const T lx = lap1_fixed<(PER & 1) != 0>(A.a[0], gi, S[l - 1], x0, S[l + 1]);
-    const T ly = lap1_fixed<(PER & 2) != 0>(A.a[1], gj, S[l - sy], x0, S[l + sy]);
-    const T lz = lap1_fixed<(PER & 4) != 0>(A.a[2], gk, S[l - sz], x0, S[l + sz]);
-    const T Ix = id1(A.a[0], gi), Iy = id1(A.a[1], gj), Iz = id1(A.a[2], gk);
-    return A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
-}
-
-// Global coordinate of region coordinate c along an axis (tile origin o,
-// halo h): wrapped on a periodic axis, -1 outside a walled one.
-template <bool P, typename T>
-__device__ inline int region_global(const op_axis<T>& a, int o, int h, int c)
+__device__ inline int axis_global(const op_axis<T>& a, int o, int h, int c)
 {
     int g = o - h + c;
     if constexpr (P) {
@@ -221,69 +204,254 @@ __device__ inline int region_global(const op_axis<T>& a, int o, int h, int c)
     }
 }
 
-template <typename T, bool ADD, bool ACC, int PER>
-__global__ void __launch_bounds__(CHEB_THREADS)
-cheb_fused_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* __restrict__ out,
-                  double* __restrict__ acc, int tiles_x, int tiles_y, int ntiles, int xcd_chunk)
+// Coefficients of lap1 along one axis at index i: L1 x = r2 (lo xm + hi xp +
+// d x0), with lo / hi 0 where the neighbour lies outside a wall.
+template <bool P, typename T>
+__device__ inline void axis_coef(const op_axis<T>& a, int i, T& d, T& lo, T& hi)
 {
-    constexpr int TX = cheb_tile<T>::X, TY = cheb_tile<T>::Y, TZ = cheb_tile<T>::Z;
-    constexpr int VX = TX + 6, VY = TY + 6, VZ = TZ + 6;
-    __shared__ T Sv[VX * VY * VZ];        // v, then s2 (on the region grown by 1)
-    __shared__ T S1[VX * VY * VZ];
+    d = -2;
+    lo = 1;
+    hi = 1;
+    if constexpr (!P) {
+        if (i == 0) d += a.lap_lo;
+        if (i == a.n - 1) d += a.lap_hi;
+        lo = i > 0 ? T(1) : T(0);
+        hi = i < a.n - 1 ? T(1) : T(0);
+    }
+}
+
+// A position of the x-y plane (the same on every plane of the march): its
+// plane offset (-1: none, or outside a wall) and x / y coefficients.
+template <typename T>
+struct xy_coef {
+    int c;
+    T Ix, Iy, IxIy;
+    T dx, xlo, xhi, dy, ylo, yhi;
+};
+
+template <int PER, typename T>
+__device__ inline xy_coef<T> make_xy_coef(const op3<T>& A, bool valid, int c, int gi, int gj)
+{
+    xy_coef<T> k;
+    k.c = valid ? c : -1;
+    gi = valid ? gi : 0;
+    gj = valid ? gj : 0;
+    k.Ix = id1(A.a[0], gi);
+    k.Iy = id1(A.a[1], gj);
+    k.IxIy = k.Ix * k.Iy;
+    axis_coef<(PER & 1) != 0>(A.a[0], gi, k.dx, k.xlo, k.xhi);
+    axis_coef<(PER & 2) != 0>(A.a[1], gj, k.dy, k.ylo, k.yhi);
+    return k;
+}
+
+// z coefficients of one plane.
+template <typename T>
+struct z_coef {
+    T Iz, dz, zlo, zhi;
+};
+
+template <bool P, typename T>
+__device__ inline z_coef<T> make_z_coef(const op_axis<T>& a, int k)
+{
+    z_coef<T> z;
+    z.Iz = id1(a, k);
+    axis_coef<P>(a, k, z.dz, z.zlo, z.zhi);
+    return z;
+}
+
+// lap1 from coefficients; periodic axes keep lap1's own expression.
+template <bool P, typename T>
+__device__ inline T lap_coef(T r2, T lo, T hi, T d, T xm, T x0, T xp)
+{
+    if constexpr (P) return r2 * (xm + xp - 2 * x0);
+    else return r2 * (lo * xm + hi * xp + d * x0);
+}
+
+// sigma I~ + tau L at position k of the centre plane Sc (row stride SY),
+// Sm / Sp the planes below / above (the expression of stencil_at).
+template <int PER, int SY, typename T>
+__device__ inline T plane_op(const op3<T>& A, const T* Sm, const T* Sc, const T* Sp, const xy_coef<T>& k,
+                             const z_coef<T>& kz)
+{
+    const int c = k.c;
+    const T x0 = Sc[c];
+    const T lx = lap_coef<(PER & 1) != 0>(A.a[0].r2, k.xlo, k.xhi, k.dx, Sc[c - 1], x0, Sc[c + 1]);
+    const T ly = lap_coef<(PER & 2) != 0>(A.a[1].r2, k.ylo, k.yhi, k.dy, Sc[c - SY], x0, Sc[c + SY]);
+    const T lz = lap_coef<(PER & 4) != 0>(A.a[2].r2, kz.zlo, kz.zhi, kz.dz, Sm[c], x0, Sp[c]);
+    return A.sigma * (k.IxIy * kz.Iz) * x0 + A.tau * (k.Iy * kz.Iz * lx + k.Ix * kz.Iz * ly + k.IxIy * lz);
+}
+
+template <typename T, bool ADD, bool ACC, int PER>
+__global__ void __launch_bounds__(MARCH_THREADS)
+cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* __restrict__ out,
+                  double* __restrict__ acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
+{
+    constexpr bool PX = (PER & 1) != 0, PY = (PER & 2) != 0, PZ = (PER & 4) != 0;
+    constexpr int TX = march_tile<T>::X, TY = march_tile<T>::Y;
+    constexpr int VX = TX + 6, VY = TY + 6, VP = VX * VY;
+    constexpr int NL = (VP + MARCH_THREADS - 1) / MARCH_THREADS;
+    constexpr int W1 = TX + 4, P1 = W1 * (TY + 4), N1 = (P1 + MARCH_THREADS - 1) / MARCH_THREADS;
+    constexpr int W2 = TX + 2, P2 = W2 * (TY + 2), N2 = (P2 + MARCH_THREADS - 1) / MARCH_THREADS;
+    constexpr int PO = TX * TY, NO = (PO + MARCH_THREADS - 1) / MARCH_THREADS;
+    constexpr int NZ0 = (2 * MARCH_NS * VP + MARCH_THREADS - 1) / MARCH_THREADS;
+    __shared__ T Rv[MARCH_NV][VP];
+    __shared__ T Rs[2][MARCH_NS][VP];   // s1, s2
+    T(*R1)[VP] = Rs[0];
+    T(*R2)[VP] = Rs[1];
+
     int b = blockIdx.x;
     if (xcd_chunk) b = (b & 7) * xcd_chunk + (b >> 3);
-    if (b >= ntiles) return;
-    const int ox = (b % tiles_x) * TX;
-    const int oy = ((b / tiles_x) % tiles_y) * TY;
-    const int oz = (b / (tiles_x * tiles_y)) * TZ;
+    if (b >= nblocks) return;
+    const int tile = b % tiles_xy, chunk = b / tiles_xy;
+    const int ox = (tile % tiles_x) * TX, oy = (tile / tiles_x) * TY;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
-    constexpr int sy = VX, sz = VX * VY;
+    const size_t nxy = (size_t) nx * ny;
+    const int z0 = chunk * zchunk, z1 = min(z0 + zchunk, nz);
+    const int tid = threadIdx.x;
 
-    // v on the region grown by 3
-    for (int t = threadIdx.x; t < VX * VY * VZ; t += CHEB_THREADS) {
-        const int cx = t % VX, cy = (t / VX) % VY, cz = t / (VX * VY);
-        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
-        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
-        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
-        T val = T(0);
-        if (gi >= 0 && gj >= 0 && gk >= 0) val = v[(size_t) gi + (size_t) nx * ((size_t) gj + (size_t) ny * gk)];
-        Sv[t] = val;
+    // s1 and s2 start at 0 where they are never computed (outside a wall):
+    // the zero neighbour weights there must not meet uninitialised values.
+#pragma unroll
+    for (int r = 0; r < NZ0; ++r) {
+        const int t = tid + r * MARCH_THREADS;
+        if (t < 2 * MARCH_NS * VP) (&Rs[0][0][0])[t] = T(0);
     }
-    __syncthreads();
-    // s1 = w1 v - w0 A v on the region grown by 2 (region offsets 1 .. V-2)
-    for (int t = threadIdx.x; t < (VX - 2) * (VY - 2) * (VZ - 2); t += CHEB_THREADS) {
-        const int cx = 1 + t % (VX - 2), cy = 1 + (t / (VX - 2)) % (VY - 2), cz = 1 + t / ((VX - 2) * (VY - 2));
-        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
-        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
-        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
-        if (gi < 0 || gj < 0 || gk < 0) continue;
-        const int l = cx + sy * cy + sz * cz;
-        S1[l] = -w0 * region_op<PER>(A, Sv, l, sy, sz, gi, gj, gk) + w1 * Sv[l];
+
+    // Positions of this thread, the same on every plane.
+    int lcol[NL];   // v loads: global column gi + nx gj (-1 outside a wall)
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+        const int t = tid + r * MARCH_THREADS;
+        int col = -1;
+        if (t < VP) {
+            const int gi = axis_global<PX>(A.a[0], ox, 3, t % VX);
+            const int gj = axis_global<PY>(A.a[1], oy, 3, t / VX);
+            if (gi >= 0 && gj >= 0) col = gi + nx * gj;
+        }
+        lcol[r] = col;
     }
-    __syncthreads();
-    // s2 = w2 v - A s1 on the region grown by 1
-    for (int t = threadIdx.x; t < (VX - 4) * (VY - 4) * (VZ - 4); t += CHEB_THREADS) {
-        const int cx = 2 + t % (VX - 4), cy = 2 + (t / (VX - 4)) % (VY - 4), cz = 2 + t / ((VX - 4) * (VY - 4));
-        const int gi = region_global<(PER & 1) != 0>(A.a[0], ox, 3, cx);
-        const int gj = region_global<(PER & 2) != 0>(A.a[1], oy, 3, cy);
-        const int gk = region_global<(PER & 4) != 0>(A.a[2], oz, 3, cz);
-        if (gi < 0 || gj < 0 || gk < 0) continue;
-        const int l = cx + sy * cy + sz * cz;
-        Sv[l] = T(-1) * region_op<PER>(A, S1, l, sy, sz, gi, gj, gk) + w2 * Sv[l];
+    xy_coef<T> k1[N1], k2[N2], ko[NO];
+    int ocol[NO];   // output: global column gi + nx gj
+#pragma unroll
+    for (int r = 0; r < N1; ++r) {
+        const int t = tid + r * MARCH_THREADS;
+        const int cx = 1 + t % W1, cy = 1 + t / W1;
+        const int gi = axis_global<PX>(A.a[0], ox, 3, cx), gj = axis_global<PY>(A.a[1], oy, 3, cy);
+        k1[r] = make_xy_coef<PER>(A, t < P1 && gi >= 0 && gj >= 0, cx + VX * cy, gi, gj);
     }
-    __syncthreads();
-    // out = w3 v - A s2 (+ out) on the tile
-    for (int t = threadIdx.x; t < TX * TY * TZ; t += CHEB_THREADS) {
-        const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
-        const int gi = ox + tx, gj = oy + ty, gk = oz + tz;
-        if (gi >= nx || gj >= ny || gk >= nz) continue;
-        const int l = (tx + 3) + sy * (ty + 3) + sz * (tz + 3);
-        const size_t p = (size_t) gi + (size_t) nx * ((size_t) gj + (size_t) ny * gk);
-        T val = T(-1) * region_op<PER>(A, Sv, l, sy, sz, gi, gj, gk);
-        if (ADD) val += T(1) * out[p];
-        val += w3 * v[p];
-        if (ACC) acc[p] += (double) val;
-        else out[p] = val;
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
+        const int t = tid + r * MARCH_THREADS;
+        const int cx = 2 + t % W2, cy = 2 + t / W2;
+        const int gi = axis_global<PX>(A.a[0], ox, 3, cx), gj = axis_global<PY>(A.a[1], oy, 3, cy);
+        k2[r] = make_xy_coef<PER>(A, t < P2 && gi >= 0 && gj >= 0, cx + VX * cy, gi, gj);
+    }
+#pragma unroll
+    for (int r = 0; r < NO; ++r) {
+        const int t = tid + r * MARCH_THREADS;
+        const int tx = t % TX, ty = t / TX;
+        const int gi = ox + tx, gj = oy + ty;
+        const bool valid = t < PO && gi < nx && gj < ny;
+        ko[r] = make_xy_coef<PER>(A, valid, (tx + 3) + VX * (ty + 3), gi, gj);
+        ocol[r] = valid ? gi + nx * gj : 0;
+    }
+
+    // Ring slot of plane z (z >= z0 - 3).
+    auto sv = [&](int z) { return (z - z0 + 3) % MARCH_NV; };
+    auto ss = [&](int z) { return (z - z0 + 3) % MARCH_NS; };
+
+    // Memory operands are loaded two phases ahead of their use, into one of
+    // two register sets (the phase loop is unrolled by two).  Every load is
+    // issued unconditionally, from a valid address (a point outside a wall or
+    // the chunk reads point 0, and is masked where it is used), so that the
+    // waits before the uses count only the loads issued before them.
+    auto fetch = [&](int z, T (&pre)[NL]) {
+        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], z, 0, 0), 0);
+#pragma unroll
+        for (int r = 0; r < NL; ++r) pre[r] = v[(size_t) max(lcol[r], 0) + nxy * k];
+    };
+    auto fetch_out = [&](int z, T (&po)[NO], double (&pa)[NO]) {
+        const size_t k = (size_t) min(max(z, z0), z1 - 1);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) {
+            const size_t p = (size_t) ocol[r] + nxy * k;
+            if (ADD) po[r] = out[p];
+            if (ACC) pa[r] = acc[p];
+        }
+    };
+
+    const int Z = z1 - z0;
+    auto phase = [&](int q, T (&pre)[NL], T (&po)[NO], double (&pa)[NO]) {
+        if (q >= 3 && q <= Z + 6) {          // s1 = w1 v - w0 A v
+            const int z = z0 - 5 + q;
+            const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
+            if (gk >= 0) {
+                const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+                const T *Vm = Rv[sv(z - 1)], *Vc = Rv[sv(z)], *Vp = Rv[sv(z + 1)];
+                T* dst = R1[ss(z)];
+#pragma unroll
+                for (int r = 0; r < N1; ++r) {
+                    if (k1[r].c < 0) continue;
+                    const int c = k1[r].c;
+                    dst[c] = -w0 * plane_op<PER, VX>(A, Vm, Vc, Vp, k1[r], kz) + w1 * Vc[c];
+                }
+            }
+        }
+        if (q >= 6 && q <= Z + 7) {          // s2 = w2 v - A s1
+            const int z = z0 - 7 + q;
+            const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
+            if (gk >= 0) {
+                const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+                const T *Sm = R1[ss(z - 1)], *Sc = R1[ss(z)], *Sp = R1[ss(z + 1)];
+                const T* Vc = Rv[sv(z)];
+                T* dst = R2[ss(z)];
+#pragma unroll
+                for (int r = 0; r < N2; ++r) {
+                    if (k2[r].c < 0) continue;
+                    const int c = k2[r].c;
+                    dst[c] = T(-1) * plane_op<PER, VX>(A, Sm, Sc, Sp, k2[r], kz) + w2 * Vc[c];
+                }
+            }
+        }
+        if (q >= MARCH_LEAD) {               // out = w3 v - A s2 (+ out)
+            const int z = z0 - MARCH_LEAD + q;
+            const z_coef<T> kz = make_z_coef<PZ>(A.a[2], z);
+            const T *Sm = R2[ss(z - 1)], *Sc = R2[ss(z)], *Sp = R2[ss(z + 1)];
+            const T* Vc = Rv[sv(z)];
+#pragma unroll
+            for (int r = 0; r < NO; ++r) {
+                if (ko[r].c < 0) continue;
+                const size_t p = (size_t) ocol[r] + nxy * (size_t) z;
+                T val = T(-1) * plane_op<PER, VX>(A, Sm, Sc, Sp, ko[r], kz);
+                if (ADD) val += T(1) * po[r];
+                val += w3 * Vc[ko[r].c];
+                if (ACC) acc[p] = pa[r] + (double) val;
+                else out[p] = val;
+            }
+        }
+        // The loads are waited for at the end of the phase, after the stages.
+        if (q <= Z + 5) {                    // v plane z0-3+q (0 outside a wall)
+            const bool in = axis_global<PZ>(A.a[2], z0 - 3 + q, 0, 0) >= 0;
+            T* dst = Rv[sv(z0 - 3 + q)];
+#pragma unroll
+            for (int r = 0; r < NL; ++r) {
+                const int t = tid + r * MARCH_THREADS;
+                if (t < VP) dst[t] = (in && lcol[r] >= 0) ? pre[r] : T(0);
+            }
+        }
+        fetch(z0 - 1 + q, pre);              // v plane stored two phases on
+        if (ADD || ACC) fetch_out(z0 - MARCH_LEAD + 2 + q, po, pa);   // output operands of phase q + 2
+        __syncthreads();
+    };
+
+    T preA[NL], preB[NL], poA[NO], poB[NO];
+    double paA[NO], paB[NO];
+    fetch(z0 - 3, preA);
+    fetch(z0 - 2, preB);
+    const int nq = Z + MARCH_LEAD;
+    for (int q = 0; q < nq; q += 2) {
+        phase(q, preA, poA, paA);
+        if (q + 1 < nq) phase(q + 1, preB, poB, paB);
     }
 }
 
@@ -786,12 +954,13 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
 // One instantiation per periodicity pattern of the three axes.
 template <typename T, bool ADD, bool ACC>
 static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A, T w0, T w1, T w2, T w3, const T* v,
-                        T* out, double* acc, int tx, int ty, int ntiles, int xcd_chunk)
+                        T* out, double* acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
 {
-#define IBM_CHEB_CASE(P)                                                                                            \
-    case P:                                                                                                         \
-        cheb_fused_kernel<T, ADD, ACC, P><<<blocks, CHEB_THREADS, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, tx, ty, \
-                                                                              ntiles, xcd_chunk);                   \
+#define IBM_CHEB_CASE(P)                                                                                    \
+    case P:                                                                                                 \
+        cheb_march_kernel<T, ADD, ACC, P><<<blocks, MARCH_THREADS, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, \
+                                                                               tiles_x, tiles_xy, zchunk,   \
+                                                                               nblocks, xcd_chunk);         \
         break;
     switch (per) {
         IBM_CHEB_CASE(0)
@@ -806,16 +975,26 @@ static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A
 #undef IBM_CHEB_CASE
 }
 
+bool cheb_fused_fits(int nx, int ny)
+{
+    return nx >= 1 && ny >= 1 && nx <= 32767 && ny <= 32767;   // packed x-y coordinates
+}
+
 template <typename T>
 ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
                              double* acc)
 {
-    constexpr int TX = cheb_tile<T>::X, TY = cheb_tile<T>::Y, TZ = cheb_tile<T>::Z;
+    constexpr int TX = march_tile<T>::X, TY = march_tile<T>::Y;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
-    const int tx = (nx + TX - 1) / TX, ty = (ny + TY - 1) / TY, tz = (nz + TZ - 1) / TZ;
-    const int ntiles = tx * ty * tz;
-    const int blocks = (ntiles + 7) / 8 * 8;
+    if (!cheb_fused_fits(nx, ny)) return fail(ctx, IBM_ERR_ARG, "fused smoothing: %d x %d planes too wide", nx, ny);
+    const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
+    // Enough z chunks for about MARCH_TARGET_BLOCKS blocks; each chunk costs
+    // MARCH_LEAD phases of pipeline fill.
+    int zchunk = (int) (((long long) nz * tiles_xy + MARCH_TARGET_BLOCKS - 1) / MARCH_TARGET_BLOCKS);
+    zchunk = std::min(nz, std::max(zchunk, MARCH_MIN_Z));
+    const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
+    const int blocks = (nblocks + 7) / 8 * 8;
     const int xcd_chunk = blocks / 8;
     const double n = (double) nx * ny * nz;
     // v read, out written (read too when adding; FP64 read-modify-write with acc)
@@ -824,11 +1003,14 @@ ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], c
     const T w0 = (T) w[0], w1 = (T) w[1], w2 = (T) w[2], w3 = (T) w[3];
     const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
     if (acc)
-        cheb_launch<T, true, true>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+        cheb_launch<T, true, true>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+                                   nblocks, xcd_chunk);
     else if (add)
-        cheb_launch<T, true, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+        cheb_launch<T, true, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+                                    nblocks, xcd_chunk);
     else
-        cheb_launch<T, false, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tx, ty, ntiles, xcd_chunk);
+        cheb_launch<T, false, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy,
+                                     zchunk, nblocks, xcd_chunk);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }

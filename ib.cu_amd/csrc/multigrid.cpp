@@ -133,9 +133,9 @@ std::vector<level_bufs<double>>& bufs<double>(ibm_mg* m) { return m->b64; }
 template <>
 std::vector<level_bufs<float>>& bufs<float>(ibm_mg* m) { return m->b32; }
 
-// Smoothing passes fused into one tiled kernel (launch_cheb_fused) or run as
-// three stencil passes.
-constexpr bool FUSED_SMOOTHING = false;
+// Smoothing in one z-marching kernel (launch_cheb_fused) or as three stencil
+// passes.
+constexpr bool FUSED_SMOOTHING = true;
 
 // out = p(A) v, the degree-4 Chebyshev polynomial by Horner's rule
 // (algo/chebyshev.h:149-163): s0 = w0 v, s_k = w_k v - A s_{k-1}.
@@ -146,7 +146,8 @@ template <typename T>
 static ibm_status cheb_apply(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* s1, T* s2,
                              T* out, bool add, double* acc = nullptr)
 {
-    if (FUSED_SMOOTHING) return launch_cheb_fused<T>(ctx, A, w, v, out, add, acc);
+    if (FUSED_SMOOTHING && cheb_fused_fits(A.a[0].n, A.a[1].n))
+        return launch_cheb_fused<T>(ctx, A, w, v, out, add, acc);
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -w[0], v, 0, nullptr, (T) w[1], v, s1));
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, s1, 0, nullptr, (T) w[2], v, s2));
     if (add && acc) return launch_stencil_acc<T>(ctx, A, (T) -1, s2, (T) 1, out, (T) w[3], v, acc);
