@@ -172,22 +172,22 @@ stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __re
 // left the first loop early held 0 there, and waves whose active lanes were
 // all such lanes took the periodic branch on a walled axis.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct march_tile;
-template <>
-struct march_tile<float> {
-    static constexpr int X = 64, Y = 8;
+// Tile configurations: WIDE (64 x 8 FP32 / 32 x 8 FP64 columns, 512 threads)
+// for large grids; narrow (16 x 8 columns, 256 threads) for the coarser
+// grids, whose launches are bound by the phases' latency, not by throughput:
+// fewer points per thread and phase, and no wrapped copies of narrow x axes.
+template <typename T, bool WIDE>
+struct march_tile {
+    static constexpr int X = WIDE ? (sizeof(T) == 4 ? 64 : 32) : 16;
+    static constexpr int Y = 8;
+    static constexpr int THREADS = WIDE ? 512 : 256;
+    static constexpr int MIN_Z = WIDE ? 4 : 2;   // shortest z chunk
 };
-template <>
-struct march_tile<double> {
-    static constexpr int X = 32, Y = 8;
-};
-constexpr int MARCH_THREADS = 512;
+constexpr size_t MARCH_WIDE_POINTS = (size_t) 1 << 19;   // grids from 512K points use the wide tile
 constexpr int MARCH_NV = 7;      // v planes alive in a phase
 constexpr int MARCH_NS = 4;      // s1 (and s2) planes alive in a phase
 constexpr int MARCH_LEAD = 9;    // phases between loading a v plane and storing its output
 constexpr int MARCH_TARGET_BLOCKS = 512;
-constexpr int MARCH_MIN_Z = 4;
 
 // Global index of coordinate o - h + c along an axis: wrapped on a periodic
 // axis, -1 outside a walled one.
@@ -281,13 +281,14 @@ __device__ inline T plane_op(const op3<T>& A, const T* Sm, const T* Sc, const T*
     return A.sigma * (k.IxIy * kz.Iz) * x0 + A.tau * (k.Iy * kz.Iz * lx + k.Ix * kz.Iz * ly + k.IxIy * lz);
 }
 
-template <typename T, bool ADD, bool ACC, int PER>
-__global__ void __launch_bounds__(MARCH_THREADS)
+template <typename T, bool ADD, bool ACC, int PER, bool WIDE>
+__global__ void __launch_bounds__((march_tile<T, WIDE>::THREADS))
 cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* __restrict__ out,
                   double* __restrict__ acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
 {
     constexpr bool PX = (PER & 1) != 0, PY = (PER & 2) != 0, PZ = (PER & 4) != 0;
-    constexpr int TX = march_tile<T>::X, TY = march_tile<T>::Y;
+    constexpr int TX = march_tile<T, WIDE>::X, TY = march_tile<T, WIDE>::Y;
+    constexpr int MARCH_THREADS = march_tile<T, WIDE>::THREADS;
     constexpr int VX = TX + 6, VY = TY + 6, VP = VX * VY;
     constexpr int NL = (VP + MARCH_THREADS - 1) / MARCH_THREADS;
     constexpr int W1 = TX + 4, P1 = W1 * (TY + 4), N1 = (P1 + MARCH_THREADS - 1) / MARCH_THREADS;
@@ -952,15 +953,16 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
 }
 
 // One instantiation per periodicity pattern of the three axes.
-template <typename T, bool ADD, bool ACC>
+template <typename T, bool ADD, bool ACC, bool WIDE>
 static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A, T w0, T w1, T w2, T w3, const T* v,
                         T* out, double* acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
 {
-#define IBM_CHEB_CASE(P)                                                                                    \
-    case P:                                                                                                 \
-        cheb_march_kernel<T, ADD, ACC, P><<<blocks, MARCH_THREADS, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, \
-                                                                               tiles_x, tiles_xy, zchunk,   \
-                                                                               nblocks, xcd_chunk);         \
+    constexpr int threads = march_tile<T, WIDE>::THREADS;
+#define IBM_CHEB_CASE(P)                                                                                     \
+    case P:                                                                                                  \
+        cheb_march_kernel<T, ADD, ACC, P, WIDE><<<blocks, threads, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, \
+                                                                               tiles_x, tiles_xy, zchunk,    \
+                                                                               nblocks, xcd_chunk);          \
         break;
     switch (per) {
         IBM_CHEB_CASE(0)
@@ -975,6 +977,33 @@ static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A
 #undef IBM_CHEB_CASE
 }
 
+template <typename T, bool WIDE>
+static void cheb_configure(const op3<T>& A, const double w[4], const T* v, T* out, bool add, double* acc,
+                           hipStream_t stream)
+{
+    constexpr int TX = march_tile<T, WIDE>::X, TY = march_tile<T, WIDE>::Y;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
+    // Enough z chunks for about MARCH_TARGET_BLOCKS blocks; each chunk costs
+    // MARCH_LEAD phases of pipeline fill.
+    int zchunk = (int) (((long long) nz * tiles_xy + MARCH_TARGET_BLOCKS - 1) / MARCH_TARGET_BLOCKS);
+    zchunk = std::min(nz, std::max(zchunk, march_tile<T, WIDE>::MIN_Z));
+    const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
+    const int blocks = (nblocks + 7) / 8 * 8;
+    const int xcd_chunk = blocks / 8;
+    const T w0 = (T) w[0], w1 = (T) w[1], w2 = (T) w[2], w3 = (T) w[3];
+    const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
+    if (acc)
+        cheb_launch<T, true, true, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+                                         nblocks, xcd_chunk);
+    else if (add)
+        cheb_launch<T, true, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+                                          nblocks, xcd_chunk);
+    else
+        cheb_launch<T, false, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy,
+                                           zchunk, nblocks, xcd_chunk);
+}
+
 bool cheb_fused_fits(int nx, int ny)
 {
     return nx >= 1 && ny >= 1 && nx <= 32767 && ny <= 32767;   // packed x-y coordinates
@@ -984,33 +1013,15 @@ template <typename T>
 ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
                              double* acc)
 {
-    constexpr int TX = march_tile<T>::X, TY = march_tile<T>::Y;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
     if (!cheb_fused_fits(nx, ny)) return fail(ctx, IBM_ERR_ARG, "fused smoothing: %d x %d planes too wide", nx, ny);
-    const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
-    // Enough z chunks for about MARCH_TARGET_BLOCKS blocks; each chunk costs
-    // MARCH_LEAD phases of pipeline fill.
-    int zchunk = (int) (((long long) nz * tiles_xy + MARCH_TARGET_BLOCKS - 1) / MARCH_TARGET_BLOCKS);
-    zchunk = std::min(nz, std::max(zchunk, MARCH_MIN_Z));
-    const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
-    const int blocks = (nblocks + 7) / 8 * 8;
-    const int xcd_chunk = blocks / 8;
     const double n = (double) nx * ny * nz;
     // v read, out written (read too when adding; FP64 read-modify-write with acc)
     const double bytes = n * (sizeof(T) + (acc ? 16.0 + sizeof(T) : (add ? 2.0 : 1.0) * sizeof(T)));
     prof_scope prof(ctx, sizeof(T) == 8 ? "chebyshev_f64" : "chebyshev_f32", bytes);
-    const T w0 = (T) w[0], w1 = (T) w[1], w2 = (T) w[2], w3 = (T) w[3];
-    const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
-    if (acc)
-        cheb_launch<T, true, true>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
-                                   nblocks, xcd_chunk);
-    else if (add)
-        cheb_launch<T, true, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
-                                    nblocks, xcd_chunk);
-    else
-        cheb_launch<T, false, false>(per, blocks, ctx->stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy,
-                                     zchunk, nblocks, xcd_chunk);
+    if ((size_t) nx * ny * nz >= MARCH_WIDE_POINTS) cheb_configure<T, true>(A, w, v, out, add, acc, ctx->stream);
+    else cheb_configure<T, false>(A, w, v, out, add, acc, ctx->stream);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
