@@ -198,6 +198,75 @@ def cpu_baseline(inputs, orc, fd, rbf, L, types, gpu_iters):
                        f"the GPU's counts); total {total:.0f} s per step")}
 
 
+# Kernel-name prefixes of the counter files for each timed kernel group.
+PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "cheb_march_kernel<double",
+               "stencil_f32": "stencil_kernel<float", "stencil_f64": "stencil_kernel<double",
+               "restrict": "restrict_kernel", "prolong": "prolong_add_kernel", "rbf_gemm": "rbf_gemm_kernel",
+               "spread_brick": "spread_brick_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
+COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
+
+
+def pmc_probe(torch, ibm, L, types):
+    """A known copy (the counters' calibration), then one adaptive and one
+    fixed-count C4 step in the headline form."""
+    x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    print("copy done", file=sys.stderr, flush=True)
+    stream = torch.cuda.Stream()
+    ctx = ibm.Context(0, stream=stream)
+    s, st, ub, _keep = make_solver(ibm, ctx, L, types, 32)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        s.step(st, boundary=ub)
+        s.step_fixed(st, boundary=ub)
+    torch.cuda.synchronize()
+    print("steps done", file=sys.stderr, flush=True)
+
+
+def pmc_summary(d):
+    """Per-kernel HBM bytes per launch from two rocprofv3 --pmc runs (FETCH_SIZE
+    and WRITE_SIZE, KiB per dispatch), each scaled so that the 256 MiB copy
+    reads 256 MiB: bytes = 1024 * (fetch_KiB * f_fetch + write_KiB * f_write)."""
+    import csv
+    import glob
+
+    def load(kind, counter):
+        (path,) = glob.glob(os.path.join(d, f"{kind}_step_counter_collection.csv"))
+        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(open(path))
+                if r["Counter_Name"] == counter]
+    fetch, write = load("fetch", "FETCH_SIZE"), load("write", "WRITE_SIZE")
+    copy_f = max(v for n, v in fetch if "copyBuffer" in n)
+    copy_w = max(v for n, v in write if "copyBuffer" in n)
+    f_fetch = COPY_BYTES / 1024 / copy_f     # one dispatch copies the whole buffer
+    f_write = COPY_BYTES / 1024 / copy_w
+    print(f"calibration: copy dispatch fetch {copy_f:.1f} KiB, write {copy_w:.1f} KiB of "
+          f"{COPY_BYTES // 1024} KiB moved each way -> f_fetch {f_fetch:.4f}, f_write {f_write:.4f}")
+    for name, prefix in PMC_KERNELS.items():
+        fs = [v for n, v in fetch if prefix in n]
+        ws = [v for n, v in write if prefix in n]
+        if not fs or not ws:
+            continue
+        per = 1024 * (sum(fs) / len(fs) * f_fetch + sum(ws) / len(ws) * f_write)
+        print(f"{name}: launches {len(fs)} fetch {sum(fs) / len(fs):.1f} KiB write {sum(ws) / len(ws):.1f} KiB "
+              f"-> traffic_bytes_per_launch: {per:.0f}")
+
+
+def recorded_traffic(kernel):
+    """HBM bytes per launch of `kernel` as recorded from the counter runs
+    (the --pmc-summary output kept in profiles/c4_step_kernels_hbm_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "c4_step_kernels_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"{kernel}:") and "traffic_bytes_per_launch:" in line:
+            return {"bytes_per_launch": float(line.rsplit(":", 1)[1]), "source": os.path.relpath(path, ROOT)}
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +274,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 form")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run a 256 MiB copy and two C4 steps only (the program rocprofv3 --pmc profiles)")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="print per-kernel HBM traffic from DIR/fetch_step_* and DIR/write_step_* counter files")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the multi-GPU "
                                    "decomposition is a design only)"}))
@@ -219,6 +295,10 @@ def main():
     t_setup = time.perf_counter()
     L, types = c4_cells(inputs, surfaces)
     print(f"setup: RBF operators built in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    if args.pmc_probe:
+        pmc_probe(torch, ibm, L, types)
+        return
 
     stream = torch.cuda.Stream()
     ctx = ibm.Context(0, stream=stream)
@@ -239,8 +319,11 @@ def main():
     else:
         achieved = d["work_per_step"] / (d["ms_per_step"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    traffic = recorded_traffic(dom_name)
     roofline = {"bound": roof["bound"], "kernel": dom_name, "achieved": achieved, "peak": roof["peak"],
-                "unit": roof["unit"], "frac": achieved / roof["peak"], "traffic": None,
+                "unit": roof["unit"], "frac": achieved / roof["peak"],
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
                 "algorithmic_per_launch": d["work_per_step"] / d["launches_per_step"],
                 "avg_launch_ms": d["ms_per_step"] / d["launches_per_step"]}
     per_kernel = {}
