@@ -26,6 +26,10 @@ ibm_status launch_stencil_acc(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x
                               const T* z, double* acc);
 template <typename T>
 ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T* coarse);
+// coarse = R (b - A x) in one pass (A on the fine grid of R).
+template <typename T>
+ibm_status launch_residual_restrict(ibm_ctx* ctx, const op3<T>& A, const transfer3<T>& R, const T* x, const T* b,
+                                    T* coarse);
 template <typename T>
 ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine);
 // y = a x + b y

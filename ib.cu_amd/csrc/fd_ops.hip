@@ -523,6 +523,201 @@ __global__ void restrict_kernel(transfer3<T> R, const T* __restrict__ fine, T* _
     coarse[t] = restrict_at(R, fine, (unsigned) t);
 }
 
+// ---------------------------------------------------------------------------
+// Residual and restriction in one pass: coarse = R (b - A x).  A block owns a
+// CX x CY tile of coarse columns (fine columns 2 cx0 - 1 .. 2 (cx0 + CX)) and
+// a chunk of coarse planes, and marches through the fine planes f it needs.
+// Phase q computes the fine residual plane f = fs + q into an LDS ring (from
+// a 4-plane ring of x with its x-y halo), then each thread with a coarse
+// column restricts the previous residual plane over its 4 x 4 x-y taps and
+// keeps the last four such partial sums in registers; a coarse plane k is
+// emitted once its last fine plane is in (every second fine plane).  Taps
+// fold at walls as restrict_at does (index -1 -> 0 times int_lo, n -> n - 1
+// times int_hi) and wrap on periodic axes; sums run in restrict_at's order.
+// ---------------------------------------------------------------------------
+template <typename T, bool WIDE>
+struct rr_tile {
+    static constexpr int CX = WIDE ? 32 : 8, CY = 4;
+    static constexpr int THREADS = WIDE ? 256 : 128;
+};
+constexpr int RR_TARGET_BLOCKS = 1024;
+
+// Fine tap of coarse index c along an axis: region offset from fine
+// coordinate o (unwrapped on periodic axes, folded on walled ones) and weight.
+template <typename T>
+__device__ inline void rr_tap(const transfer_axis<T>& a, int c, int s, int o, int& off, T& w)
+{
+    w = a.w[s] / 2;
+    int f = 2 * c + s - 1;
+    if (!a.periodic) f = fold(a, f, a.nf, w);
+    off = f - o;
+}
+
+template <typename T, int PER, bool WIDE>
+__global__ void __launch_bounds__((rr_tile<T, WIDE>::THREADS))
+residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, const T* __restrict__ b,
+                         T* __restrict__ coarse, int tiles_x, int tiles_xy, int kchunk, int nblocks, int xcd_chunk)
+{
+    constexpr bool PX = (PER & 1) != 0, PY = (PER & 2) != 0, PZ = (PER & 4) != 0;
+    constexpr int CX = rr_tile<T, WIDE>::CX, CY = rr_tile<T, WIDE>::CY, NT = rr_tile<T, WIDE>::THREADS;
+    constexpr int XX = 2 * CX + 4, XY = 2 * CY + 4, XP = XX * XY;      // x plane: fine tile grown by 2
+    constexpr int RX = 2 * CX + 2, RP = RX * (2 * CY + 2);              // residual plane: grown by 1
+    constexpr int NL = (XP + NT - 1) / NT, NR = (RP + NT - 1) / NT;
+    static_assert(CX * CY <= NT, "one coarse column per thread");
+    __shared__ T Xr[4][XP];
+    __shared__ T Rr[2][XP];
+
+    int blk = blockIdx.x;
+    if (xcd_chunk) blk = (blk & 7) * xcd_chunk + (blk >> 3);
+    if (blk >= nblocks) return;
+    const int tile = blk % tiles_xy, chunk = blk / tiles_xy;
+    const int cx0 = (tile % tiles_x) * CX, cy0 = (tile / tiles_x) * CY;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const size_t nxy = (size_t) nx * ny;
+    const int ncx = R.a[0].nc, ncy = R.a[1].nc, ncz = R.a[2].nc;
+    const int k0 = chunk * kchunk, k1 = min(k0 + kchunk, ncz);
+    const int tid = threadIdx.x;
+    const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;    // fine origin of the x plane region
+
+    // x loads
+    int lcol[NL];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+        const int t = tid + r * NT;
+        int col = -1;
+        if (t < XP) {
+            const int gi = axis_global<PX>(A.a[0], fx0, 0, t % XX), gj = axis_global<PY>(A.a[1], fy0, 0, t / XX);
+            if (gi >= 0 && gj >= 0) col = gi + nx * gj;
+        }
+        lcol[r] = col;
+    }
+    // residual positions (x-plane layout, offset by the halo of 1)
+    xy_coef<T> kr[NR];
+    int rcol[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int t = tid + r * NT;
+        const int i = 1 + t % RX, j = 1 + t / RX;
+        const int gi = axis_global<PX>(A.a[0], fx0, 0, i), gj = axis_global<PY>(A.a[1], fy0, 0, j);
+        const bool valid = t < RP && gi >= 0 && gj >= 0;
+        kr[r] = make_xy_coef<PER>(A, valid, i + XX * j, gi, gj);
+        rcol[r] = valid ? gi + nx * gj : 0;
+    }
+    // this thread's coarse column and its x-y taps
+    const int ci = cx0 + tid % CX, cj = cy0 + tid / CX;
+    const bool has_c = tid < CX * CY && ci < ncx && cj < ncy;
+    int tx[4], ty[4];
+    T wx[4], wy[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        rr_tap(R.a[0], has_c ? ci : cx0, s, fx0, tx[s], wx[s]);
+        rr_tap(R.a[1], has_c ? cj : cy0, s, fy0, ty[s], wy[s]);
+    }
+
+    // fine planes of the chunk: fs .. fe (unwrapped on a periodic z axis)
+    const int nfz = R.a[2].nf;
+    const int fs = PZ ? 2 * k0 - 1 : max(2 * k0 - 1, 0);
+    const int fe = PZ ? 2 * k1 : min(2 * k1, nfz - 1);
+    auto emit_plane = [&](int k) { return PZ ? 2 * k + 2 : min(2 * k + 2, nfz - 1); };
+    auto sx = [&](int f) { return (f - fs + 1) & 3; };     // x ring slot of plane f >= fs - 1
+
+    auto fetch_x = [&](int f, T (&pre)[NL]) {
+        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+#pragma unroll
+        for (int r = 0; r < NL; ++r) pre[r] = x[(size_t) max(lcol[r], 0) + nxy * k];
+    };
+    auto store_x = [&](int f, const T (&pre)[NL]) {
+        const bool in = axis_global<PZ>(A.a[2], f, 0, 0) >= 0;
+        T* dst = Xr[sx(f)];
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            const int t = tid + r * NT;
+            if (t < XP) dst[t] = (in && lcol[r] >= 0) ? pre[r] : T(0);
+        }
+    };
+    auto fetch_b = [&](int f, T (&pb)[NR]) {
+        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) pb[r] = b[(size_t) rcol[r] + nxy * k];
+    };
+
+    // x planes fs - 1 .. fs + 1 before the march
+    {
+        T p0[NL], p1[NL], p2[NL];
+        fetch_x(fs - 1, p0);
+        fetch_x(fs, p1);
+        fetch_x(fs + 1, p2);
+        store_x(fs - 1, p0);
+        store_x(fs, p1);
+        store_x(fs + 1, p2);
+    }
+    T P0 = 0, P1 = 0, P2 = 0, P3 = 0;      // partial sums of fine planes f-3 .. f
+    int next_k = k0;
+    const size_t ncxy = (size_t) ncx * ncy;
+    const size_t ccol = has_c ? (size_t) ci + (size_t) ncx * cj : 0;
+
+    auto phase = [&](int q, T (&px)[NL], T (&pb)[NR]) {
+        const int fp = fs + q - 1;           // residual plane of the previous phase
+        if (q >= 1) {
+            const T* Rc = Rr[(q - 1) & 1];
+            T acc = 0;
+#pragma unroll
+            for (int sy = 0; sy < 4; ++sy) {
+                const T* row = Rc + XX * ty[sy];
+                const T ax = wx[0] * row[tx[0]] + wx[1] * row[tx[1]] + wx[2] * row[tx[2]] + wx[3] * row[tx[3]];
+                acc += wy[sy] * ax;
+            }
+            P0 = P1;
+            P1 = P2;
+            P2 = P3;
+            P3 = acc;
+            // coarse planes whose last fine plane is fp
+            while (next_k < k1 && emit_plane(next_k) == fp) {
+                const int k = next_k++;
+                T out = 0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    T wz = R.a[2].w[s] / 2;
+                    int f = 2 * k + s - 1;
+                    if (!PZ) f = fold(R.a[2], f, nfz, wz);
+                    const int d = fp - f;                 // 0 .. 3
+                    const T pv = d == 0 ? P3 : d == 1 ? P2 : d == 2 ? P1 : P0;
+                    out += wz * pv;
+                }
+                if (has_c) coarse[ccol + ncxy * (size_t) k] = out;
+            }
+        }
+        const int f = fs + q;                // residual plane of this phase
+        if (f <= fe) {
+            const int gk = axis_global<PZ>(A.a[2], f, 0, 0);
+            const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+            const T *Xm = Xr[sx(f - 1)], *Xc = Xr[sx(f)], *Xp = Xr[sx(f + 1)];
+            T* dst = Rr[q & 1];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (kr[r].c < 0) continue;
+                dst[kr[r].c] = T(-1) * plane_op<PER, XX>(A, Xm, Xc, Xp, kr[r], kz) + T(1) * pb[r];
+            }
+        }
+        store_x(f + 2, px);                  // fetched two phases ago
+        fetch_x(f + 4, px);
+        fetch_b(f + 2, pb);
+        __syncthreads();
+    };
+
+    T pxA[NL], pxB[NL], pbA[NR], pbB[NR];
+    fetch_x(fs + 2, pxA);
+    fetch_x(fs + 3, pxB);
+    fetch_b(fs, pbA);
+    fetch_b(fs + 1, pbB);
+    __syncthreads();
+    const int nq = fe - fs + 2;
+    for (int q = 0; q < nq; q += 2) {
+        phase(q, pxA, pbA);
+        if (q + 1 < nq) phase(q + 1, pxB, pbB);
+    }
+}
+
 // The coarse-to-fine interpolation of one fine point.
 template <typename T>
 __device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coarse, unsigned t)
@@ -1054,6 +1249,57 @@ ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T
     return IBM_OK;
 }
 
+template <typename T, bool WIDE>
+static void rr_configure(const op3<T>& A, const transfer3<T>& R, const T* x, const T* b, T* coarse,
+                         hipStream_t stream)
+{
+    constexpr int CX = rr_tile<T, WIDE>::CX, CY = rr_tile<T, WIDE>::CY;
+    const int ncx = R.a[0].nc, ncy = R.a[1].nc, ncz = R.a[2].nc;
+    const int tiles_x = (ncx + CX - 1) / CX, tiles_xy = tiles_x * ((ncy + CY - 1) / CY);
+    int kchunk = (int) (((long long) ncz * tiles_xy + RR_TARGET_BLOCKS - 1) / RR_TARGET_BLOCKS);
+    kchunk = std::min(ncz, std::max(kchunk, 2));
+    const int nblocks = tiles_xy * ((ncz + kchunk - 1) / kchunk);
+    const int blocks = (nblocks + 7) / 8 * 8;
+    const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
+    constexpr int threads = rr_tile<T, WIDE>::THREADS;
+#define IBM_RR_CASE(P)                                                                                       \
+    case P:                                                                                                  \
+        residual_restrict_kernel<T, P, WIDE><<<blocks, threads, 0, stream>>>(A, R, x, b, coarse, tiles_x,    \
+                                                                            tiles_xy, kchunk, nblocks,       \
+                                                                            blocks / 8);                     \
+        break;
+    switch (per) {
+        IBM_RR_CASE(0)
+        IBM_RR_CASE(1)
+        IBM_RR_CASE(2)
+        IBM_RR_CASE(3)
+        IBM_RR_CASE(4)
+        IBM_RR_CASE(5)
+        IBM_RR_CASE(6)
+        IBM_RR_CASE(7)
+    }
+#undef IBM_RR_CASE
+}
+
+template <typename T>
+ibm_status launch_residual_restrict(ibm_ctx* ctx, const op3<T>& A, const transfer3<T>& R, const T* x, const T* b,
+                                    T* coarse)
+{
+    const size_t nf = (size_t) R.a[0].nf * R.a[1].nf * R.a[2].nf;
+    const size_t nc = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
+    if (nf > MAX_GRID_POINTS || (size_t) R.a[0].nf * R.a[1].nf > 0x7fffffff)
+        return fail(ctx, IBM_ERR_SIZE, "grid too large");
+    for (int d = 0; d < 3; ++d)
+        if (R.a[d].nf != A.a[d].n || (R.a[d].periodic != 0) != (A.a[d].periodic != 0))
+            return fail(ctx, IBM_ERR_ARG, "residual/restriction: operator and transfer grids differ");
+    // x and b read, coarse written
+    prof_scope prof(ctx, "residual_restrict", (double) sizeof(T) * (2 * nf + nc));
+    if (nf >= MARCH_WIDE_POINTS) rr_configure<T, true>(A, R, x, b, coarse, ctx->stream);
+    else rr_configure<T, false>(A, R, x, b, coarse, ctx->stream);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
 template <typename T>
 ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine)
 {
@@ -1204,6 +1450,7 @@ ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes)
     template ibm_status launch_stencil_acc<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, double*); \
     template ibm_status launch_cheb_fused<T>(ibm_ctx*, const op3<T>&, const double*, const T*, T*, bool, double*); \
     template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
+    template ibm_status launch_residual_restrict<T>(ibm_ctx*, const op3<T>&, const transfer3<T>&, const T*, const T*, T*); \
     template ibm_status launch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);            \
     template ibm_status launch_fill<T>(ibm_ctx*, size_t, T, T*);                                       \
     template ibm_status launch_sum<T>(ibm_ctx*, size_t, const T*, double*, double*);                   \

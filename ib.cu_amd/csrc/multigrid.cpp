@@ -136,6 +136,9 @@ std::vector<level_bufs<float>>& bufs<float>(ibm_mg* m) { return m->b32; }
 // Smoothing in one z-marching kernel (launch_cheb_fused) or as three stencil
 // passes.
 constexpr bool FUSED_SMOOTHING = true;
+// Residual and restriction of the down-leg in one kernel
+// (launch_residual_restrict) or as a stencil pass and a restriction.
+constexpr bool FUSED_RESTRICT = true;
 
 // out = p(A) v, the degree-4 Chebyshev polynomial by Horner's rule
 // (algo/chebyshev.h:149-163): s0 = w0 v, s_k = w_k v - A s_{k-1}.
@@ -196,8 +199,12 @@ static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l, double* acc = nullpt
     IBM_TRY(cheb_apply<T>(ctx, A, L.w, B.b, B.s1, B.s2, B.x, false));
     if (l + 1 == m->lv.size()) return IBM_OK;
     level_bufs<T>& C = bufs<T>(m)[l + 1];
-    IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
-    IBM_TRY(launch_restrict<T>(ctx, level_tr<T>(L), B.r, C.b));
+    if (FUSED_RESTRICT) {
+        IBM_TRY(launch_residual_restrict<T>(ctx, A, level_tr<T>(L), B.x, B.b, C.b));
+    } else {
+        IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
+        IBM_TRY(launch_restrict<T>(ctx, level_tr<T>(L), B.r, C.b));
+    }
     IBM_TRY(vcycle<T>(ctx, m, l + 1));
     IBM_TRY(launch_prolong_add<T>(ctx, level_tr<T>(L), C.x, B.x));
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
