@@ -227,6 +227,10 @@ struct xy_coef {
     int c;
     T Ix, Iy, IxIy;
     T dx, xlo, xhi, dy, ylo, yhi;
+    // The operator on a plane away from z walls (Iz = 1, lz = r2z (zm + zp -
+    // 2 x0)), collapsed to seven coefficients:
+    // A x = a0 x0 + axm xm + axp xp + aym ym + ayp yp + az (zm + zp).
+    T a0, axm, axp, aym, ayp, az;
 };
 
 template <int PER, typename T>
@@ -241,6 +245,13 @@ __device__ inline xy_coef<T> make_xy_coef(const op3<T>& A, bool valid, int c, in
     k.IxIy = k.Ix * k.Iy;
     axis_coef<(PER & 1) != 0>(A.a[0], gi, k.dx, k.xlo, k.xhi);
     axis_coef<(PER & 2) != 0>(A.a[1], gj, k.dy, k.ylo, k.yhi);
+    const T tx = A.tau * k.Iy * A.a[0].r2, ty = A.tau * k.Ix * A.a[1].r2, tz = A.tau * k.IxIy * A.a[2].r2;
+    k.a0 = A.sigma * k.IxIy + tx * k.dx + ty * k.dy - 2 * tz;
+    k.axm = tx * k.xlo;
+    k.axp = tx * k.xhi;
+    k.aym = ty * k.ylo;
+    k.ayp = ty * k.yhi;
+    k.az = tz;
     return k;
 }
 
@@ -279,6 +290,25 @@ __device__ inline T plane_op(const op3<T>& A, const T* Sm, const T* Sc, const T*
     const T ly = lap_coef<(PER & 2) != 0>(A.a[1].r2, k.ylo, k.yhi, k.dy, Sc[c - SY], x0, Sc[c + SY]);
     const T lz = lap_coef<(PER & 4) != 0>(A.a[2].r2, kz.zlo, kz.zhi, kz.dz, Sm[c], x0, Sp[c]);
     return A.sigma * (k.IxIy * kz.Iz) * x0 + A.tau * (k.Iy * kz.Iz * lx + k.Ix * kz.Iz * ly + k.IxIy * lz);
+}
+
+// The same on a plane away from z walls, from the collapsed coefficients.
+template <int SY, typename T>
+__device__ inline T plane_op_int(const T* Sm, const T* Sc, const T* Sp, const xy_coef<T>& k)
+{
+    const int c = k.c;
+    return k.a0 * Sc[c] + k.axm * Sc[c - 1] + k.axp * Sc[c + 1] + k.aym * Sc[c - SY] + k.ayp * Sc[c + SY] +
+           k.az * (Sm[c] + Sp[c]);
+}
+
+// Plane op choosing the collapsed form where the plane is away from z walls
+// (uniform per plane).
+template <int PER, int SY, typename T>
+__device__ inline T plane_apply(const op3<T>& A, bool interior, const T* Sm, const T* Sc, const T* Sp,
+                                const xy_coef<T>& k, const z_coef<T>& kz)
+{
+    if ((PER & 4) != 0 || interior) return plane_op_int<SY>(Sm, Sc, Sp, k);
+    return plane_op<PER, SY>(A, Sm, Sc, Sp, k, kz);
 }
 
 template <typename T, bool ADD, bool ACC, int PER, bool WIDE>
@@ -388,13 +418,14 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
             const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
             if (gk >= 0) {
                 const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+                const bool in = gk > 0 && gk < nz - 1;
                 const T *Vm = Rv[sv(z - 1)], *Vc = Rv[sv(z)], *Vp = Rv[sv(z + 1)];
                 T* dst = R1[ss(z)];
 #pragma unroll
                 for (int r = 0; r < N1; ++r) {
                     if (k1[r].c < 0) continue;
                     const int c = k1[r].c;
-                    dst[c] = -w0 * plane_op<PER, VX>(A, Vm, Vc, Vp, k1[r], kz) + w1 * Vc[c];
+                    dst[c] = -w0 * plane_apply<PER, VX>(A, in, Vm, Vc, Vp, k1[r], kz) + w1 * Vc[c];
                 }
             }
         }
@@ -403,6 +434,7 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
             const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
             if (gk >= 0) {
                 const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+                const bool in = gk > 0 && gk < nz - 1;
                 const T *Sm = R1[ss(z - 1)], *Sc = R1[ss(z)], *Sp = R1[ss(z + 1)];
                 const T* Vc = Rv[sv(z)];
                 T* dst = R2[ss(z)];
@@ -410,20 +442,21 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
                 for (int r = 0; r < N2; ++r) {
                     if (k2[r].c < 0) continue;
                     const int c = k2[r].c;
-                    dst[c] = T(-1) * plane_op<PER, VX>(A, Sm, Sc, Sp, k2[r], kz) + w2 * Vc[c];
+                    dst[c] = T(-1) * plane_apply<PER, VX>(A, in, Sm, Sc, Sp, k2[r], kz) + w2 * Vc[c];
                 }
             }
         }
         if (q >= MARCH_LEAD) {               // out = w3 v - A s2 (+ out)
             const int z = z0 - MARCH_LEAD + q;
             const z_coef<T> kz = make_z_coef<PZ>(A.a[2], z);
+            const bool in = z > 0 && z < nz - 1;
             const T *Sm = R2[ss(z - 1)], *Sc = R2[ss(z)], *Sp = R2[ss(z + 1)];
             const T* Vc = Rv[sv(z)];
 #pragma unroll
             for (int r = 0; r < NO; ++r) {
                 if (ko[r].c < 0) continue;
                 const size_t p = (size_t) ocol[r] + nxy * (size_t) z;
-                T val = T(-1) * plane_op<PER, VX>(A, Sm, Sc, Sp, ko[r], kz);
+                T val = T(-1) * plane_apply<PER, VX>(A, in, Sm, Sc, Sp, ko[r], kz);
                 if (ADD) val += T(1) * po[r];
                 val += w3 * Vc[ko[r].c];
                 if (ACC) acc[p] = pa[r] + (double) val;
@@ -691,12 +724,13 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
         if (f <= fe) {
             const int gk = axis_global<PZ>(A.a[2], f, 0, 0);
             const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
+            const bool in = gk > 0 && gk < nz - 1;
             const T *Xm = Xr[sx(f - 1)], *Xc = Xr[sx(f)], *Xp = Xr[sx(f + 1)];
             T* dst = Rr[q & 1];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 if (kr[r].c < 0) continue;
-                dst[kr[r].c] = T(-1) * plane_op<PER, XX>(A, Xm, Xc, Xp, kr[r], kz) + T(1) * pb[r];
+                dst[kr[r].c] = T(-1) * plane_apply<PER, XX>(A, in, Xm, Xc, Xp, kr[r], kz) + T(1) * pb[r];
             }
         }
         store_x(f + 2, px);                  // fetched two phases ago
