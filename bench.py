@@ -89,7 +89,8 @@ def make_solver(ibm, ctx, L, types, precision):
 
 
 KERNELS = ("spread_brick", "bin", "interp", "rbf_gemm", "geometry", "stencil_f64", "stencil_f32", "restrict",
-           "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64", "residual_restrict")
+           "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64", "residual_restrict",
+           "prolong_residual")
 
 
 def run_form(ibm, ctx, stream, L, types, precision, steps, warmup):
@@ -202,7 +203,8 @@ def cpu_baseline(inputs, orc, fd, rbf, L, types, gpu_iters):
 PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "cheb_march_kernel<double",
                "stencil_f32": "stencil_kernel<float", "stencil_f64": "stencil_kernel<double",
                "restrict": "restrict_kernel<", "residual_restrict": "residual_restrict_kernel",
-               "prolong": "prolong_add_kernel", "rbf_gemm": "rbf_gemm_kernel",
+               "prolong": "prolong_add_kernel", "prolong_residual": "prolong_residual_kernel",
+               "rbf_gemm": "rbf_gemm_kernel",
                "spread_brick": "spread_brick_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
 COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
 

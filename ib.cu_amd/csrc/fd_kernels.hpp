@@ -12,11 +12,12 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
                           const T* z, T* out);
 // Planes the one-pass smoother handles (x and y extents below 32768).
 bool cheb_fused_fits(int nx, int ny);
-// out (+)= p(A) v, the degree-4 Chebyshev smoothing of weights w in one
-// pass; with acc, acc += out + p(A) v instead (add must be true).
+// out = p(A) v, or out = src + p(A) v with add (src = addsrc, or out when
+// null), the degree-4 Chebyshev smoothing of weights w in one pass; with acc,
+// acc += src + p(A) v instead (add must be true).
 template <typename T>
 ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
-                             double* acc);
+                             double* acc, const T* addsrc = nullptr);
 // out = alpha A x + beta y + gamma z, also stored as floats in out32
 ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
                                 const double* y, double gamma, const double* z, double* out, float* out32);
@@ -32,6 +33,10 @@ ibm_status launch_residual_restrict(ibm_ctx* ctx, const op3<T>& A, const transfe
                                     T* coarse);
 template <typename T>
 ibm_status launch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, const T* coarse, T* fine);
+// xn = x + P e and r = b - A xn in one pass (A on the fine grid of P).
+template <typename T>
+ibm_status launch_prolong_residual(ibm_ctx* ctx, const op3<T>& A, const transfer3<T>& P, const T* e, const T* x,
+                                   const T* b, T* xn, T* r);
 // y = a x + b y
 template <typename T, typename U>
 ibm_status launch_axpby(ibm_ctx* ctx, size_t n, T a, const U* x, T b, T* y);

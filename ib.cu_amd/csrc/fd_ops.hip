@@ -313,7 +313,7 @@ __device__ inline T plane_apply(const op3<T>& A, bool interior, const T* Sm, con
 
 template <typename T, bool ADD, bool ACC, int PER, bool WIDE>
 __global__ void __launch_bounds__((march_tile<T, WIDE>::THREADS))
-cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* __restrict__ out,
+cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* out, const T* addsrc,
                   double* __restrict__ acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
 {
     constexpr bool PX = (PER & 1) != 0, PY = (PER & 2) != 0, PZ = (PER & 4) != 0;
@@ -406,7 +406,7 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
 #pragma unroll
         for (int r = 0; r < NO; ++r) {
             const size_t p = (size_t) ocol[r] + nxy * k;
-            if (ADD) po[r] = out[p];
+            if (ADD) po[r] = addsrc[p];
             if (ACC) pa[r] = acc[p];
         }
     };
@@ -789,6 +789,231 @@ __device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coar
         acc += wt[2][b] * az;
     }
     return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Prolongation and residual in one pass: xn = x + P e, r = b - A xn, with xn
+// written to its own vector (neighbouring blocks still read x in their
+// halos).  A block owns a TX x TY fine column tile and a chunk of fine planes
+// and marches through z: phase q forms xn on plane z0-1+q over the tile grown
+// by 1 (from x and a 3-plane LDS ring of e over the matching coarse columns;
+// prolong_at's taps, folds and summation order) into a 4-plane ring, and
+// computes the residual on plane z0-3+q from the planes of earlier phases.  Points outside walls
+// hold 0 in the rings (the collapsed operator weights them by 0).
+// ---------------------------------------------------------------------------
+template <typename T, bool WIDE>
+struct pr_tile {
+    static constexpr int X = WIDE ? 64 : 16, Y = 8;
+    static constexpr int THREADS = WIDE ? 256 : 128;
+};
+constexpr int PR_TARGET_BLOCKS = 1024;
+
+// Coarse taps of fine index j along an axis (prolong_at's ca, cb), as region
+// offsets from coarse coordinate o (unwrapped on periodic axes, folded on
+// walled ones) and weights.
+template <typename T>
+__device__ inline void pr_taps(const transfer_axis<T>& a, int j, int o, int off[2], T w[2])
+{
+    const int ca = (j + 1) >> 1, cb = ca - 1;
+    T wa = a.w[j - 2 * ca + 1], wb = a.w[j - 2 * cb + 1];
+    int ia = ca, ib = cb;
+    if (!a.periodic) {
+        ia = fold(a, ca, a.nc, wa);
+        ib = fold(a, cb, a.nc, wb);
+    }
+    off[0] = ia - o;
+    off[1] = ib - o;
+    w[0] = wa;
+    w[1] = wb;
+}
+
+template <typename T, int PER, bool WIDE>
+__global__ void __launch_bounds__((pr_tile<T, WIDE>::THREADS))
+prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const T* __restrict__ x,
+                        const T* __restrict__ b, T* __restrict__ xn, T* __restrict__ r, int tiles_x, int tiles_xy,
+                        int zchunk, int nblocks, int xcd_chunk)
+{
+    constexpr bool PX = (PER & 1) != 0, PY = (PER & 2) != 0, PZ = (PER & 4) != 0;
+    constexpr int TX = pr_tile<T, WIDE>::X, TY = pr_tile<T, WIDE>::Y, NT = pr_tile<T, WIDE>::THREADS;
+    constexpr int NXX = TX + 2, NXP = NXX * (TY + 2);            // xn plane: tile grown by 1
+    constexpr int CXX = TX / 2 + 3, CP = CXX * (TY / 2 + 3);      // e plane: matching coarse columns
+    constexpr int NN = (NXP + NT - 1) / NT, NR = (TX * TY + NT - 1) / NT, NC = (CP + NT - 1) / NT;
+    __shared__ T Xn[4][NXP];          // xn planes f - 3 .. f of phase q (f = z0 - 1 + q)
+    __shared__ T Ec[3][CP];
+
+    int blk = blockIdx.x;
+    if (xcd_chunk) blk = (blk & 7) * xcd_chunk + (blk >> 3);
+    if (blk >= nblocks) return;
+    const int tile = blk % tiles_xy, chunk = blk / tiles_xy;
+    const int ox = (tile % tiles_x) * TX, oy = (tile / tiles_x) * TY;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const size_t nxy = (size_t) nx * ny;
+    const int ncx = P.a[0].nc, ncy = P.a[1].nc, ncz = P.a[2].nc;
+    const size_t ncxy = (size_t) ncx * ncy;
+    const int z0 = chunk * zchunk, z1 = min(z0 + zchunk, nz);
+    const int tid = threadIdx.x;
+    const int cxo = (ox >> 1) - 1, cyo = (oy >> 1) - 1;    // coarse origin of the e region
+
+    // xn positions: fine (ox - 1 + i, oy - 1 + j); their x / y coarse taps
+    int ncol[NN], nc_[NN], ntx[NN][2], nty[NN][2];
+    T nwx[NN][2], nwy[NN][2];
+    bool nout[NN];     // inside the tile: written to xn
+#pragma unroll
+    for (int q = 0; q < NN; ++q) {
+        const int t = tid + q * NT;
+        const int i = t % NXX, j = t / NXX;
+        const int gi = axis_global<PX>(A.a[0], ox, 1, i), gj = axis_global<PY>(A.a[1], oy, 1, j);
+        const bool valid = t < NXP && gi >= 0 && gj >= 0;
+        ncol[q] = valid ? gi + nx * gj : -1;
+        nc_[q] = t < NXP ? t : -1;
+        nout[q] = valid && i >= 1 && i <= TX && j >= 1 && j <= TY;
+        // unwrapped fine index on periodic axes, the in-range index on walled ones
+        pr_taps(P.a[0], PX ? ox - 1 + i : (gi >= 0 ? gi : 0), cxo, ntx[q], nwx[q]);
+        pr_taps(P.a[1], PY ? oy - 1 + j : (gj >= 0 ? gj : 0), cyo, nty[q], nwy[q]);
+    }
+    // residual positions: the tile
+    xy_coef<T> kr[NR];
+    int rcol[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const int t = tid + q * NT;
+        const int i = t % TX, j = t / TX;
+        const int gi = ox + i, gj = oy + j;
+        const bool valid = t < TX * TY && gi < nx && gj < ny;
+        kr[q] = make_xy_coef<PER>(A, valid, (i + 1) + NXX * (j + 1), gi, gj);
+        rcol[q] = valid ? gi + nx * gj : 0;
+    }
+    // e loads: coarse region (cxo + u, cyo + v)
+    int ecol[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const int t = tid + q * NT;
+        int col = -1;
+        if (t < CP) {
+            int u = cxo + t % CXX, v = cyo + t / CXX;
+            if (PX) { u %= ncx; if (u < 0) u += ncx; } else if (u < 0 || u >= ncx) u = -1;
+            if (PY) { v %= ncy; if (v < 0) v += ncy; } else if (v < 0 || v >= ncy) v = -1;
+            if (u >= 0 && v >= 0) col = u + ncx * v;
+        }
+        ecol[q] = col;
+    }
+
+    // coarse planes: physical index on a walled z axis, unwrapped on a periodic one
+    auto need = [&](int f) { const int ca = (f + 1) >> 1; return PZ ? ca : min(ca, ncz - 1); };
+    auto cslot = [&](int c) { return (c + 3 * ncz + 3) % 3; };
+    auto fetch_e = [&](int c, T (&pe)[NC]) {
+        int k = c;
+        if (PZ) { k %= ncz; if (k < 0) k += ncz; } else k = min(max(k, 0), ncz - 1);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) pe[q] = e[(size_t) max(ecol[q], 0) + ncxy * (size_t) k];
+    };
+    auto store_e = [&](int c, const T (&pe)[NC]) {
+        T* dst = Ec[cslot(c)];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const int t = tid + q * NT;
+            if (t < CP) dst[t] = ecol[q] >= 0 ? pe[q] : T(0);
+        }
+    };
+    auto fetch_x = [&](int f, T (&px)[NN]) {
+        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+#pragma unroll
+        for (int q = 0; q < NN; ++q) px[q] = x[(size_t) max(ncol[q], 0) + nxy * k];
+    };
+    auto fetch_b = [&](int f, T (&pb)[NR]) {
+        const size_t k = (size_t) min(max(f, z0), z1 - 1);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) pb[q] = b[(size_t) rcol[q] + nxy * k];
+    };
+
+    // e planes for the first xn plane (z0 - 1) and the next one prefetched
+    int cmax = need(z0 - 1);
+    T pe[NC];
+    fetch_e(cmax - 1, pe);
+    store_e(cmax - 1, pe);
+    fetch_e(cmax, pe);
+    store_e(cmax, pe);
+    fetch_e(cmax + 1, pe);
+
+    auto sx = [&](int f) { return (f - z0 + 1) & 3; };    // xn ring slot, f >= z0 - 1
+    auto phase = [&](int q, T (&px)[NN], T (&pb)[NR]) {
+        {   // xn on plane f
+            const int f = z0 - 1 + q;
+            const int gk = axis_global<PZ>(A.a[2], f, 0, 0);
+            T* dst = Xn[sx(f)];
+            if (f <= z1) {
+                if (gk < 0) {
+#pragma unroll
+                    for (int i = 0; i < NN; ++i)
+                        if (nc_[i] >= 0) dst[nc_[i]] = T(0);
+                } else {
+                    // z taps (prolong_at's ca, cb on the fine plane index)
+                    int oz[2];
+                    T wz[2];
+                    pr_taps(P.a[2], PZ ? f : gk, 0, oz, wz);
+                    const T* Ea = Ec[cslot(oz[0])];
+                    const T* Eb = Ec[cslot(oz[1])];
+#pragma unroll
+                    for (int i = 0; i < NN; ++i) {
+                        if (nc_[i] < 0) continue;
+                        T val = T(0);
+                        if (ncol[i] >= 0) {
+                            T acc = 0;
+#pragma unroll
+                            for (int bz = 0; bz < 2; ++bz) {
+                                const T* E = bz == 0 ? Ea : Eb;
+                                T az = 0;
+#pragma unroll
+                                for (int a = 0; a < 2; ++a) {
+                                    const T* row = E + CXX * nty[i][a];
+                                    az += nwy[i][a] * (nwx[i][0] * row[ntx[i][0]] + nwx[i][1] * row[ntx[i][1]]);
+                                }
+                                acc += wz[bz] * az;
+                            }
+                            val = px[i] + acc;
+                            if (nout[i] && f >= z0 && f < z1) xn[(size_t) ncol[i] + nxy * (size_t) gk] = val;
+                        }
+                        dst[nc_[i]] = val;
+                    }
+                }
+            }
+        }
+        if (q >= 2) {   // residual on plane f - 2 (xn planes f - 3 .. f - 1 from earlier phases)
+            const int z = z0 - 3 + q;
+            if (z >= z0 && z < z1) {
+                const z_coef<T> kz = make_z_coef<PZ>(A.a[2], z);
+                const bool in = z > 0 && z < nz - 1;
+                const T *Sm = Xn[sx(z - 1)], *Sc = Xn[sx(z)], *Sp = Xn[sx(z + 1)];
+#pragma unroll
+                for (int i = 0; i < NR; ++i) {
+                    if (kr[i].c < 0) continue;
+                    r[(size_t) rcol[i] + nxy * (size_t) z] =
+                        T(-1) * plane_apply<PER, NXX>(A, in, Sm, Sc, Sp, kr[i], kz) + T(1) * pb[i];
+                }
+            }
+        }
+        // operands two phases on; the e plane the next phase needs
+        fetch_x(z0 + 1 + q, px);
+        fetch_b(z0 - 1 + q, pb);
+        if (need(z0 + q) > cmax) {
+            ++cmax;
+            store_e(cmax, pe);
+            fetch_e(cmax + 1, pe);
+        }
+        __syncthreads();
+    };
+
+    T pxA[NN], pxB[NN], pbA[NR], pbB[NR];
+    fetch_x(z0 - 1, pxA);
+    fetch_x(z0, pxB);
+    fetch_b(z0 - 3, pbA);
+    fetch_b(z0 - 2, pbB);
+    __syncthreads();
+    const int nq = (z1 - z0) + 3;
+    for (int q = 0; q < nq; q += 2) {
+        phase(q, pxA, pbA);
+        if (q + 1 < nq) phase(q + 1, pxB, pbB);
+    }
 }
 
 template <typename T>
@@ -1184,12 +1409,13 @@ ibm_status launch_stencil(ibm_ctx* ctx, const op3<T>& A, T alpha, const T* x, T 
 // One instantiation per periodicity pattern of the three axes.
 template <typename T, bool ADD, bool ACC, bool WIDE>
 static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A, T w0, T w1, T w2, T w3, const T* v,
-                        T* out, double* acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
+                        T* out, const T* addsrc, double* acc, int tiles_x, int tiles_xy, int zchunk, int nblocks,
+                        int xcd_chunk)
 {
     constexpr int threads = march_tile<T, WIDE>::THREADS;
 #define IBM_CHEB_CASE(P)                                                                                     \
     case P:                                                                                                  \
-        cheb_march_kernel<T, ADD, ACC, P, WIDE><<<blocks, threads, 0, stream>>>(A, w0, w1, w2, w3, v, out, acc, \
+        cheb_march_kernel<T, ADD, ACC, P, WIDE><<<blocks, threads, 0, stream>>>(A, w0, w1, w2, w3, v, out, addsrc, acc, \
                                                                                tiles_x, tiles_xy, zchunk,    \
                                                                                nblocks, xcd_chunk);          \
         break;
@@ -1207,8 +1433,8 @@ static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A
 }
 
 template <typename T, bool WIDE>
-static void cheb_configure(const op3<T>& A, const double w[4], const T* v, T* out, bool add, double* acc,
-                           hipStream_t stream)
+static void cheb_configure(const op3<T>& A, const double w[4], const T* v, T* out, const T* addsrc, bool add,
+                           double* acc, hipStream_t stream)
 {
     constexpr int TX = march_tile<T, WIDE>::X, TY = march_tile<T, WIDE>::Y;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
@@ -1223,13 +1449,13 @@ static void cheb_configure(const op3<T>& A, const double w[4], const T* v, T* ou
     const T w0 = (T) w[0], w1 = (T) w[1], w2 = (T) w[2], w3 = (T) w[3];
     const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
     if (acc)
-        cheb_launch<T, true, true, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+        cheb_launch<T, true, true, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, addsrc, acc, tiles_x, tiles_xy, zchunk,
                                          nblocks, xcd_chunk);
     else if (add)
-        cheb_launch<T, true, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy, zchunk,
+        cheb_launch<T, true, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, addsrc, acc, tiles_x, tiles_xy, zchunk,
                                           nblocks, xcd_chunk);
     else
-        cheb_launch<T, false, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, acc, tiles_x, tiles_xy,
+        cheb_launch<T, false, false, WIDE>(per, blocks, stream, A, w0, w1, w2, w3, v, out, addsrc, acc, tiles_x, tiles_xy,
                                            zchunk, nblocks, xcd_chunk);
 }
 
@@ -1240,8 +1466,9 @@ bool cheb_fused_fits(int nx, int ny)
 
 template <typename T>
 ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
-                             double* acc)
+                             double* acc, const T* addsrc)
 {
+    if (!addsrc) addsrc = out;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
     if (!cheb_fused_fits(nx, ny)) return fail(ctx, IBM_ERR_ARG, "fused smoothing: %d x %d planes too wide", nx, ny);
@@ -1249,8 +1476,8 @@ ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], c
     // v read, out written (read too when adding; FP64 read-modify-write with acc)
     const double bytes = n * (sizeof(T) + (acc ? 16.0 + sizeof(T) : (add ? 2.0 : 1.0) * sizeof(T)));
     prof_scope prof(ctx, sizeof(T) == 8 ? "chebyshev_f64" : "chebyshev_f32", bytes);
-    if ((size_t) nx * ny * nz >= MARCH_WIDE_POINTS) cheb_configure<T, true>(A, w, v, out, add, acc, ctx->stream);
-    else cheb_configure<T, false>(A, w, v, out, add, acc, ctx->stream);
+    if ((size_t) nx * ny * nz >= MARCH_WIDE_POINTS) cheb_configure<T, true>(A, w, v, out, addsrc, add, acc, ctx->stream);
+    else cheb_configure<T, false>(A, w, v, out, addsrc, add, acc, ctx->stream);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -1330,6 +1557,57 @@ ibm_status launch_residual_restrict(ibm_ctx* ctx, const op3<T>& A, const transfe
     prof_scope prof(ctx, "residual_restrict", (double) sizeof(T) * (2 * nf + nc));
     if (nf >= MARCH_WIDE_POINTS) rr_configure<T, true>(A, R, x, b, coarse, ctx->stream);
     else rr_configure<T, false>(A, R, x, b, coarse, ctx->stream);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T, bool WIDE>
+static void pr_configure(const op3<T>& A, const transfer3<T>& P, const T* e, const T* x, const T* b, T* xn, T* r,
+                         hipStream_t stream)
+{
+    constexpr int TX = pr_tile<T, WIDE>::X, TY = pr_tile<T, WIDE>::Y;
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
+    int zchunk = (int) (((long long) nz * tiles_xy + PR_TARGET_BLOCKS - 1) / PR_TARGET_BLOCKS);
+    zchunk = std::min(nz, std::max(zchunk, 2));
+    const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
+    const int blocks = (nblocks + 7) / 8 * 8;
+    const int per = (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0);
+    constexpr int threads = pr_tile<T, WIDE>::THREADS;
+#define IBM_PR_CASE(PP)                                                                                      \
+    case PP:                                                                                                 \
+        prolong_residual_kernel<T, PP, WIDE><<<blocks, threads, 0, stream>>>(A, P, e, x, b, xn, r, tiles_x,  \
+                                                                            tiles_xy, zchunk, nblocks,       \
+                                                                            blocks / 8);                     \
+        break;
+    switch (per) {
+        IBM_PR_CASE(0)
+        IBM_PR_CASE(1)
+        IBM_PR_CASE(2)
+        IBM_PR_CASE(3)
+        IBM_PR_CASE(4)
+        IBM_PR_CASE(5)
+        IBM_PR_CASE(6)
+        IBM_PR_CASE(7)
+    }
+#undef IBM_PR_CASE
+}
+
+template <typename T>
+ibm_status launch_prolong_residual(ibm_ctx* ctx, const op3<T>& A, const transfer3<T>& P, const T* e, const T* x,
+                                   const T* b, T* xn, T* r)
+{
+    const size_t nf = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    const size_t nc = (size_t) P.a[0].nc * P.a[1].nc * P.a[2].nc;
+    if (nf > MAX_GRID_POINTS || (size_t) P.a[0].nf * P.a[1].nf > 0x7fffffff)
+        return fail(ctx, IBM_ERR_SIZE, "grid too large");
+    for (int d = 0; d < 3; ++d)
+        if (P.a[d].nf != A.a[d].n || (P.a[d].periodic != 0) != (A.a[d].periodic != 0))
+            return fail(ctx, IBM_ERR_ARG, "prolongation/residual: operator and transfer grids differ");
+    // e, x, b read, xn and r written
+    prof_scope prof(ctx, "prolong_residual", (double) sizeof(T) * (nc + 4 * nf));
+    if (nf >= MARCH_WIDE_POINTS) pr_configure<T, true>(A, P, e, x, b, xn, r, ctx->stream);
+    else pr_configure<T, false>(A, P, e, x, b, xn, r, ctx->stream);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -1482,10 +1760,11 @@ ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes)
 #define IBM_INSTANTIATE(T)                                                                              \
     template ibm_status launch_stencil<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, T*); \
     template ibm_status launch_stencil_acc<T>(ibm_ctx*, const op3<T>&, T, const T*, T, const T*, T, const T*, double*); \
-    template ibm_status launch_cheb_fused<T>(ibm_ctx*, const op3<T>&, const double*, const T*, T*, bool, double*); \
+    template ibm_status launch_cheb_fused<T>(ibm_ctx*, const op3<T>&, const double*, const T*, T*, bool, double*, const T*); \
     template ibm_status launch_restrict<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);               \
     template ibm_status launch_residual_restrict<T>(ibm_ctx*, const op3<T>&, const transfer3<T>&, const T*, const T*, T*); \
     template ibm_status launch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, const T*, T*);            \
+    template ibm_status launch_prolong_residual<T>(ibm_ctx*, const op3<T>&, const transfer3<T>&, const T*, const T*, const T*, T*, T*); \
     template ibm_status launch_fill<T>(ibm_ctx*, size_t, T, T*);                                       \
     template ibm_status launch_sum<T>(ibm_ctx*, size_t, const T*, double*, double*);                   \
     template ibm_status launch_remove_mean<T>(ibm_ctx*, size_t, T*, double*, double*);                 \

@@ -139,6 +139,9 @@ constexpr bool FUSED_SMOOTHING = true;
 // Residual and restriction of the down-leg in one kernel
 // (launch_residual_restrict) or as a stencil pass and a restriction.
 constexpr bool FUSED_RESTRICT = true;
+// Prolongation and residual of the up-leg in one kernel
+// (launch_prolong_residual) or as a prolongation and a stencil pass.
+constexpr bool FUSED_PROLONG = true;
 
 // out = p(A) v, the degree-4 Chebyshev polynomial by Horner's rule
 // (algo/chebyshev.h:149-163): s0 = w0 v, s_k = w_k v - A s_{k-1}.
@@ -206,6 +209,11 @@ static ibm_status vcycle(ibm_ctx* ctx, ibm_mg* m, size_t l, double* acc = nullpt
         IBM_TRY(launch_restrict<T>(ctx, level_tr<T>(L), B.r, C.b));
     }
     IBM_TRY(vcycle<T>(ctx, m, l + 1));
+    if (FUSED_PROLONG && FUSED_SMOOTHING && cheb_fused_fits(A.a[0].n, A.a[1].n)) {
+        // xn = x + P e (into s1) and r = b - A xn in one pass; x = xn + S r
+        IBM_TRY(launch_prolong_residual<T>(ctx, A, level_tr<T>(L), C.x, B.x, B.b, B.s1, B.r));
+        return launch_cheb_fused<T>(ctx, A, L.w, B.r, B.x, true, acc, B.s1);
+    }
     IBM_TRY(launch_prolong_add<T>(ctx, level_tr<T>(L), C.x, B.x));
     IBM_TRY(launch_stencil<T>(ctx, A, (T) -1, B.x, 0, nullptr, (T) 1, B.b, B.r));
     return cheb_apply<T>(ctx, A, L.w, B.r, B.s1, B.s2, B.x, true, acc);
