@@ -62,6 +62,28 @@ __global__ void accel_kernel(size_t n, double inv_rho, const double* __restrict_
         h[i] = F[i] * inv_rho - h[i];
 }
 
+// Initial guess of a warm-started solve from its last two solutions at the
+// same step size: x <- 2 x - prev (linear extrapolation in time) when the
+// device step counter *hist says both exist, x unchanged otherwise; prev <-
+// the incoming x.  The counter lives on the device so that a replayed graph
+// and an eager step decide alike.
+__global__ void extrapolate_kernel(size_t n, const int* hist, double* x, double* prev)
+{
+    const int two = *hist >= 2;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x) {
+        const double c = x[i];
+        const double g = two ? 2.0 * c - prev[i] : c;
+        prev[i] = c;
+        x[i] = g;
+    }
+}
+
+// Step counter of the warm starts: set (v >= 0) or advanced towards 2 (v < 0).
+__global__ void history_kernel(int* hist, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hist = v >= 0 ? v : min(*hist + 1, 2);
+}
+
 // x *= a in place
 __global__ void scale_kernel(size_t n, double a, double* x)
 {
@@ -94,6 +116,10 @@ struct ibm_solver {
     };
     std::vector<type_t> types;
     double *F[3]{}, *H[3]{}, *rhs[3]{}, *du[3]{}, *div = nullptr;
+    double *du_prev[3]{}, *p_prev = nullptr;   // the solutions one step further back
+    int* warm_hist = nullptr;          // device: solutions known at step size history_dt (0, 1, 2)
+    double history_dt = 0;
+    bool extrapolating = false;        // the adaptive counts are those of extrapolated starts
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
     int max_iters[4]{};                // largest adaptive iteration counts seen
@@ -108,6 +134,17 @@ struct ibm_solver {
 };
 
 namespace {
+
+// Restart the warm-start history when the step size differs from the one
+// the stored solutions belong to (host side, outside any captured graph).
+ibm_status sync_history(ibm_ctx* ctx, ibm_solver* s, double dt)
+{
+    if (dt == s->history_dt) return IBM_OK;
+    history_kernel<<<1, 64, 0, ctx->stream>>>(s->warm_hist, 0);
+    IBM_LAUNCH_CHECK(ctx);
+    s->history_dt = dt;
+    return IBM_OK;
+}
 
 ibm_status alloc(ibm_ctx* ctx, ibm_solver* s, size_t n, double** p)
 {
@@ -218,6 +255,20 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     // 2b. step size (ins::tracker) from the force bound
     if (P.fscale > 0 && !fixed) IBM_TRY(track(ctx, s));
     const double dt = s->dt;
+    // Warm starts: extrapolated from the last two solutions once both exist
+    // at this step size (the counter is reset by sync_history before a step
+    // at a new size).  The fixed V-cycle counts are taken from adaptive
+    // steps that started the same way.
+    if (!fixed) {
+        IBM_TRY(sync_history(ctx, s, dt));
+        int h = 0;
+        IBM_HIP(ctx, hipMemcpyAsync(&h, s->warm_hist, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (h >= 2 && !s->extrapolating) {
+            s->extrapolating = true;
+            for (int i = 0; i < 4; ++i) s->max_iters[i] = 0;
+        }
+    }
     // 3. spread all cell types onto the MAC grid
     {
         int counts[MAX_TYPES];
@@ -247,6 +298,8 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
                                        nullptr, s->rhs[i]));
         // rhs += dt nu L u
         IBM_TRY(launch_stencil<double>(ctx, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr, s->rhs[i]));
+        extrapolate_kernel<<<blocks(n), 256, 0, ctx->stream>>>(n, s->warm_hist, s->du[i], s->du_prev[i]);
+        IBM_LAUNCH_CHECK(ctx);
         if (fixed) {
             IBM_TRY(ibm_mg_solve_fixed(ctx, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, 1, s->info + i));
         } else {
@@ -265,6 +318,8 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         const double* u[3] = {st->u[0], st->u[1], st->u[2]};
         IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
         scale_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, 1.0 / dt, s->div);
+        IBM_LAUNCH_CHECK(ctx);
+        extrapolate_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, s->warm_hist, st->p, s->p_prev);
         IBM_LAUNCH_CHECK(ctx);
         if (fixed) {
             IBM_TRY(ibm_mg_solve_fixed(ctx, s->pois, s->div, st->p, s->max_iters[3] + 1, 1, s->info + 3));
@@ -312,6 +367,8 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             if (counts[t])
                 IBM_TRY(launch_axpby<double, double>(ctx, (size_t) 3 * counts[t], dt, s->types[t].U, 1.0, st->x[t]));
     }
+    history_kernel<<<1, 64, 0, ctx->stream>>>(s->warm_hist, -1);
+    IBM_LAUNCH_CHECK(ctx);
     if (!fixed) {
         for (int i = 0; i < 4; ++i) s->max_iters[i] = std::max(s->max_iters[i], s->last_iters[i]);
         ++s->adaptive_steps;
@@ -370,10 +427,25 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         if ((st = alloc(ctx, s, s->nu[i], &s->F[i]))) break;
         if ((st = alloc(ctx, s, s->nu[i], &s->H[i]))) break;
         if ((st = alloc(ctx, s, s->nu[i], &s->rhs[i]))) break;
-        st = alloc(ctx, s, s->nu[i], &s->du[i]);
+        if ((st = alloc(ctx, s, s->nu[i], &s->du[i]))) break;
+        st = alloc(ctx, s, s->nu[i], &s->du_prev[i]);
     }
     // The velocity increments are the warm start of the first Helmholtz solves.
-    for (int i = 0; i < 3 && st == IBM_OK; ++i) st = launch_fill<double>(ctx, s->nu[i], 0.0, s->du[i]);
+    for (int i = 0; i < 3 && st == IBM_OK; ++i) {
+        st = launch_fill<double>(ctx, s->nu[i], 0.0, s->du[i]);
+        if (st == IBM_OK) st = launch_fill<double>(ctx, s->nu[i], 0.0, s->du_prev[i]);
+    }
+    if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->p_prev);
+    if (st == IBM_OK) st = launch_fill<double>(ctx, s->np, 0.0, s->p_prev);
+    if (st == IBM_OK) {
+        double* h = nullptr;
+        st = alloc(ctx, s, 1, &h);
+        s->warm_hist = (int*) h;
+        if (st == IBM_OK) {
+            history_kernel<<<1, 64, 0, ctx->stream>>>(s->warm_hist, 0);
+            st = hipGetLastError() == hipSuccess ? IBM_OK : fail(ctx, IBM_ERR_HIP, "history reset");
+        }
+    }
     if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->div);
     if (st == IBM_OK) st = alloc(ctx, s, 8, &s->info);
     if (st == IBM_OK) st = launch_fill<double>(ctx, 8, 0.0, s->info);
@@ -432,6 +504,7 @@ ibm_status ibm_solver_step_fixed(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, con
     if (!ctx || !s) return IBM_ERR_ARG;
     IBM_TRY(check_state(ctx, s, st));
     if (!s->adaptive_steps) return fail(ctx, IBM_ERR_ARG, "run an adaptive step first to calibrate the solves");
+    IBM_TRY(sync_history(ctx, s, s->dt));
     IBM_TRY(step_impl(ctx, s, st, boundary, true));
     st->t += s->dt;
     return IBM_OK;
@@ -454,6 +527,7 @@ ibm_status ibm_solver_capture(ibm_ctx* ctx, ibm_solver* s, const ibm_state* st, 
     const bool profiling = ctx->profiling;
     ctx->profiling = false;
     ibm_state copy = *st;
+    IBM_TRY(sync_history(ctx, s, s->dt));
     IBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ibm_status rc = step_impl(ctx, s, &copy, boundary, true);
     hipGraph_t g = nullptr;
@@ -478,6 +552,7 @@ ibm_status ibm_solver_replay(ibm_ctx* ctx, ibm_solver* s, ibm_state* st)
     if (!st || std::memcmp(st->u, s->captured.u, sizeof(st->u)) || st->p != s->captured.p ||
         std::memcmp(st->x, s->captured.x, sizeof(st->x)))
         return fail(ctx, IBM_ERR_ARG, "the state's buffers differ from the captured ones");
+    IBM_TRY(sync_history(ctx, s, s->captured_dt));
     IBM_HIP(ctx, hipGraphLaunch(s->exec, ctx->stream));
     st->t += s->captured_dt;
     return IBM_OK;
