@@ -40,7 +40,7 @@ STIFFNESS = 0.02
 TETHER_SCALE = 1.02
 TOL = 1e-10
 FRAC = 0.3               # share of the cells the CPU baseline times
-ADAPTIVE_STEPS = 4       # adaptive steps before the capture
+ADAPTIVE_STEPS = 8       # adaptive steps before the capture
 
 
 def _load(name, path):
@@ -101,9 +101,10 @@ def run_form(ibm, ctx, stream, L, types, precision, steps, warmup):
     s, st, ub, keep = make_solver(ibm, ctx, L, types, precision)
     torch.cuda.synchronize()
     with torch.cuda.stream(stream):
-        # Adaptive steps set the V-cycle counts of the captured step: the
-        # first two warm-start from the previous solution, the later ones
-        # from the extrapolation of the last two (the form every replay uses).
+        # Adaptive steps set the V-cycle counts of the captured step (the
+        # largest of the last four, + 1): the first two warm-start from the
+        # previous solution, the later ones from the extrapolation of the
+        # last two (the form every replay uses).
         for _ in range(ADAPTIVE_STEPS):
             s.step(st, boundary=ub)
         s.capture(st, boundary=ub)

@@ -98,6 +98,9 @@ int blocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 65536); }
 
 using namespace ibm;
 
+// Adaptive steps whose V-cycle counts set the fixed-count steps'.
+constexpr int ITER_WINDOW = 4;
+
 struct ibm_solver {
     ibm_solver_params par{};
     ibm_grid g[3]{}, gp{};
@@ -119,11 +122,12 @@ struct ibm_solver {
     double *du_prev[3]{}, *p_prev = nullptr;   // the solutions one step further back
     int* warm_hist = nullptr;          // device: solutions known at step size history_dt (0, 1, 2)
     double history_dt = 0;
-    bool extrapolating = false;        // the adaptive counts are those of extrapolated starts
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
-    int max_iters[4]{};                // largest adaptive iteration counts seen
+    int max_iters[4]{};                // largest adaptive iteration counts of the last ITER_WINDOW steps
     int last_iters[4]{};
+    int recent_iters[ITER_WINDOW][4]{};   // iteration counts of the last ITER_WINDOW adaptive steps
+    int recent_next = 0, recent_len = 0;
     int adaptive_steps = 0;
     int last_exponent = 0;
     hipGraph_t graph = nullptr;
@@ -256,19 +260,9 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     if (P.fscale > 0 && !fixed) IBM_TRY(track(ctx, s));
     const double dt = s->dt;
     // Warm starts: extrapolated from the last two solutions once both exist
-    // at this step size (the counter is reset by sync_history before a step
-    // at a new size).  The fixed V-cycle counts are taken from adaptive
-    // steps that started the same way.
-    if (!fixed) {
-        IBM_TRY(sync_history(ctx, s, dt));
-        int h = 0;
-        IBM_HIP(ctx, hipMemcpyAsync(&h, s->warm_hist, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (h >= 2 && !s->extrapolating) {
-            s->extrapolating = true;
-            for (int i = 0; i < 4; ++i) s->max_iters[i] = 0;
-        }
-    }
+    // at this step size (sync_history restarts the count before a step at a
+    // new size; fixed-count steps and graphs do that in their entry points).
+    if (!fixed) IBM_TRY(sync_history(ctx, s, dt));
     // 3. spread all cell types onto the MAC grid
     {
         int counts[MAX_TYPES];
@@ -370,7 +364,14 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     history_kernel<<<1, 64, 0, ctx->stream>>>(s->warm_hist, -1);
     IBM_LAUNCH_CHECK(ctx);
     if (!fixed) {
-        for (int i = 0; i < 4; ++i) s->max_iters[i] = std::max(s->max_iters[i], s->last_iters[i]);
+        // fixed-count steps use the largest counts of the recent adaptive steps
+        for (int i = 0; i < 4; ++i) s->recent_iters[s->recent_next][i] = s->last_iters[i];
+        s->recent_next = (s->recent_next + 1) % ITER_WINDOW;
+        s->recent_len = std::min(s->recent_len + 1, ITER_WINDOW);
+        for (int i = 0; i < 4; ++i) {
+            s->max_iters[i] = 0;
+            for (int k = 0; k < s->recent_len; ++k) s->max_iters[i] = std::max(s->max_iters[i], s->recent_iters[k][i]);
+        }
         ++s->adaptive_steps;
     } else {
         for (int i = 0; i < 4; ++i) s->last_iters[i] = s->max_iters[i] + 1;
