@@ -121,6 +121,12 @@ struct ibm_solver {
     double *F[3]{}, *H[3]{}, *rhs[3]{}, *du[3]{}, *div = nullptr;
     double *du_prev[3]{}, *p_prev = nullptr;   // the solutions one step further back
     int* warm_hist = nullptr;          // device: solutions known at step size history_dt (0, 1, 2)
+    // Fixed-count steps run the three Helmholtz solves concurrently: the
+    // second and third on contexts with streams of their own, forked from and
+    // joined back into the caller's stream with events.
+    ibm_ctx* aux[2]{};
+    hipStream_t aux_stream[2]{};
+    hipEvent_t fork_ev = nullptr, join_ev[2]{};
     double history_dt = 0;
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
@@ -138,6 +144,14 @@ struct ibm_solver {
 };
 
 namespace {
+
+// A status from work on an auxiliary context, with its message copied to
+// the caller's context.
+ibm_status step_status(ibm_ctx* ctx, ibm_ctx* c, ibm_status st)
+{
+    if (st != IBM_OK && c != ctx) ctx->error = c->error;
+    return st;
+}
 
 // Restart the warm-start history when the step size differs from the one
 // the stored solutions belong to (host side, outside any captured graph).
@@ -163,6 +177,12 @@ void release(ibm_solver* s)
 {
     if (s->exec) hipGraphExecDestroy(s->exec);
     if (s->graph) hipGraphDestroy(s->graph);
+    for (int k = 0; k < 2; ++k) {
+        if (s->aux[k]) ibm_destroy(s->aux[k]);
+        if (s->aux_stream[k]) (void) hipStreamDestroy(s->aux_stream[k]);
+        if (s->join_ev[k]) (void) hipEventDestroy(s->join_ev[k]);
+    }
+    if (s->fork_ev) (void) hipEventDestroy(s->fork_ev);
     if (s->par.fscale > 0) {
         for (int e = 0; e < 5; ++e)
             for (int i = 0; i < 3; ++i)
@@ -280,22 +300,33 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         const double* u[3] = {st->u[0], st->u[1], st->u[2]};
         IBM_TRY(ibm_advection(ctx, s->g, u, s->H));
     }
-    // 5. diffusion, component by component
+    // 5. diffusion, component by component; fixed-count steps run the three
+    // components concurrently (no host synchronisation inside; profiled
+    // steps stay on one stream so that every launch is timed)
+    const bool concurrent = fixed && !ctx->profiling && s->aux[0] && s->aux[1];
+    ibm_ctx* cc[3] = {ctx, concurrent ? s->aux[0] : ctx, concurrent ? s->aux[1] : ctx};
+    if (concurrent) {
+        IBM_HIP(ctx, hipEventRecord(s->fork_ev, ctx->stream));
+        for (int k = 0; k < 2; ++k) IBM_HIP(ctx, hipStreamWaitEvent(s->aux[k]->stream, s->fork_ev, 0));
+    }
     for (int i = 0; i < 3; ++i) {
+        ibm_ctx* c = cc[i];
         const size_t n = s->nu[i];
-        accel_kernel<<<blocks(n), 256, 0, ctx->stream>>>(n, 1.0 / P.density, s->F[i], s->H[i]);
-        IBM_LAUNCH_CHECK(ctx);
+        accel_kernel<<<blocks(n), 256, 0, c->stream>>>(n, 1.0 / P.density, s->F[i], s->H[i]);
+        IBM_LAUNCH_CHECK(c);
         op3<double> I = make_op<double>(s->g[i], 1.0, 0.0);
         op3<double> L = make_op<double>(s->g[i], 0.0, 1.0);
         // rhs = dt I~ (F/rho - H) + dt nu b
-        IBM_TRY(launch_stencil<double>(ctx, I, dt, s->H[i], ub && ub[i] ? dt * nu : 0.0, ub ? ub[i] : nullptr, 0.0,
-                                       nullptr, s->rhs[i]));
+        IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, I, dt, s->H[i], ub && ub[i] ? dt * nu : 0.0,
+                                                            ub ? ub[i] : nullptr, 0.0, nullptr, s->rhs[i])));
         // rhs += dt nu L u
-        IBM_TRY(launch_stencil<double>(ctx, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr, s->rhs[i]));
-        extrapolate_kernel<<<blocks(n), 256, 0, ctx->stream>>>(n, s->warm_hist, s->du[i], s->du_prev[i]);
-        IBM_LAUNCH_CHECK(ctx);
+        IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr,
+                                                            s->rhs[i])));
+        extrapolate_kernel<<<blocks(n), 256, 0, c->stream>>>(n, s->warm_hist, s->du[i], s->du_prev[i]);
+        IBM_LAUNCH_CHECK(c);
         if (fixed) {
-            IBM_TRY(ibm_mg_solve_fixed(ctx, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, 1, s->info + i));
+            IBM_TRY(step_status(ctx, c, ibm_mg_solve_fixed(c, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, 1,
+                                                            s->info + i)));
         } else {
             int it = 0;
             double rr = 0;
@@ -305,7 +336,13 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             IBM_HIP(ctx, hipMemcpyAsync(s->info + i, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
-        IBM_TRY(launch_axpby<double, double>(ctx, n, 1.0, s->du[i], 1.0, st->u[i]));
+        IBM_TRY(step_status(ctx, c, launch_axpby<double, double>(c, n, 1.0, s->du[i], 1.0, st->u[i])));
+    }
+    if (concurrent) {
+        for (int k = 0; k < 2; ++k) {
+            IBM_HIP(ctx, hipEventRecord(s->join_ev[k], s->aux[k]->stream));
+            IBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->join_ev[k], 0));
+        }
     }
     // 6. projection
     {
@@ -438,6 +475,20 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     }
     if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->p_prev);
     if (st == IBM_OK) st = launch_fill<double>(ctx, s->np, 0.0, s->p_prev);
+    for (int k = 0; k < 2 && st == IBM_OK; ++k) {
+        if (ibm_create(&s->aux[k], ctx->device) != IBM_OK) {
+            st = fail(ctx, IBM_ERR_HIP, "solver: auxiliary context");
+            break;
+        }
+        if (hipStreamCreateWithFlags(&s->aux_stream[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->join_ev[k], hipEventDisableTiming) != hipSuccess) {
+            st = fail(ctx, IBM_ERR_HIP, "solver: auxiliary stream");
+            break;
+        }
+        s->aux[k]->stream = s->aux_stream[k];
+    }
+    if (st == IBM_OK && hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess)
+        st = fail(ctx, IBM_ERR_HIP, "solver: fork event");
     if (st == IBM_OK) {
         double* h = nullptr;
         st = alloc(ctx, s, 1, &h);
