@@ -63,6 +63,15 @@ struct prof_scope {
     ~prof_scope();
 };
 
+// ibm_rbf_eval restricted to operators [j0, j1) of every batch entry and
+// launched on `stream`.  With do_pack the data-site coordinates are packed
+// into ctx's SLOT_RBF_X scratch first and *packed receives its address;
+// without, *packed must hold an earlier packing of the same batch that no
+// later evaluation has overwritten.
+ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
+                          const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
+                          unsigned char** packed);
+
 } // namespace ibm
 
 #define IBM_HIP(ctx, ...)                                                   \

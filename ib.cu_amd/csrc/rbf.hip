@@ -326,6 +326,78 @@ struct ibm_rbf {
     const unsigned char* plane(int op, int p) const { return mem + (size_t) (op * planes(mode) + p) * plane_bytes; }
 };
 
+namespace ibm {
+
+ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
+                          const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
+                          unsigned char** packed)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (count <= 0 || !ops || !N || !X || !Y) return fail(ctx, IBM_ERR_ARG, "empty or null batch");
+    int mode = ops[0] ? ops[0]->mode : -1;
+    int groups = 0;
+    size_t xbytes = 0;
+    for (int b = 0; b < count; ++b) {
+        if (!ops[b] || !X[b] || !Y[b] || N[b] < 0) return fail(ctx, IBM_ERR_ARG, "bad batch entry %d", b);
+        if (ops[b]->mode != mode) return fail(ctx, IBM_ERR_ARG, "batch mixes precisions");
+        if (ops[b]->device != ctx->device) return fail(ctx, IBM_ERR_ARG, "operator %d lives on another device", b);
+        groups += std::max(0, std::min(j1, ops[b]->nops) - j0);
+        xbytes += (size_t) round_up(std::max(N[b], 1), TILE) * ops[b]->Kp * elem_bytes(mode) * planes(mode);
+    }
+    if (groups > MAX_GROUPS) return fail(ctx, IBM_ERR_SIZE, "at most %d operator blocks per launch", MAX_GROUPS);
+    if (do_pack) {
+        void* scratch_mem;
+        IBM_TRY(scratch(ctx, SLOT_RBF_X, xbytes, &scratch_mem));
+        *packed = (unsigned char*) scratch_mem;
+    } else if (!*packed) {
+        return fail(ctx, IBM_ERR_ARG, "no packed operand");
+    }
+    unsigned char* xp = *packed;
+
+    gemm_groups G{};
+    int tiles = 0;
+    for (int b = 0; b < count; ++b) {
+        const ibm_rbf& op = *ops[b];
+        int cols_pad = round_up(std::max(N[b], 1), TILE);
+        size_t plane = (size_t) cols_pad * op.Kp * elem_bytes(mode);
+        unsigned char* hi = xp;
+        unsigned char* lo = planes(mode) > 1 ? xp + plane : nullptr;
+        xp += plane * planes(mode);
+        if (N[b] == 0) continue;
+        if (do_pack) pack(mode, stream, X[b], op.K, N[b], op.K, cols_pad, op.Kp, hi, lo);
+        for (int j = j0; j < std::min(j1, op.nops); ++j) {
+            gemm_group& g = G.g[G.count++];
+            g.E[0] = op.plane(j, 0);
+            g.E[1] = planes(mode) > 1 ? op.plane(j, 1) : g.E[0];
+            g.X[0] = hi;
+            g.X[1] = lo ? lo : hi;
+            g.Y = Y[b] + (size_t) j * op.M * N[b];
+            g.M = op.M;
+            g.N = N[b];
+            g.Kp = op.Kp;
+            g.tiles_n = cols_pad / TILE;
+            g.first_tile = tiles;
+            tiles += (op.rows_pad / TILE) * g.tiles_n;
+        }
+    }
+    IBM_LAUNCH_CHECK(ctx);
+    if (!tiles) return IBM_OK;
+    G.total_tiles = tiles;
+    int blocks = round_up(tiles, 8);
+    G.xcd_chunk = blocks / 8;
+    double flops = 0;
+    for (int b = 0; b < count; ++b)
+        flops += 2.0 * ops[b]->M * (double) N[b] * ops[b]->K * std::max(0, std::min(j1, ops[b]->nops) - j0);
+    prof_scope prof(ctx, "rbf_gemm", flops);
+    if (mode == MODE_F64) rbf_gemm_kernel<MODE_F64><<<blocks, NTHREADS, 0, stream>>>(G);
+    else if (mode == MODE_F32) rbf_gemm_kernel<MODE_F32><<<blocks, NTHREADS, 0, stream>>>(G);
+    else rbf_gemm_kernel<MODE_BF16X3><<<blocks, NTHREADS, 0, stream>>>(G);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+} // namespace ibm
+
 extern "C" {
 
 ibm_status ibm_rbf_create(ibm_ctx* ctx, int precision, int M, int K, int nops, const double* const* E, int lde,
@@ -377,61 +449,8 @@ ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int*
                         double* const* Y)
 {
     if (!ctx) return IBM_ERR_ARG;
-    if (count <= 0 || !ops || !N || !X || !Y) return fail(ctx, IBM_ERR_ARG, "empty or null batch");
-    int mode = ops[0] ? ops[0]->mode : -1;
-    int groups = 0;
-    size_t xbytes = 0;
-    for (int b = 0; b < count; ++b) {
-        if (!ops[b] || !X[b] || !Y[b] || N[b] < 0) return fail(ctx, IBM_ERR_ARG, "bad batch entry %d", b);
-        if (ops[b]->mode != mode) return fail(ctx, IBM_ERR_ARG, "batch mixes precisions");
-        if (ops[b]->device != ctx->device) return fail(ctx, IBM_ERR_ARG, "operator %d lives on another device", b);
-        groups += ops[b]->nops;
-        xbytes += (size_t) round_up(std::max(N[b], 1), TILE) * ops[b]->Kp * elem_bytes(mode) * planes(mode);
-    }
-    if (groups > MAX_GROUPS) return fail(ctx, IBM_ERR_SIZE, "at most %d operator blocks per launch", MAX_GROUPS);
-    void* scratch_mem;
-    IBM_TRY(scratch(ctx, SLOT_RBF_X, xbytes, &scratch_mem));
-    unsigned char* xp = (unsigned char*) scratch_mem;
-
-    gemm_groups G{};
-    int tiles = 0;
-    for (int b = 0; b < count; ++b) {
-        const ibm_rbf& op = *ops[b];
-        int cols_pad = round_up(std::max(N[b], 1), TILE);
-        size_t plane = (size_t) cols_pad * op.Kp * elem_bytes(mode);
-        unsigned char* hi = xp;
-        unsigned char* lo = planes(mode) > 1 ? xp + plane : nullptr;
-        xp += plane * planes(mode);
-        if (N[b] == 0) continue;
-        pack(mode, ctx->stream, X[b], op.K, N[b], op.K, cols_pad, op.Kp, hi, lo);
-        for (int j = 0; j < op.nops; ++j) {
-            gemm_group& g = G.g[G.count++];
-            g.E[0] = op.plane(j, 0);
-            g.E[1] = planes(mode) > 1 ? op.plane(j, 1) : g.E[0];
-            g.X[0] = hi;
-            g.X[1] = lo ? lo : hi;
-            g.Y = Y[b] + (size_t) j * op.M * N[b];
-            g.M = op.M;
-            g.N = N[b];
-            g.Kp = op.Kp;
-            g.tiles_n = cols_pad / TILE;
-            g.first_tile = tiles;
-            tiles += (op.rows_pad / TILE) * g.tiles_n;
-        }
-    }
-    IBM_LAUNCH_CHECK(ctx);
-    if (!tiles) return IBM_OK;
-    G.total_tiles = tiles;
-    int blocks = round_up(tiles, 8);
-    G.xcd_chunk = blocks / 8;
-    double flops = 0;
-    for (int b = 0; b < count; ++b) flops += 2.0 * ops[b]->M * (double) N[b] * ops[b]->K * ops[b]->nops;
-    prof_scope prof(ctx, "rbf_gemm", flops);
-    if (mode == MODE_F64) rbf_gemm_kernel<MODE_F64><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
-    else if (mode == MODE_F32) rbf_gemm_kernel<MODE_F32><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
-    else rbf_gemm_kernel<MODE_BF16X3><<<blocks, NTHREADS, 0, ctx->stream>>>(G);
-    IBM_LAUNCH_CHECK(ctx);
-    return IBM_OK;
+    unsigned char* packed = nullptr;
+    return ibm::rbf_eval_range(ctx, ctx->stream, count, ops, N, X, Y, 0, 8, true, &packed);
 }
 
 ibm_status ibm_surface_geometry(ibm_ctx* ctx, int ns, int nc, const double* y, const double* t1, const double* t2,

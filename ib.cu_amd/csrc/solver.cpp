@@ -127,6 +127,10 @@ struct ibm_solver {
     ibm_ctx* aux[2]{};
     hipStream_t aux_stream[2]{};
     hipEvent_t fork_ev = nullptr, join_ev[2]{};
+    // ... and evaluate the two tangent operators and the surface geometry
+    // (outputs of the step that nothing later in it reads) on the first
+    // auxiliary stream, beside the tether forces, spreading and advection.
+    hipEvent_t geo_ev = nullptr;
     double history_dt = 0;
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
@@ -183,6 +187,7 @@ void release(ibm_solver* s)
         if (s->join_ev[k]) (void) hipEventDestroy(s->join_ev[k]);
     }
     if (s->fork_ev) (void) hipEventDestroy(s->fork_ev);
+    if (s->geo_ev) (void) hipEventDestroy(s->geo_ev);
     if (s->par.fscale > 0) {
         for (int e = 0; e < 5; ++e)
             for (int i = 0; i < 3; ++i)
@@ -247,8 +252,15 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     const ibm_solver_params& P = s->par;
     const double nu = P.diffusivity;
     const int nt = (int) s->types.size();
+    // Fixed-count steps overlap independent work on the auxiliary streams
+    // (no host synchronisation inside; profiled steps stay on one stream so
+    // that every launch is timed).  Everything forked is joined back into
+    // the caller's stream before the projection.
+    const bool concurrent = fixed && !ctx->profiling && s->aux[0] && s->aux[1];
 
-    // 1. geometry (one launch for every cell type and operator)
+    // 1. geometry: positions, tangents, normals and surface measures of
+    // every cell type (the positions in one launch for all types, the
+    // tangents in a second one)
     {
         ibm_rbf* ops[MAX_TYPES];
         int N[MAX_TYPES];
@@ -264,13 +276,24 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             Y[count] = T.Y;
             ++count;
         }
-        if (count) IBM_TRY(ibm_rbf_eval(ctx, count, ops, N, X, Y));
+        // Positions (operator 0) first: the forces and the spreading need
+        // only them.
+        ibm_ctx* gc = concurrent ? s->aux[0] : ctx;
+        unsigned char* packed = nullptr;
+        if (count) {
+            IBM_TRY(rbf_eval_range(ctx, ctx->stream, count, ops, N, X, Y, 0, 1, true, &packed));
+            if (concurrent) {
+                IBM_HIP(ctx, hipEventRecord(s->geo_ev, ctx->stream));
+                IBM_HIP(ctx, hipStreamWaitEvent(gc->stream, s->geo_ev, 0));
+            }
+            IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 8, false, &packed)));
+        }
         for (int t = 0; t < nt; ++t) {
             auto& T = s->types[t];
             if (!T.nc) continue;
             size_t blk = (size_t) T.ns * 3 * T.nc;
-            IBM_TRY(ibm_surface_geometry(ctx, T.ns, T.nc, T.Y, T.Y + blk, T.Y + 2 * blk, T.w, T.normal, T.sigma,
-                                         T.area, T.volume));
+            IBM_TRY(step_status(ctx, gc, ibm_surface_geometry(gc, T.ns, T.nc, T.Y, T.Y + blk, T.Y + 2 * blk, T.w,
+                                                               T.normal, T.sigma, T.area, T.volume)));
             // 2. tether forces
             tether_kernel<<<blocks(blk), 256, 0, ctx->stream>>>(blk, T.stiffness, T.Y, T.ref, T.f);
             IBM_LAUNCH_CHECK(ctx);
@@ -301,9 +324,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         IBM_TRY(ibm_advection(ctx, s->g, u, s->H));
     }
     // 5. diffusion, component by component; fixed-count steps run the three
-    // components concurrently (no host synchronisation inside; profiled
-    // steps stay on one stream so that every launch is timed)
-    const bool concurrent = fixed && !ctx->profiling && s->aux[0] && s->aux[1];
+    // components concurrently
     ibm_ctx* cc[3] = {ctx, concurrent ? s->aux[0] : ctx, concurrent ? s->aux[1] : ctx};
     if (concurrent) {
         IBM_HIP(ctx, hipEventRecord(s->fork_ev, ctx->stream));
@@ -487,7 +508,8 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         }
         s->aux[k]->stream = s->aux_stream[k];
     }
-    if (st == IBM_OK && hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess)
+    if (st == IBM_OK && (hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&s->geo_ev, hipEventDisableTiming) != hipSuccess))
         st = fail(ctx, IBM_ERR_HIP, "solver: fork event");
     if (st == IBM_OK) {
         double* h = nullptr;

@@ -135,6 +135,31 @@ def test_graph_replay_matches_fixed_steps(ibm, ctx, types):
     gctx.close()
 
 
+def test_fixed_step_geometry_matches_adaptive(ibm, ctx, types):
+    """A fixed-count step evaluates the tangents and the surface geometry on
+    a second stream; from the same positions they equal, bitwise, those of
+    an adaptive step, which runs everything on the caller's stream."""
+    import torch
+    a, sa, _ = _solver(ibm, ctx, types, True, precision=32, tol=1e-10)
+    b, sb, _ = _solver(ibm, ctx, types, True, precision=32, tol=1e-10)
+    a.step(sa)
+    b.step(sb)
+    torch.cuda.synchronize()
+    for x, y in zip(sa.xs, sb.xs):
+        assert torch.equal(x, y)
+    a.step_fixed(sa)
+    b.step(sb)
+    torch.cuda.synchronize()
+    for j in range(len(types)):
+        ca, cb = a.cells(j), b.cells(j)
+        for key in ("positions", "forces", "normals", "sigma", "area", "volume"):
+            assert torch.equal(ca[key], cb[key]), (j, key)
+        for t in range(2):
+            assert torch.equal(ca["tangents"][t], cb["tangents"][t]), (j, t)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("order", [0, 2])
 def test_boundary_apply_matches_csr(ibm, orc, ctx, order):
     dom = C.domain(ibm, True)
