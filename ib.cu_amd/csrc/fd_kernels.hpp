@@ -18,7 +18,8 @@ bool cheb_fused_fits(int nx, int ny);
 template <typename T>
 ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], const T* v, T* out, bool add,
                              double* acc, const T* addsrc = nullptr);
-// out = alpha A x + beta y + gamma z, also stored as floats in out32
+// out = alpha A x + beta y + gamma z, also stored as floats in out32 (only
+// there when out is null)
 ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
                                 const double* y, double gamma, const double* z, double* out, float* out32);
 // acc += alpha A x + beta y + gamma z (acc in FP64)

@@ -80,7 +80,8 @@ __device__ inline T stencil_at(const op3<T>& A, T alpha, const T* __restrict__ x
 // OUT selects what is done with each value v: 0 stores out = v; 1 also
 // stores a float copy (the FP32 right-hand side of the next V-cycle); 2 adds
 // it to the FP64 vector acc instead of storing it (the last smoothing pass of
-// a V-cycle folded into the defect correction x += e).
+// a V-cycle folded into the defect correction x += e); 3 stores the float
+// copy alone (a residual that nothing reads in FP64).
 template <typename T, bool HY, bool HZ, int OUT>
 __global__ void __launch_bounds__(STX * STY)
 stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
@@ -142,6 +143,8 @@ stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __re
         const size_t p = col + sxy * (size_t) k;
         if (OUT == 2) {
             acc[p] += (double) v;
+        } else if (OUT == 3) {
+            out32[p] = (float) v;
         } else {
             out[p] = v;
             if (OUT == 1) out32[p] = (float) v;
@@ -1387,8 +1390,8 @@ static ibm_status stencil_dispatch(ibm_ctx* ctx, const op3<T>& A, T alpha, const
     dim3 grid((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
     // x, y, z read and out written; the float copy adds 4 bytes, the FP64
     // read-modify-write of acc replaces the store of out by 16 bytes
-    const double streams = (OUT == 2 ? 1.0 : 2.0) + (y ? 1.0 : 0.0) + (z ? 1.0 : 0.0);
-    const double extra = OUT == 1 ? 4.0 : OUT == 2 ? 16.0 : 0.0;
+    const double streams = (OUT >= 2 ? 1.0 : 2.0) + (y ? 1.0 : 0.0) + (z ? 1.0 : 0.0);
+    const double extra = OUT == 1 || OUT == 3 ? 4.0 : OUT == 2 ? 16.0 : 0.0;
     prof_scope prof(ctx, sizeof(T) == 8 ? "stencil_f64" : "stencil_f32",
                     (streams * sizeof(T) + extra) * nx * ny * (double) nz);
     if (y && z) stencil_kernel<T, true, true, OUT><<<grid, block, 0, ctx->stream>>>(A, alpha, x, beta, y, gamma, z, out, out32, acc);
@@ -1485,6 +1488,7 @@ ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], c
 ibm_status launch_stencil_split(ibm_ctx* ctx, const op3<double>& A, double alpha, const double* x, double beta,
                                 const double* y, double gamma, const double* z, double* out, float* out32)
 {
+    if (!out) return stencil_dispatch<double, 3>(ctx, A, alpha, x, beta, y, gamma, z, nullptr, out32, nullptr);
     return stencil_dispatch<double, 1>(ctx, A, alpha, x, beta, y, gamma, z, out, out32, nullptr);
 }
 

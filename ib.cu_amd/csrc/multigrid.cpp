@@ -230,16 +230,18 @@ static bool fused(const ibm_mg* m)
 }
 
 // One defect-correction iteration: x += V(r), r = b - A x; with fused(),
-// B0.b already holds r in FP32 and is refreshed by the residual pass.
+// B0.b already holds r in FP32 and is refreshed by the residual pass, which
+// skips the FP64 copy of r unless keep_r.
 template <typename T>
-static ibm_status correct(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x)
+static ibm_status correct(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x, bool keep_r = true)
 {
     const mg_level& L0 = m->lv[0];
     const size_t n = L0.n;
     level_bufs<T>& B0 = bufs<T>(m)[0];
     if (fused<T>(m)) {
         IBM_TRY(vcycle<T>(ctx, m, 0, x));
-        return launch_stencil_split(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, m->r, (float*) B0.b);
+        return launch_stencil_split(ctx, L0.A64, -1.0, x, 0.0, nullptr, 1.0, b, keep_r ? m->r : nullptr,
+                                    (float*) B0.b);
     }
     IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
     IBM_TRY(vcycle<T>(ctx, m, 0));
@@ -333,7 +335,8 @@ static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* 
     }
     IBM_TRY(start(ctx, m, b, x, warm));
     if (fused<T>(m) && iters > 0) IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
-    for (int it = 0; it < iters; ++it) IBM_TRY(correct<T>(ctx, m, b, x));
+    // Only the last residual is read (for relres_dev) in FP64.
+    for (int it = 0; it < iters; ++it) IBM_TRY(correct<T>(ctx, m, b, x, relres_dev && it == iters - 1));
     if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
     if (relres_dev) {
         IBM_TRY(launch_dot<double, double>(ctx, n, b, b, m->partial, m->scalars + 2));

@@ -145,17 +145,20 @@ __global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int wi
                                  int* __restrict__ code, int* __restrict__ count,
                                  bin_stats* __restrict__ stats)
 {
-    // No early return: every lane takes part in the wavefront reduction.
+    // No early return: every lane takes part in the wavefront reduction and
+    // in the grouping of the rank atomics.
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long fbits = 0;
-    if (m < sets.total) {
+    const bool valid = m < sets.total;
+    int bin = -1;
+    if (valid) {
         int k = set_of(sets, m);
         int l = m - sets.offset[k];
         int n = sets.offset[k + 1] - sets.offset[k];
         const double* x = sets.x[k];
         double p[3] = {x[l], x[n + l], x[2 * n + l]};
         int q[3];
-        int bin = 0;
+        bin = 0;
         int stride = 1;
         for (int j = 0; j < 3; ++j) {
             stencil_axis st = kstencil(g.a[bd.fc[j]][j], p[j]);
@@ -173,12 +176,36 @@ __global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int wi
             }
         key[m] = bin;
         code[m] = c9;
-        rank[m] = atomicAdd(&count[bin], 1);
         if (with_forces) {
             const double* f = sets.f[k];
             double a = fmax(fabs(f[l]), fmax(fabs(f[n + l]), fabs(f[2 * n + l])));
             fbits = (unsigned long long) __double_as_longlong(a);
         }
+    }
+    // Ranks inside a bin: consecutive markers mostly share a bin, so the
+    // lanes of a wavefront are grouped by bin (one ballot per distinct bin)
+    // and each group's first lane takes the group's ranks with one atomic;
+    // the groups' atomics go out together, to distinct addresses.
+    {
+        const int lane = threadIdx.x & 63;
+        const unsigned long long lt = (1ull << lane) - 1;
+        unsigned long long todo = __ballot(valid);
+        int leader = lane, before = 0, size = 0;
+        while (todo) {
+            const int l = __ffsll((long long) todo) - 1;
+            const int lb = __shfl(bin, l, 64);
+            const unsigned long long grp = __ballot(valid && bin == lb) & todo;
+            if ((grp >> lane) & 1) {
+                leader = l;
+                before = __popcll(grp & lt);
+            }
+            if (lane == l) size = __popcll(grp);
+            todo &= ~grp;
+        }
+        int base = 0;
+        if (valid && leader == lane) base = atomicAdd(&count[bin], size);
+        base = __shfl(base, leader, 64);
+        if (valid) rank[m] = base + before;
     }
     if (with_forces) {
         fbits = wave_max(fbits);
@@ -186,19 +213,21 @@ __global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int wi
     }
 }
 
+// Places each marker's ordering key (code, marker id) in its bin's range.
 __global__ void bin_scatter_kernel(int total, const int* __restrict__ key, const int* __restrict__ rank,
-                                   const int* __restrict__ start, int* __restrict__ tmp)
+                                   const int* __restrict__ code, const int* __restrict__ start,
+                                   long long* __restrict__ tmp)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= total) return;
-    tmp[start[key[m]] + rank[m]] = m;
+    tmp[start[key[m]] + rank[m]] = ((long long) code[m] << 32) | (unsigned) m;
 }
 
 // Canonical order inside a bin: by (code, marker id).  Writes the permutation
 // and sorted copies of positions and forces (SoA, 6 * total doubles).
 __global__ void bin_canon_kernel(marker_sets sets, int with_forces,
-                                 const int* __restrict__ key, const int* __restrict__ code,
-                                 const int* __restrict__ start, const int* __restrict__ tmp,
+                                 const int* __restrict__ key, const int* __restrict__ start,
+                                 const long long* __restrict__ tmp,
                                  int* __restrict__ perm, double* __restrict__ sorted,
                                  bin_stats* __restrict__ stats)
 {
@@ -206,16 +235,12 @@ __global__ void bin_canon_kernel(marker_sets sets, int with_forces,
     int total = sets.total;
     unsigned long long cnt = 0;
     if (p < total) {
-        int m = tmp[p];
+        const long long mine = tmp[p];
+        int m = (int) (mine & 0xffffffffll);
         int b = key[m];
         int s = start[b], e = start[b + 1];
-        long long mine = ((long long) code[m] << 32) | (unsigned) m;
         int r = 0;
-        for (int i = s; i < e; ++i) {
-            int m2 = tmp[i];
-            long long other = ((long long) code[m2] << 32) | (unsigned) m2;
-            r += other < mine;
-        }
+        for (int i = s; i < e; ++i) r += tmp[i] < mine;
         int dst = s + r;
         perm[dst] = m;
         cnt = (unsigned long long) (e - s);
@@ -823,14 +848,15 @@ ibm_status bin_markers(ibm_ctx* ctx, const marker_sets& sets, const kgrids& kg, 
 {
     const int total = sets.total;
     const int nbins = bd.nbins;
-    int *count, *start, *key, *rank, *tmp, *perm, *code;
+    int *count, *start, *key, *rank, *perm, *code;
+    long long* tmp;
     double* sorted;
     void* p;
     IBM_TRY(scratch(ctx, SLOT_BIN_COUNT, sizeof(int) * (size_t) (nbins + 1), &p)); count = (int*) p;
     IBM_TRY(scratch(ctx, SLOT_BIN_START, sizeof(int) * (size_t) (nbins + 1), &p)); start = (int*) p;
     IBM_TRY(scratch(ctx, SLOT_KEY, sizeof(int) * (size_t) total, &p)); key = (int*) p;
     IBM_TRY(scratch(ctx, SLOT_RANK, sizeof(int) * (size_t) total, &p)); rank = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_TMP, sizeof(int) * (size_t) total, &p)); tmp = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_TMP, sizeof(long long) * (size_t) total, &p)); tmp = (long long*) p;
     IBM_TRY(scratch(ctx, SLOT_PERM, sizeof(int) * (size_t) total, &p)); perm = (int*) p;
     IBM_TRY(scratch(ctx, SLOT_CODE, sizeof(int) * (size_t) total, &p)); code = (int*) p;
     IBM_TRY(scratch(ctx, SLOT_SORTED, sizeof(double) * 6 * (size_t) total, &p)); sorted = (double*) p;
@@ -856,10 +882,10 @@ ibm_status bin_markers(ibm_ctx* ctx, const marker_sets& sets, const kgrids& kg, 
     IBM_TRY(scratch(ctx, SLOT_SCAN_TMP, tbytes, &p));
     IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(p, tbytes, count, start, nbins + 1, ctx->stream));
     if (total > 0) {
-        bin_scatter_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(total, key, rank, start, tmp);
+        bin_scatter_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(total, key, rank, code, start, tmp);
         IBM_LAUNCH_CHECK(ctx);
-        bin_canon_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, with_forces, key, code, start,
-                                                                           tmp, perm, sorted, stats);
+        bin_canon_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, with_forces, key, start, tmp, perm,
+                                                                           sorted, stats);
         IBM_LAUNCH_CHECK(ctx);
     }
     *start_out = start;
