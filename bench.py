@@ -207,8 +207,8 @@ def cpu_baseline(inputs, orc, fd, rbf, L, types, gpu_iters):
 # Kernel-name prefixes of the counter files for each timed kernel group.
 PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "cheb_march_kernel<double",
                "stencil_f32": "stencil_kernel<float", "stencil_f64": "stencil_kernel<double",
-               "restrict": "restrict_kernel<", "residual_restrict": "residual_restrict_kernel",
-               "prolong": "prolong_add_kernel", "prolong_residual": "prolong_residual_kernel",
+               "restrict": "::restrict_kernel<", "residual_restrict": "residual_restrict_kernel",
+               "prolong": "::prolong_add_kernel", "prolong_residual": "prolong_residual_kernel",
                "rbf_gemm": "rbf_gemm_kernel",
                "spread_brick": "spread_brick_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
 COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
