@@ -8,6 +8,7 @@
 // +8/+4 per extra input vector) instead of ~100 B of CSR data per row.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <vector>
 #include "context.hpp"
@@ -187,10 +188,16 @@ struct march_tile {
     static constexpr int MIN_Z = WIDE ? 4 : 2;   // shortest z chunk
 };
 constexpr size_t MARCH_WIDE_POINTS = (size_t) 1 << 19;   // grids from 512K points use the wide tile
-constexpr int MARCH_NV = 7;      // v planes alive in a phase
+// Ring sizes are powers of two and the phase loop is unrolled by MARCH_NV, so
+// every ring slot a phase touches is a compile-time constant: the per-phase
+// uniform work (issued by the scalar unit, one instruction per SIMD every
+// four cycles, like a wave64 VALU op) is then a few compares and adds.
+constexpr int MARCH_NV = 8;      // v planes in the ring (7 alive in a phase)
 constexpr int MARCH_NS = 4;      // s1 (and s2) planes alive in a phase
 constexpr int MARCH_LEAD = 9;    // phases between loading a v plane and storing its output
 constexpr int MARCH_TARGET_BLOCKS = 512;
+constexpr int MARCH_PF = 2;      // phases between loading a plane's operands and using them
+static_assert(MARCH_NV % MARCH_NS == 0 && MARCH_NV % MARCH_PF == 0, "ring periods must divide the unroll");
 
 // Global index of coordinate o - h + c along an axis: wrapped on a periodic
 // axis, -1 outside a walled one.
@@ -199,8 +206,13 @@ __device__ inline int axis_global(const op_axis<T>& a, int o, int h, int c)
 {
     int g = o - h + c;
     if constexpr (P) {
-        g %= a.n;
+        // One period either side without a division (the usual case).
         if (g < 0) g += a.n;
+        else if (g >= a.n) g -= a.n;
+        if ((unsigned) g >= (unsigned) a.n) {
+            g %= a.n;
+            if (g < 0) g += a.n;
+        }
         return g;
     } else {
         return (g < 0 || g >= a.n) ? -1 : g;
@@ -304,6 +316,31 @@ __device__ inline T plane_op_int(const T* Sm, const T* Sc, const T* Sp, const xy
            k.az * (Sm[c] + Sp[c]);
 }
 
+// The seven operands of one plane op, loaded before any arithmetic.
+template <typename T>
+struct nbr7 {
+    T x0, xm, xp, ym, yp, zm, zp;
+};
+
+template <int SY, typename T>
+__device__ inline nbr7<T> load7(const T* Sm, const T* Sc, const T* Sp, int c)
+{
+    return {Sc[c], Sc[c - 1], Sc[c + 1], Sc[c - SY], Sc[c + SY], Sm[c], Sp[c]};
+}
+
+// plane_apply on loaded operands (the same expressions).
+template <int PER, typename T>
+__device__ inline T apply7(const op3<T>& A, bool interior, const nbr7<T>& n, const xy_coef<T>& k,
+                           const z_coef<T>& kz)
+{
+    if ((PER & 4) != 0 || interior)
+        return k.a0 * n.x0 + k.axm * n.xm + k.axp * n.xp + k.aym * n.ym + k.ayp * n.yp + k.az * (n.zm + n.zp);
+    const T lx = lap_coef<(PER & 1) != 0>(A.a[0].r2, k.xlo, k.xhi, k.dx, n.xm, n.x0, n.xp);
+    const T ly = lap_coef<(PER & 2) != 0>(A.a[1].r2, k.ylo, k.yhi, k.dy, n.ym, n.x0, n.yp);
+    const T lz = lap_coef<(PER & 4) != 0>(A.a[2].r2, kz.zlo, kz.zhi, kz.dz, n.zm, n.x0, n.zp);
+    return A.sigma * (k.IxIy * kz.Iz) * n.x0 + A.tau * (k.Iy * kz.Iz * lx + k.Ix * kz.Iz * ly + k.IxIy * lz);
+}
+
 // Plane op choosing the collapsed form where the plane is away from z walls
 // (uniform per plane).
 template <int PER, int SY, typename T>
@@ -314,8 +351,10 @@ __device__ inline T plane_apply(const op3<T>& A, bool interior, const T* Sm, con
     return plane_op<PER, SY>(A, Sm, Sc, Sp, k, kz);
 }
 
+// Two blocks per CU (four waves per SIMD for the wide tile): at most 128 VGPRs.
 template <typename T, bool ADD, bool ACC, int PER, bool WIDE>
 __global__ void __launch_bounds__((march_tile<T, WIDE>::THREADS))
+__attribute__((amdgpu_waves_per_eu(march_tile<T, WIDE>::THREADS / 128)))
 cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* out, const T* addsrc,
                   double* __restrict__ acc, int tiles_x, int tiles_xy, int zchunk, int nblocks, int xcd_chunk)
 {
@@ -327,11 +366,16 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
     constexpr int W1 = TX + 4, P1 = W1 * (TY + 4), N1 = (P1 + MARCH_THREADS - 1) / MARCH_THREADS;
     constexpr int W2 = TX + 2, P2 = W2 * (TY + 2), N2 = (P2 + MARCH_THREADS - 1) / MARCH_THREADS;
     constexpr int PO = TX * TY, NO = (PO + MARCH_THREADS - 1) / MARCH_THREADS;
-    constexpr int NZ0 = (2 * MARCH_NS * VP + MARCH_THREADS - 1) / MARCH_THREADS;
+    // s1 / s2 planes carry 64 slots past the grown plane (one per lane) that
+    // positions without a value write to and nothing reads, so the stages run
+    // without per-position branches (and so issue all their LDS reads before
+    // one wait).  Such positions read at SAFE, whose neighbours lie in the plane.
+    constexpr int VPS = VP + 64, SAFE = VX + 1;
+    constexpr int NZ0 = (2 * MARCH_NS * VPS + MARCH_THREADS - 1) / MARCH_THREADS;
     __shared__ T Rv[MARCH_NV][VP];
-    __shared__ T Rs[2][MARCH_NS][VP];   // s1, s2
-    T(*R1)[VP] = Rs[0];
-    T(*R2)[VP] = Rs[1];
+    __shared__ T Rs[2][MARCH_NS][VPS];   // s1, s2
+    T(*R1)[VPS] = Rs[0];
+    T(*R2)[VPS] = Rs[1];
 
     int b = blockIdx.x;
     if (xcd_chunk) b = (b & 7) * xcd_chunk + (b >> 3);
@@ -339,7 +383,6 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
     const int tile = b % tiles_xy, chunk = b / tiles_xy;
     const int ox = (tile % tiles_x) * TX, oy = (tile / tiles_x) * TY;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
-    const size_t nxy = (size_t) nx * ny;
     const int z0 = chunk * zchunk, z1 = min(z0 + zchunk, nz);
     const int tid = threadIdx.x;
 
@@ -348,7 +391,7 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
 #pragma unroll
     for (int r = 0; r < NZ0; ++r) {
         const int t = tid + r * MARCH_THREADS;
-        if (t < 2 * MARCH_NS * VP) (&Rs[0][0][0])[t] = T(0);
+        if (t < 2 * MARCH_NS * VPS) (&Rs[0][0][0])[t] = T(0);
     }
 
     // Positions of this thread, the same on every plane.
@@ -365,6 +408,8 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
         lcol[r] = col;
     }
     xy_coef<T> k1[N1], k2[N2], ko[NO];
+    int ws1[N1], ws2[N2];   // s1 / s2 write slots (past the plane where no value)
+    bool ovalid[NO];
     int ocol[NO];   // output: global column gi + nx gj
 #pragma unroll
     for (int r = 0; r < N1; ++r) {
@@ -372,6 +417,8 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
         const int cx = 1 + t % W1, cy = 1 + t / W1;
         const int gi = axis_global<PX>(A.a[0], ox, 3, cx), gj = axis_global<PY>(A.a[1], oy, 3, cy);
         k1[r] = make_xy_coef<PER>(A, t < P1 && gi >= 0 && gj >= 0, cx + VX * cy, gi, gj);
+        ws1[r] = k1[r].c >= 0 ? k1[r].c : VP + (tid & 63);
+        if (k1[r].c < 0) k1[r].c = SAFE;
     }
 #pragma unroll
     for (int r = 0; r < N2; ++r) {
@@ -379,6 +426,8 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
         const int cx = 2 + t % W2, cy = 2 + t / W2;
         const int gi = axis_global<PX>(A.a[0], ox, 3, cx), gj = axis_global<PY>(A.a[1], oy, 3, cy);
         k2[r] = make_xy_coef<PER>(A, t < P2 && gi >= 0 && gj >= 0, cx + VX * cy, gi, gj);
+        ws2[r] = k2[r].c >= 0 ? k2[r].c : VP + (tid & 63);
+        if (k2[r].c < 0) k2[r].c = SAFE;
     }
 #pragma unroll
     for (int r = 0; r < NO; ++r) {
@@ -387,109 +436,147 @@ cheb_march_kernel(op3<T> A, T w0, T w1, T w2, T w3, const T* __restrict__ v, T* 
         const int gi = ox + tx, gj = oy + ty;
         const bool valid = t < PO && gi < nx && gj < ny;
         ko[r] = make_xy_coef<PER>(A, valid, (tx + 3) + VX * (ty + 3), gi, gj);
+        ovalid[r] = valid;
+        if (!valid) ko[r].c = SAFE;
         ocol[r] = valid ? gi + nx * gj : 0;
     }
 
-    // Ring slot of plane z (z >= z0 - 3).
-    auto sv = [&](int z) { return (z - z0 + 3) % MARCH_NV; };
-    auto ss = [&](int z) { return (z - z0 + 3) % MARCH_NS; };
+    // Plane index along z: wrapped on a periodic axis, -1 outside a walled
+    // one (uniform; no division for planes within a period of the grid).
+    auto zglobal = [&](int z) -> int {
+        if constexpr (PZ) {
+            z = z < 0 ? z + nz : z;
+            z = z >= nz ? z - nz : z;
+            if ((unsigned) z >= (unsigned) nz) {   // z axes shorter than the march lead
+                while (z < 0) z += nz;
+                while (z >= nz) z -= nz;
+            }
+            return z;
+        } else {
+            return (z < 0 || z >= nz) ? -1 : z;
+        }
+    };
+    const unsigned nxy = (unsigned) nx * (unsigned) ny;   // grids hold < 2^32 points
 
-    // Memory operands are loaded two phases ahead of their use, into one of
-    // two register sets (the phase loop is unrolled by two).  Every load is
-    // issued unconditionally, from a valid address (a point outside a wall or
-    // the chunk reads point 0, and is masked where it is used), so that the
-    // waits before the uses count only the loads issued before them.
+    // Memory operands are loaded MARCH_PF phases ahead of their use, into one
+    // of MARCH_PF register sets.  Every load is issued unconditionally, from a
+    // valid address (a point outside a wall or the chunk reads point 0, and is
+    // masked where it is used), so that the waits before the uses count only
+    // the loads issued before them.
     auto fetch = [&](int z, T (&pre)[NL]) {
-        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], z, 0, 0), 0);
+        const unsigned k = (unsigned) max(zglobal(z), 0) * nxy;
 #pragma unroll
-        for (int r = 0; r < NL; ++r) pre[r] = v[(size_t) max(lcol[r], 0) + nxy * k];
+        for (int r = 0; r < NL; ++r) pre[r] = v[(unsigned) max(lcol[r], 0) + k];
     };
     auto fetch_out = [&](int z, T (&po)[NO], double (&pa)[NO]) {
-        const size_t k = (size_t) min(max(z, z0), z1 - 1);
+        const unsigned k = (unsigned) min(max(z, z0), z1 - 1) * nxy;
 #pragma unroll
         for (int r = 0; r < NO; ++r) {
-            const size_t p = (size_t) ocol[r] + nxy * k;
+            const unsigned p = (unsigned) ocol[r] + k;
             if (ADD) po[r] = addsrc[p];
             if (ACC) pa[r] = acc[p];
         }
     };
 
     const int Z = z1 - z0;
-    auto phase = [&](int q, T (&pre)[NL], T (&po)[NO], double (&pa)[NO]) {
-        if (q >= 3 && q <= Z + 6) {          // s1 = w1 v - w0 A v
+    T pre[MARCH_PF][NL], po[MARCH_PF][NO];
+    double pa[MARCH_PF][NO];
+
+    // Phase q = (multiple of MARCH_NV) + P.  Ring slot of plane z: (z - z0 + 3)
+    // mod the ring size; the v plane stored in phase q is z0 - 3 + q, so the
+    // slots below are constants given P.
+    auto phase = [&]<int P>(int q) {
+        constexpr int F = P % MARCH_PF;
+        if (q >= 3 && q <= Z + 6) {          // s1 = w1 v - w0 A v on plane z0-5+q
             const int z = z0 - 5 + q;
-            const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
+            const int gk = zglobal(z);
             if (gk >= 0) {
                 const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
                 const bool in = gk > 0 && gk < nz - 1;
-                const T *Vm = Rv[sv(z - 1)], *Vc = Rv[sv(z)], *Vp = Rv[sv(z + 1)];
-                T* dst = R1[ss(z)];
+                const T *Vm = Rv[(P - 3) & (MARCH_NV - 1)], *Vc = Rv[(P - 2) & (MARCH_NV - 1)],
+                        *Vp = Rv[(P - 1) & (MARCH_NV - 1)];
+                T* dst = R1[(P - 2) & (MARCH_NS - 1)];
+                nbr7<T> nb[N1];
 #pragma unroll
-                for (int r = 0; r < N1; ++r) {
-                    if (k1[r].c < 0) continue;
-                    const int c = k1[r].c;
-                    dst[c] = -w0 * plane_apply<PER, VX>(A, in, Vm, Vc, Vp, k1[r], kz) + w1 * Vc[c];
-                }
+                for (int r = 0; r < N1; ++r) nb[r] = load7<VX>(Vm, Vc, Vp, k1[r].c);
+                __builtin_amdgcn_sched_barrier(0);   // every row's reads in flight before the arithmetic
+#pragma unroll
+                for (int r = 0; r < N1; ++r)
+                    dst[ws1[r]] = -w0 * apply7<PER>(A, in, nb[r], k1[r], kz) + w1 * nb[r].x0;
             }
         }
-        if (q >= 6 && q <= Z + 7) {          // s2 = w2 v - A s1
+        if (q >= 6 && q <= Z + 7) {          // s2 = w2 v - A s1 on plane z0-7+q
             const int z = z0 - 7 + q;
-            const int gk = axis_global<PZ>(A.a[2], z, 0, 0);
+            const int gk = zglobal(z);
             if (gk >= 0) {
                 const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
                 const bool in = gk > 0 && gk < nz - 1;
-                const T *Sm = R1[ss(z - 1)], *Sc = R1[ss(z)], *Sp = R1[ss(z + 1)];
-                const T* Vc = Rv[sv(z)];
-                T* dst = R2[ss(z)];
+                const T *Sm = R1[(P - 5) & (MARCH_NS - 1)], *Sc = R1[(P - 4) & (MARCH_NS - 1)],
+                        *Sp = R1[(P - 3) & (MARCH_NS - 1)];
+                const T* Vc = Rv[(P - 4) & (MARCH_NV - 1)];
+                T* dst = R2[(P - 4) & (MARCH_NS - 1)];
+                nbr7<T> nb[N2];
+                T vc[N2];
 #pragma unroll
                 for (int r = 0; r < N2; ++r) {
-                    if (k2[r].c < 0) continue;
-                    const int c = k2[r].c;
-                    dst[c] = T(-1) * plane_apply<PER, VX>(A, in, Sm, Sc, Sp, k2[r], kz) + w2 * Vc[c];
+                    nb[r] = load7<VX>(Sm, Sc, Sp, k2[r].c);
+                    vc[r] = Vc[k2[r].c];
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < N2; ++r)
+                    dst[ws2[r]] = T(-1) * apply7<PER>(A, in, nb[r], k2[r], kz) + w2 * vc[r];
             }
         }
-        if (q >= MARCH_LEAD) {               // out = w3 v - A s2 (+ out)
+        if (q >= MARCH_LEAD) {               // out = w3 v - A s2 (+ out) on plane z0-9+q
             const int z = z0 - MARCH_LEAD + q;
             const z_coef<T> kz = make_z_coef<PZ>(A.a[2], z);
             const bool in = z > 0 && z < nz - 1;
-            const T *Sm = R2[ss(z - 1)], *Sc = R2[ss(z)], *Sp = R2[ss(z + 1)];
-            const T* Vc = Rv[sv(z)];
+            const T *Sm = R2[(P - 7) & (MARCH_NS - 1)], *Sc = R2[(P - 6) & (MARCH_NS - 1)],
+                    *Sp = R2[(P - 5) & (MARCH_NS - 1)];
+            const T* Vc = Rv[(P - 6) & (MARCH_NV - 1)];
+            nbr7<T> nb[NO];
+            T vc[NO];
 #pragma unroll
             for (int r = 0; r < NO; ++r) {
-                if (ko[r].c < 0) continue;
-                const size_t p = (size_t) ocol[r] + nxy * (size_t) z;
-                T val = T(-1) * plane_apply<PER, VX>(A, in, Sm, Sc, Sp, ko[r], kz);
-                if (ADD) val += T(1) * po[r];
-                val += w3 * Vc[ko[r].c];
-                if (ACC) acc[p] = pa[r] + (double) val;
-                else out[p] = val;
+                nb[r] = load7<VX>(Sm, Sc, Sp, ko[r].c);
+                vc[r] = Vc[ko[r].c];
+            }
+            const unsigned kz0 = (unsigned) z * nxy;
+#pragma unroll
+            for (int r = 0; r < NO; ++r) {
+                const unsigned p = (unsigned) ocol[r] + kz0;
+                T val = T(-1) * apply7<PER>(A, in, nb[r], ko[r], kz);
+                if (ADD) val += T(1) * po[F][r];
+                val += w3 * vc[r];
+                if (ovalid[r]) {
+                    if (ACC) acc[p] = pa[F][r] + (double) val;
+                    else out[p] = val;
+                }
             }
         }
         // The loads are waited for at the end of the phase, after the stages.
         if (q <= Z + 5) {                    // v plane z0-3+q (0 outside a wall)
-            const bool in = axis_global<PZ>(A.a[2], z0 - 3 + q, 0, 0) >= 0;
-            T* dst = Rv[sv(z0 - 3 + q)];
+            const bool in = zglobal(z0 - 3 + q) >= 0;
+            T* dst = Rv[P & (MARCH_NV - 1)];
 #pragma unroll
             for (int r = 0; r < NL; ++r) {
                 const int t = tid + r * MARCH_THREADS;
-                if (t < VP) dst[t] = (in && lcol[r] >= 0) ? pre[r] : T(0);
+                if (t < VP) dst[t] = (in && lcol[r] >= 0) ? pre[F][r] : T(0);
             }
         }
-        fetch(z0 - 1 + q, pre);              // v plane stored two phases on
-        if (ADD || ACC) fetch_out(z0 - MARCH_LEAD + 2 + q, po, pa);   // output operands of phase q + 2
+        fetch(z0 - 3 + MARCH_PF + q, pre[F]);   // v plane stored MARCH_PF phases on
+        if (ADD || ACC) fetch_out(z0 - MARCH_LEAD + MARCH_PF + q, po[F], pa[F]);   // operands of phase q + MARCH_PF
         __syncthreads();
     };
+    auto group = [&]<int... Ps>(int q0, std::integer_sequence<int, Ps...>) {
+        ((q0 + Ps < Z + MARCH_LEAD ? (phase.template operator()<Ps>(q0 + Ps), 0) : 0), ...);
+    };
 
-    T preA[NL], preB[NL], poA[NO], poB[NO];
-    double paA[NO], paB[NO];
-    fetch(z0 - 3, preA);
-    fetch(z0 - 2, preB);
+#pragma unroll
+    for (int f = 0; f < MARCH_PF; ++f) fetch(z0 - 3 + f, pre[f]);
     const int nq = Z + MARCH_LEAD;
-    for (int q = 0; q < nq; q += 2) {
-        phase(q, preA, poA, paA);
-        if (q + 1 < nq) phase(q + 1, preB, poB, paB);
-    }
+    for (int q0 = 0; q0 < nq; q0 += MARCH_NV) group(q0, std::make_integer_sequence<int, MARCH_NV>{});
 }
 
 // ---------------------------------------------------------------------------
