@@ -687,8 +687,12 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
     constexpr int RX = 2 * CX + 2, RP = RX * (2 * CY + 2);              // residual plane: grown by 1
     constexpr int NL = (XP + NT - 1) / NT, NR = (RP + NT - 1) / NT;
     static_assert(CX * CY <= NT, "one coarse column per thread");
+    // Residual planes carry 64 slots past the plane (one per lane) that
+    // positions without a value write to and nothing reads; those positions
+    // read at the in-plane position XX + 1, so the rows run unbranched.
+    constexpr int RPS = XP + 64;
     __shared__ T Xr[4][XP];
-    __shared__ T Rr[2][XP];
+    __shared__ T Rr[2][RPS];
 
     int blk = blockIdx.x;
     if (xcd_chunk) blk = (blk & 7) * xcd_chunk + (blk >> 3);
@@ -696,7 +700,7 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
     const int tile = blk % tiles_xy, chunk = blk / tiles_xy;
     const int cx0 = (tile % tiles_x) * CX, cy0 = (tile / tiles_x) * CY;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
-    const size_t nxy = (size_t) nx * ny;
+    const unsigned nxy = (unsigned) nx * (unsigned) ny;   // grids hold < 2^32 points
     const int ncx = R.a[0].nc, ncy = R.a[1].nc, ncz = R.a[2].nc;
     const int k0 = chunk * kchunk, k1 = min(k0 + kchunk, ncz);
     const int tid = threadIdx.x;
@@ -716,7 +720,7 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
     }
     // residual positions (x-plane layout, offset by the halo of 1)
     xy_coef<T> kr[NR];
-    int rcol[NR];
+    int rcol[NR], rws[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int t = tid + r * NT;
@@ -724,6 +728,8 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
         const int gi = axis_global<PX>(A.a[0], fx0, 0, i), gj = axis_global<PY>(A.a[1], fy0, 0, j);
         const bool valid = t < RP && gi >= 0 && gj >= 0;
         kr[r] = make_xy_coef<PER>(A, valid, i + XX * j, gi, gj);
+        rws[r] = valid ? kr[r].c : XP + (tid & 63);
+        if (!valid) kr[r].c = XX + 1;
         rcol[r] = valid ? gi + nx * gj : 0;
     }
     // this thread's coarse column and its x-y taps
@@ -745,23 +751,23 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
     auto sx = [&](int f) { return (f - fs + 1) & 3; };     // x ring slot of plane f >= fs - 1
 
     auto fetch_x = [&](int f, T (&pre)[NL]) {
-        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+        const unsigned k = (unsigned) max(axis_global<PZ>(A.a[2], f, 0, 0), 0) * nxy;
 #pragma unroll
-        for (int r = 0; r < NL; ++r) pre[r] = x[(size_t) max(lcol[r], 0) + nxy * k];
+        for (int r = 0; r < NL; ++r) pre[r] = x[(unsigned) max(lcol[r], 0) + k];
     };
-    auto store_x = [&](int f, const T (&pre)[NL]) {
+    auto store_x_slot = [&](int f, T* dst, const T (&pre)[NL]) {
         const bool in = axis_global<PZ>(A.a[2], f, 0, 0) >= 0;
-        T* dst = Xr[sx(f)];
 #pragma unroll
         for (int r = 0; r < NL; ++r) {
             const int t = tid + r * NT;
             if (t < XP) dst[t] = (in && lcol[r] >= 0) ? pre[r] : T(0);
         }
     };
+    auto store_x = [&](int f, const T (&pre)[NL]) { store_x_slot(f, Xr[sx(f)], pre); };
     auto fetch_b = [&](int f, T (&pb)[NR]) {
-        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+        const unsigned k = (unsigned) max(axis_global<PZ>(A.a[2], f, 0, 0), 0) * nxy;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) pb[r] = b[(size_t) rcol[r] + nxy * k];
+        for (int r = 0; r < NR; ++r) pb[r] = b[(unsigned) rcol[r] + k];
     };
 
     // x planes fs - 1 .. fs + 1 before the march
@@ -776,13 +782,17 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
     }
     T P0 = 0, P1 = 0, P2 = 0, P3 = 0;      // partial sums of fine planes f-3 .. f
     int next_k = k0;
-    const size_t ncxy = (size_t) ncx * ncy;
-    const size_t ccol = has_c ? (size_t) ci + (size_t) ncx * cj : 0;
+    const unsigned ncxy = (unsigned) ncx * (unsigned) ncy;
+    const unsigned ccol = has_c ? (unsigned) ci + (unsigned) ncx * cj : 0;
 
-    auto phase = [&](int q, T (&px)[NL], T (&pb)[NR]) {
+    // Phase q = (even) + S: plane f = fs + q sits in x slot (q + 1) & 3 and in
+    // residual slot S (a constant; the loop is unrolled by two).
+    T px[2][NL], pb[2][NR];
+    auto phase = [&]<int S>(int q) {
+        constexpr int F = S & 1;
         const int fp = fs + q - 1;           // residual plane of the previous phase
         if (q >= 1) {
-            const T* Rc = Rr[(q - 1) & 1];
+            const T* Rc = Rr[(S - 1) & 1];
             T acc = 0;
 #pragma unroll
             for (int sy = 0; sy < 4; ++sy) {
@@ -807,7 +817,7 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
                     const T pv = d == 0 ? P3 : d == 1 ? P2 : d == 2 ? P1 : P0;
                     out += wz * pv;
                 }
-                if (has_c) coarse[ccol + ncxy * (size_t) k] = out;
+                if (has_c) coarse[ccol + ncxy * (unsigned) k] = out;
             }
         }
         const int f = fs + q;                // residual plane of this phase
@@ -815,31 +825,28 @@ residual_restrict_kernel(op3<T> A, transfer3<T> R, const T* __restrict__ x, cons
             const int gk = axis_global<PZ>(A.a[2], f, 0, 0);
             const z_coef<T> kz = make_z_coef<PZ>(A.a[2], gk);
             const bool in = gk > 0 && gk < nz - 1;
-            const T *Xm = Xr[sx(f - 1)], *Xc = Xr[sx(f)], *Xp = Xr[sx(f + 1)];
-            T* dst = Rr[q & 1];
+            const T *Xm = Xr[q & 3], *Xc = Xr[(q + 1) & 3], *Xp = Xr[(q + 2) & 3];
+            T* dst = Rr[F];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (kr[r].c < 0) continue;
-                dst[kr[r].c] = T(-1) * plane_apply<PER, XX>(A, in, Xm, Xc, Xp, kr[r], kz) + T(1) * pb[r];
-            }
+            for (int r = 0; r < NR; ++r)
+                dst[rws[r]] = T(-1) * plane_apply<PER, XX>(A, in, Xm, Xc, Xp, kr[r], kz) + T(1) * pb[F][r];
         }
-        store_x(f + 2, px);                  // fetched two phases ago
-        fetch_x(f + 4, px);
-        fetch_b(f + 2, pb);
+        store_x_slot(f + 2, Xr[(q + 3) & 3], px[F]);   // fetched two phases ago
+        fetch_x(f + 4, px[F]);
+        fetch_b(f + 2, pb[F]);
         __syncthreads();
     };
 
-    T pxA[NL], pxB[NL], pbA[NR], pbB[NR];
-    fetch_x(fs + 2, pxA);
-    fetch_x(fs + 3, pxB);
-    fetch_b(fs, pbA);
-    fetch_b(fs + 1, pbB);
+    fetch_x(fs + 2, px[0]);
+    fetch_x(fs + 3, px[1]);
+    fetch_b(fs, pb[0]);
+    fetch_b(fs + 1, pb[1]);
     __syncthreads();
     const int nq = fe - fs + 2;
-    for (int q = 0; q < nq; q += 2) {
-        phase(q, pxA, pbA);
-        if (q + 1 < nq) phase(q + 1, pxB, pbB);
-    }
+    auto group = [&]<int... Ss>(int q0, std::integer_sequence<int, Ss...>) {
+        ((q0 + Ss < nq ? (phase.template operator()<Ss>(q0 + Ss), 0) : 0), ...);
+    };
+    for (int q0 = 0; q0 < nq; q0 += 2) group(q0, std::make_integer_sequence<int, 2>{});
 }
 
 // The coarse-to-fine interpolation of one fine point.
@@ -929,7 +936,7 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
     constexpr int CXX = TX / 2 + 3, CP = CXX * (TY / 2 + 3);      // e plane: matching coarse columns
     constexpr int NN = (NXP + NT - 1) / NT, NR = (TX * TY + NT - 1) / NT, NC = (CP + NT - 1) / NT;
     __shared__ T Xn[4][NXP];          // xn planes f - 3 .. f of phase q (f = z0 - 1 + q)
-    __shared__ T Ec[3][CP];
+    __shared__ T Ec[4][CP];           // three coarse planes alive; a power-of-two ring
 
     int blk = blockIdx.x;
     if (xcd_chunk) blk = (blk & 7) * xcd_chunk + (blk >> 3);
@@ -937,9 +944,9 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
     const int tile = blk % tiles_xy, chunk = blk / tiles_xy;
     const int ox = (tile % tiles_x) * TX, oy = (tile / tiles_x) * TY;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
-    const size_t nxy = (size_t) nx * ny;
+    const unsigned nxy = (unsigned) nx * (unsigned) ny;   // grids hold < 2^32 points
     const int ncx = P.a[0].nc, ncy = P.a[1].nc, ncz = P.a[2].nc;
-    const size_t ncxy = (size_t) ncx * ncy;
+    const unsigned ncxy = (unsigned) ncx * (unsigned) ncy;
     const int z0 = chunk * zchunk, z1 = min(z0 + zchunk, nz);
     const int tid = threadIdx.x;
     const int cxo = (ox >> 1) - 1, cyo = (oy >> 1) - 1;    // coarse origin of the e region
@@ -961,9 +968,11 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
         pr_taps(P.a[0], PX ? ox - 1 + i : (gi >= 0 ? gi : 0), cxo, ntx[q], nwx[q]);
         pr_taps(P.a[1], PY ? oy - 1 + j : (gj >= 0 ? gj : 0), cyo, nty[q], nwy[q]);
     }
-    // residual positions: the tile
+    // residual positions: the tile (positions without a value read at the
+    // in-plane position NXX + 1 and store nothing, so the rows run unbranched)
     xy_coef<T> kr[NR];
     int rcol[NR];
+    bool rvalid[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
         const int t = tid + q * NT;
@@ -971,6 +980,8 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
         const int gi = ox + i, gj = oy + j;
         const bool valid = t < TX * TY && gi < nx && gj < ny;
         kr[q] = make_xy_coef<PER>(A, valid, (i + 1) + NXX * (j + 1), gi, gj);
+        rvalid[q] = valid;
+        if (!valid) kr[q].c = NXX + 1;
         rcol[q] = valid ? gi + nx * gj : 0;
     }
     // e loads: coarse region (cxo + u, cyo + v)
@@ -990,12 +1001,22 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
 
     // coarse planes: physical index on a walled z axis, unwrapped on a periodic one
     auto need = [&](int f) { const int ca = (f + 1) >> 1; return PZ ? ca : min(ca, ncz - 1); };
-    auto cslot = [&](int c) { return (c + 3 * ncz + 3) % 3; };
+    auto cslot = [&](int c) { return c & 3; };
     auto fetch_e = [&](int c, T (&pe)[NC]) {
         int k = c;
-        if (PZ) { k %= ncz; if (k < 0) k += ncz; } else k = min(max(k, 0), ncz - 1);
+        if (PZ) {
+            k = k < 0 ? k + ncz : k;
+            k = k >= ncz ? k - ncz : k;
+            if ((unsigned) k >= (unsigned) ncz) {   // coarse z axes of one or two planes
+                while (k < 0) k += ncz;
+                while (k >= ncz) k -= ncz;
+            }
+        } else {
+            k = min(max(k, 0), ncz - 1);
+        }
+        const unsigned kk = (unsigned) k * ncxy;
 #pragma unroll
-        for (int q = 0; q < NC; ++q) pe[q] = e[(size_t) max(ecol[q], 0) + ncxy * (size_t) k];
+        for (int q = 0; q < NC; ++q) pe[q] = e[(unsigned) max(ecol[q], 0) + kk];
     };
     auto store_e = [&](int c, const T (&pe)[NC]) {
         T* dst = Ec[cslot(c)];
@@ -1006,14 +1027,14 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
         }
     };
     auto fetch_x = [&](int f, T (&px)[NN]) {
-        const size_t k = (size_t) max(axis_global<PZ>(A.a[2], f, 0, 0), 0);
+        const unsigned k = (unsigned) max(axis_global<PZ>(A.a[2], f, 0, 0), 0) * nxy;
 #pragma unroll
-        for (int q = 0; q < NN; ++q) px[q] = x[(size_t) max(ncol[q], 0) + nxy * k];
+        for (int q = 0; q < NN; ++q) px[q] = x[(unsigned) max(ncol[q], 0) + k];
     };
     auto fetch_b = [&](int f, T (&pb)[NR]) {
-        const size_t k = (size_t) min(max(f, z0), z1 - 1);
+        const unsigned k = (unsigned) min(max(f, z0), z1 - 1) * nxy;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) pb[q] = b[(size_t) rcol[q] + nxy * k];
+        for (int q = 0; q < NR; ++q) pb[q] = b[(unsigned) rcol[q] + k];
     };
 
     // e planes for the first xn plane (z0 - 1) and the next one prefetched
@@ -1025,12 +1046,15 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
     store_e(cmax, pe);
     fetch_e(cmax + 1, pe);
 
-    auto sx = [&](int f) { return (f - z0 + 1) & 3; };    // xn ring slot, f >= z0 - 1
-    auto phase = [&](int q, T (&px)[NN], T (&pb)[NR]) {
+    // xn ring slot of plane f >= z0 - 1: (f - z0 + 1) & 3, so plane z0 - 1 + q
+    // of phase q = (multiple of 4) + S sits in slot S, a constant.
+    T px[2][NN], pb[2][NR];
+    auto phase = [&]<int S>(int q) {
+        constexpr int F = S & 1;
         {   // xn on plane f
             const int f = z0 - 1 + q;
             const int gk = axis_global<PZ>(A.a[2], f, 0, 0);
-            T* dst = Xn[sx(f)];
+            T* dst = Xn[S];
             if (f <= z1) {
                 if (gk < 0) {
 #pragma unroll
@@ -1060,8 +1084,8 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
                                 }
                                 acc += wz[bz] * az;
                             }
-                            val = px[i] + acc;
-                            if (nout[i] && f >= z0 && f < z1) xn[(size_t) ncol[i] + nxy * (size_t) gk] = val;
+                            val = px[F][i] + acc;
+                            if (nout[i] && f >= z0 && f < z1) xn[(unsigned) ncol[i] + (unsigned) gk * nxy] = val;
                         }
                         dst[nc_[i]] = val;
                     }
@@ -1073,18 +1097,21 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
             if (z >= z0 && z < z1) {
                 const z_coef<T> kz = make_z_coef<PZ>(A.a[2], z);
                 const bool in = z > 0 && z < nz - 1;
-                const T *Sm = Xn[sx(z - 1)], *Sc = Xn[sx(z)], *Sp = Xn[sx(z + 1)];
+                const T *Sm = Xn[(S - 3) & 3], *Sc = Xn[(S - 2) & 3], *Sp = Xn[(S - 1) & 3];
+                nbr7<T> nb[NR];
+#pragma unroll
+                for (int i = 0; i < NR; ++i) nb[i] = load7<NXX>(Sm, Sc, Sp, kr[i].c);
+                const unsigned kz0 = (unsigned) z * nxy;
 #pragma unroll
                 for (int i = 0; i < NR; ++i) {
-                    if (kr[i].c < 0) continue;
-                    r[(size_t) rcol[i] + nxy * (size_t) z] =
-                        T(-1) * plane_apply<PER, NXX>(A, in, Sm, Sc, Sp, kr[i], kz) + T(1) * pb[i];
+                    const T val = T(-1) * apply7<PER>(A, in, nb[i], kr[i], kz) + T(1) * pb[F][i];
+                    if (rvalid[i]) r[(unsigned) rcol[i] + kz0] = val;
                 }
             }
         }
         // operands two phases on; the e plane the next phase needs
-        fetch_x(z0 + 1 + q, px);
-        fetch_b(z0 - 1 + q, pb);
+        fetch_x(z0 + 1 + q, px[F]);
+        fetch_b(z0 - 1 + q, pb[F]);
         if (need(z0 + q) > cmax) {
             ++cmax;
             store_e(cmax, pe);
@@ -1093,17 +1120,16 @@ prolong_residual_kernel(op3<T> A, transfer3<T> P, const T* __restrict__ e, const
         __syncthreads();
     };
 
-    T pxA[NN], pxB[NN], pbA[NR], pbB[NR];
-    fetch_x(z0 - 1, pxA);
-    fetch_x(z0, pxB);
-    fetch_b(z0 - 3, pbA);
-    fetch_b(z0 - 2, pbB);
+    fetch_x(z0 - 1, px[0]);
+    fetch_x(z0, px[1]);
+    fetch_b(z0 - 3, pb[0]);
+    fetch_b(z0 - 2, pb[1]);
     __syncthreads();
     const int nq = (z1 - z0) + 3;
-    for (int q = 0; q < nq; q += 2) {
-        phase(q, pxA, pbA);
-        if (q + 1 < nq) phase(q + 1, pxB, pbB);
-    }
+    auto group = [&]<int... Ss>(int q0, std::integer_sequence<int, Ss...>) {
+        ((q0 + Ss < nq ? (phase.template operator()<Ss>(q0 + Ss), 0) : 0), ...);
+    };
+    for (int q0 = 0; q0 < nq; q0 += 4) group(q0, std::make_integer_sequence<int, 4>{});
 }
 
 template <typename T>
