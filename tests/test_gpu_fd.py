@@ -86,6 +86,33 @@ def test_chebyshev_apply_matches_oracle(ibm, orc, ctx):
     assert _rel(out.cpu().numpy(), fd.chebyshev_apply(A, a, b, r)) < 1e-12
 
 
+@pytest.mark.parametrize("dom_kind,ref,comp", [
+    ("flat", 4, "w"),          # periodic z of 4 planes: shorter than the march's 12-plane lead
+    ("flat_walls", 4, "v"),
+    ("walls", 4, "w"),
+    ("all_walls", 4, "p"),
+    ("periodic", 8, "u"),
+    ("all_walls", 8, "v"),
+    ("walls", 24, "u"),        # 829K points: the wide tile
+])
+def test_chebyshev_apply_shapes(ibm, orc, ctx, dom_kind, ref, comp):
+    """The one-pass smoother on every wall pattern, on z axes shorter than
+    its pipeline and on grids large enough for the wide tile."""
+    doms = dict(DOMAINS,
+                all_walls=lambda ibm: [ibm.dirichlet(1.0), ibm.dirichlet(0.75), ibm.neumann(1.25)],
+                flat=lambda ibm: [ibm.periodic(1.0), ibm.periodic(0.75), ibm.periodic(0.25)],
+                flat_walls=lambda ibm: [ibm.periodic(1.0), ibm.dirichlet(0.75), ibm.periodic(0.25)])
+    dom = doms[dom_kind](ibm)
+    g = ibm.grid_make(dom, ref, ALIGNS[comp])
+    ax = _axes(orc, dom, ref, ALIGNS[comp])
+    op = ibm.helmholtz(0.4 / g.resolution ** 2)
+    A = fd.operator(ax, *op)
+    a, b = fd.smoother_interval(ax, A)
+    r = np.random.default_rng(ref).standard_normal(g.size)
+    out = ibm.chebyshev_apply(ctx, g, op, a, b, _t(r))
+    assert _rel(out.cpu().numpy(), fd.chebyshev_apply(A, a, b, r)) < 1e-12
+
+
 def _mg_case(ibm, orc, comp, dom_kind, ref=8):
     dom = DOMAINS[dom_kind](ibm)
     g = ibm.grid_make(dom, ref, ALIGNS[comp])
