@@ -134,6 +134,23 @@ def test_vcycle_fp64_matches_oracle(ibm, orc, ctx, comp):
     mg.close()
 
 
+@pytest.mark.parametrize("zlen,ref,comp", [(0.0625, 4, "v"), (0.125, 4, "w"), (0.0625, 8, "w")])
+def test_vcycle_fp64_thin_z_matches_oracle(ibm, orc, ctx, zlen, ref, comp):
+    """V-cycles whose fused transfer kernels see coarse z axes of two planes
+    (64 x 47 x 4 over 32 x 23 x 2, ...): periodic wraps by more than one
+    period, and a coarse-plane ring longer than the coarse axis."""
+    dom = [ibm.periodic(1.0), ibm.dirichlet(0.75), ibm.periodic(zlen)]
+    g = ibm.grid_make(dom, ref, ALIGNS[comp])
+    op = ibm.helmholtz(0.4 / g.resolution ** 2)
+    mg = ibm.Multigrid(ctx, g, op, precision=64)
+    om = fd.Multigrid(lambda r: _axes(orc, dom, r, ALIGNS[comp]), ref, *op, order="x-fastest")
+    assert len(mg.levels()) == len(om.levels) >= 2
+    b = np.random.default_rng(9).standard_normal(g.size)
+    x = mg.vcycle(_t(b)).cpu().numpy()
+    assert _rel(x, om.vcycle(b)) < 1e-11
+    mg.close()
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("dom_kind", ["periodic", "walls"])
 def test_helmholtz_mg_solve(ibm, orc, ctx, precision, dom_kind):
