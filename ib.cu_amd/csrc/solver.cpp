@@ -286,7 +286,15 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
                 IBM_HIP(ctx, hipEventRecord(s->geo_ev, ctx->stream));
                 IBM_HIP(ctx, hipStreamWaitEvent(gc->stream, s->geo_ev, 0));
             }
-            IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 8, false, &packed)));
+            // One tangent operator per launch when concurrent: half as many
+            // long-lived matrix-core blocks resident beside the force /
+            // binning / spreading chain.
+            if (concurrent) {
+                IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 2, false, &packed)));
+                IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 2, 8, false, &packed)));
+            } else {
+                IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 8, false, &packed)));
+            }
         }
         for (int t = 0; t < nt; ++t) {
             auto& T = s->types[t];
