@@ -43,7 +43,7 @@ enum scratch_slot {
     SLOT_FLAGS,
     SLOT_REDUCE,        // reduction partials of the solvers (REDUCE_PARTS + 8 doubles)
     SLOT_RBF_X,         // packed data-site coordinates of an RBF evaluation
-    SLOT_STEP_BASE,     // timestep workspaces; slots above are the driver's
+    SLOT_STEP_BASE,     // timestep workspaces; slots from here on belong to the timestep solver
     SLOT_COUNT = 64
 };
 
