@@ -893,8 +893,9 @@ __device__ inline T prolong_at(const transfer3<T>& P, const T* __restrict__ coar
 // written to its own vector (neighbouring blocks still read x in their
 // halos).  A block owns a TX x TY fine column tile and a chunk of fine planes
 // and marches through z: phase q forms xn on plane z0-1+q over the tile grown
-// by 1 (from x and a 3-plane LDS ring of e over the matching coarse columns;
-// prolong_at's taps, folds and summation order) into a 4-plane ring, and
+// by 1 (from x and an LDS ring of e over the matching coarse columns, three
+// planes alive in four slots; prolong_at's taps, folds and summation order)
+// into a 4-plane ring, and
 // computes the residual on plane z0-3+q from the planes of earlier phases.  Points outside walls
 // hold 0 in the rings (the collapsed operator weights them by 0).
 // ---------------------------------------------------------------------------
