@@ -45,7 +45,7 @@ ibm_status scratch(ibm_ctx* ctx, int slot, size_t bytes, void** out)
 
 prof_scope::prof_scope(ibm_ctx* c, const char* name, double work) : ctx(c)
 {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || !name) return;
     if (ctx->prof_used == ctx->prof.size()) {
         ibm_ctx::prof_record r;
         r.name = name;

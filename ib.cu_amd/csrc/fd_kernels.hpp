@@ -113,6 +113,18 @@ struct boundary_op {
 };
 ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, const double* g, double value, double* y);
 
+// ins::boundary(grid, dim) (ins/boundary.h:16-56): a diagonal operator whose
+// entry at a point is the sum, over the walled axes other than `dim`, of the
+// ghost-cell boundary coefficient (times resolution^order) when the point is
+// on that axis's first (lo) or last (hi) row.  y = alpha B x + beta y.
+struct ins_boundary_op {
+    int n[3];
+    double lo[3], hi[3];   // 0 on periodic axes and on `dim`
+};
+ibm_status launch_ins_boundary(ibm_ctx* ctx, const ins_boundary_op& B, double alpha, const double* x, double beta,
+                               double* y);
+ins_boundary_op make_ins_boundary(const ibm_grid& g, int dim, int order);
+
 // The differential (kind 0) or average (kind 1) of grid `src` along `view`.
 ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst);
 

@@ -7,6 +7,7 @@
 // stencil pass moves only the vectors (16 B per point in FP64, 8 in FP32,
 // +8/+4 per extra input vector) instead of ~100 B of CSR data per row.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <algorithm>
 #include <utility>
 #include <cmath>
@@ -1415,19 +1416,26 @@ __global__ void shift_kernel(size_t n, const double* __restrict__ sum, double sc
     for (; i < n; i += stride) y[i] -= m;
 }
 
+// Maximum that carries a NaN through (fmax would drop it): a blown-up
+// field must not report a finite maximum.
+__device__ inline double nan_max(double m, double v)
+{
+    return (isnan(v) || v > m) ? v : m;
+}
+
 __global__ void __launch_bounds__(REDUCE_THREADS)
 max_abs_partial_kernel(size_t n, const double* __restrict__ x, double* __restrict__ partial)
 {
     __shared__ double part[REDUCE_THREADS / 64];
     double m = 0;
     for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
-        m = fmax(m, fabs(x[i]));
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+        m = nan_max(m, fabs(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = 0;
-        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = fmax(r, part[w]);
+        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = nan_max(r, part[w]);
         partial[blockIdx.x] = r;
     }
 }
@@ -1437,22 +1445,53 @@ finish_max_kernel(int nparts, const double* __restrict__ partial, double* __rest
 {
     __shared__ double part[REDUCE_THREADS / 64];
     double m = 0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, partial[i]);
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = nan_max(m, partial[i]);
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = 0;
-        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = fmax(r, part[w]);
+        for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = nan_max(r, part[w]);
         *out = r;
     }
 }
 
+// sqrt(num / den); 0 for a zero denominator (an empty right-hand side), NaN
+// when either operand is not finite.
 __global__ void sqrt_ratio_kernel(const double* __restrict__ num, const double* __restrict__ den,
                                   double* __restrict__ out)
 {
-    double d = *den;
-    *out = d > 0 ? sqrt(*num / d) : 0.0;
+    const double d = *den, n = *num;
+    if (!isfinite(d) || !isfinite(n)) *out = __longlong_as_double(0x7ff8000000000000ll);
+    else *out = d > 0 ? sqrt(n / d) : 0.0;
+}
+
+// y = alpha * B x + beta * y for the diagonal ins::boundary operator: one
+// thread per point; beta = 1 leaves the interior untouched (only the points
+// on some axis's first or last row are read and written).
+__global__ void ins_boundary_kernel(ins_boundary_op B, double alpha, const double* __restrict__ x, double beta,
+                                    double* __restrict__ y)
+{
+    const size_t n = (size_t) B.n[0] * B.n[1] * B.n[2];
+    for (size_t k = (size_t) blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t) gridDim.x * blockDim.x) {
+        const int i0 = (int) (k % B.n[0]);
+        const size_t q = k / B.n[0];
+        const int i1 = (int) (q % B.n[1]);
+        const int i2 = (int) (q / B.n[1]);
+        const int idx[3] = {i0, i1, i2};
+        double w = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (idx[d] == 0) w += B.lo[d];
+            if (idx[d] == B.n[d] - 1) w += B.hi[d];
+        }
+        if (beta == 1.0) {
+            if (w != 0.0) y[k] += alpha * w * x[k];
+        } else {
+            const double old = beta == 0.0 ? 0.0 : beta * y[k];
+            y[k] = alpha * w * x[k] + old;
+        }
+    }
 }
 
 // y[row] += alpha * scale * g[face] * prod_{d != view} id_d(i_d) on the face
@@ -1840,6 +1879,17 @@ ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den,
     return IBM_OK;
 }
 
+ibm_status launch_ins_boundary(ibm_ctx* ctx, const ins_boundary_op& B, double alpha, const double* x, double beta,
+                               double* y)
+{
+    const size_t n = (size_t) B.n[0] * B.n[1] * B.n[2];
+    if (!n) return IBM_OK;
+    prof_scope prof(ctx, "ins_boundary", (beta == 0.0 ? 16.0 : 24.0) * (double) n);
+    ins_boundary_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(B, alpha, x, beta, y);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
 ibm_status launch_boundary(ibm_ctx* ctx, const boundary_op& B, double alpha, const double* g, double value, double* y)
 {
     int a0 = B.view == 0 ? 1 : 0, a1 = B.view == 2 ? 1 : 2;
@@ -1866,12 +1916,20 @@ size_t coarse_lds_limit(int device)
     return (size_t) v;
 }
 
+// The attribute is process-wide per kernel instantiation, while hierarchies
+// of different sizes may be created in any order: it is only ever raised,
+// so a hierarchy created later never lowers what an earlier one launches with.
 template <typename T>
 ibm_status prepare_coarse_vcycle(ibm_ctx* ctx, size_t lds_bytes)
 {
+    static std::mutex lock;
+    static size_t set_bytes = 0;
+    std::lock_guard<std::mutex> guard(lock);
+    if (lds_bytes <= set_bytes) return IBM_OK;
     hipError_t e = hipFuncSetAttribute((const void*) coarse_vcycle_kernel<T>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);
     if (e != hipSuccess) return hip_fail(ctx, e, "coarse multigrid kernel LDS size");
+    set_bytes = lds_bytes;
     return IBM_OK;
 }
 

@@ -35,6 +35,27 @@ ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<do
     return IBM_OK;
 }
 
+// ins::boundary(grid, dim) (ins/boundary.h:16-56): per axis, the
+// discretization's boundary entries at (0, 0) and (n-1, n-1)
+// (fd/discretization.h:102-108,138-141; zero on periodic axes) scaled by
+// `solid` = 1, kron the zeroth-order identity of the other axes, summed over
+// the axes; the axis `dim` contributes nothing (its comp == dim test).  order
+// 2 scales each axis's entries by its resolution^2, the factor the ghost
+// value carries in the Laplacian's wall rows.
+ins_boundary_op make_ins_boundary(const ibm_grid& g, int dim, int order)
+{
+    ins_boundary_op B;
+    for (int d = 0; d < 3; ++d) {
+        const ibm_axis& a = g.axis[d];
+        B.n[d] = a.points;
+        const double scale = order == 2 ? a.resolution * a.resolution : 1.0;
+        const bool on = a.solid && d != dim;
+        B.lo[d] = on ? scale * a.boundary[0] : 0.0;
+        B.hi[d] = on ? scale * a.boundary[1] : 0.0;
+    }
+    return B;
+}
+
 static bool same_points(const ibm_grid& a, const ibm_grid& b)
 {
     for (int d = 0; d < 3; ++d)
@@ -113,6 +134,17 @@ ibm_status ibm_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int view, int sid
     B.row = side ? a.points - 1 : 0;
     B.scale = std::pow(a.resolution, order) * a.boundary[side];
     return launch_boundary(ctx, B, alpha, values, value, y);
+}
+
+ibm_status ibm_ins_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int dim, int order, double alpha, const double* x,
+                                  double beta, double* y)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    IBM_TRY(check3(ctx, g, 1));
+    if (dim < -1 || dim > 2 || (order != 0 && order != 2))
+        return fail(ctx, IBM_ERR_ARG, "bad ins::boundary arguments (dim %d order %d)", dim, order);
+    if (!x || !y) return fail(ctx, IBM_ERR_ARG, "null vector");
+    return launch_ins_boundary(ctx, make_ins_boundary(*g, dim, order), alpha, x, beta, y);
 }
 
 // ins::divergence (ins/differential.h:46-83): out = sum_i D_i u_i on the

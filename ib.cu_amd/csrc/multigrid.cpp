@@ -300,20 +300,28 @@ static ibm_status solve_loop(ibm_ctx* ctx, ibm_mg* m, const double* b, double* x
     int it = 0;
     double nr = nb;
     if (warm) IBM_TRY(norm2(ctx, m, n, m->r, &nr));
+    // A NaN or infinity in b, x or the operator makes the norms non-finite:
+    // no cycle can fix that, and the comparisons below are written so that a
+    // NaN fails them (nr <= stop is false for a NaN).
+    if (!std::isfinite(nb) || !std::isfinite(nr)) {
+        if (iters) *iters = 0;
+        if (relres) *relres = NAN;
+        return fail(ctx, IBM_ERR_NOT_CONVERGED, "multigrid: non-finite right-hand side or initial residual");
+    }
     const double stop = std::max(rtol * nb, atol);
     if (fused<T>(m)) IBM_TRY(launch_convert<T, double>(ctx, n, m->r, B0.b));
-    while (nr > stop && it < maxit) {
+    while (!(nr <= stop) && it < maxit) {
         IBM_TRY(correct<T>(ctx, m, b, x));
         ++it;
         IBM_TRY(norm2(ctx, m, n, m->r, &nr));
-        if (nr <= stop) break;
+        if (!std::isfinite(nr)) break;
     }
     // A 1 = 0 on the singular operators, so the constant component of x does
     // not change the residuals above: it is projected out once, here.
     if (m->singular) IBM_TRY(launch_remove_mean<double>(ctx, n, x, m->partial, m->scalars + 1));
     if (iters) *iters = it;
-    if (relres) *relres = nb > 0 ? nr / nb : 0.0;
-    if (nr > stop)
+    if (relres) *relres = nb > 0 ? nr / nb : (std::isfinite(nr) ? 0.0 : NAN);
+    if (!(nr <= stop))
         return fail(ctx, IBM_ERR_NOT_CONVERGED, "multigrid: relative residual %.3e after %d cycles", nr / nb, it);
     return IBM_OK;
 }
@@ -618,7 +626,12 @@ ibm_status ibm_chebpcg_solve(ibm_ctx* ctx, const ibm_grid* g, double sigma, doub
     double delta;
     IBM_TRY(dot(r, d, &delta));
     int it = 0;
-    while (nb > 0 && it < maxit && nr > rtol * nb) {
+    if (!std::isfinite(nb)) {
+        if (iters) *iters = 0;
+        if (relres) *relres = NAN;
+        return fail(ctx, IBM_ERR_NOT_CONVERGED, "chebpcg: non-finite right-hand side");
+    }
+    while (nb > 0 && it < maxit && !(nr <= rtol * nb)) {
         IBM_TRY(launch_stencil<double>(ctx, A, 1.0, d, 0.0, nullptr, 0.0, nullptr, q));   // q = A d
         double nu;
         IBM_TRY(dot(d, q, &nu));
@@ -635,10 +648,11 @@ ibm_status ibm_chebpcg_solve(ibm_ctx* ctx, const ibm_grid* g, double sigma, doub
         double rr;
         IBM_TRY(dot(r, r, &rr));
         nr = std::sqrt(rr);
+        if (!std::isfinite(nr)) break;
     }
     if (iters) *iters = it;
     if (relres) *relres = nb > 0 ? nr / nb : 0.0;
-    if (nb > 0 && nr > rtol * nb) return fail(ctx, IBM_ERR_NOT_CONVERGED, "chebpcg: relative residual %.3e after %d iterations", nr / nb, it);
+    if (nb > 0 && !(nr <= rtol * nb)) return fail(ctx, IBM_ERR_NOT_CONVERGED, "chebpcg: relative residual %.3e after %d iterations", nr / nb, it);
     return IBM_OK;
 }
 

@@ -15,7 +15,11 @@
 //        rhs = dt nu (b + L u) + dt I~ (F / rho - H),
 //        (I~ - lambda L) du = rhs,  lambda = dt nu / 2,  u* = u + du,
 //      b the boundary data of moving walls (fd::upper_boundary x values,
-//      test/ib-whole.cc:129-152), solved by multigrid;
+//      test/ib-whole.cc:129-152), solved by multigrid; with
+//      IBM_SOLVER_PSEUDO_PRESSURE the new time level's wall values are
+//      g + dt grad p (the w = u + k grad phi of ins/boundary.h:16-18, p the
+//      previous step's pressure, tangential components only), i.e.
+//        rhs += lambda dt B_i (G_i p),  B_i = ins::boundary(grid_i, i);
 //   6. projection: L_p phi = (D u*) / dt with L_p = D G the Neumann
 //      Laplacian of the cell centres (singular, mean-free solution),
 //      u = u* - dt G phi, p = phi;
@@ -111,6 +115,7 @@ struct ibm_solver {
     int hist[16]{};                    // tracker exponents of the last steps
     int hist_len = 0, hist_next = 0;
     ibm_mg* pois = nullptr;
+    ins_boundary_op Bw[3]{};           // ins::boundary(g[i], i), order 2 (pseudo-pressure walls)
     struct type_t {
         ibm_rbf* ops;
         int nd, ns, nc;
@@ -351,6 +356,15 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         // rhs += dt nu L u
         IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr,
                                                             s->rhs[i])));
+        if (P.flags & IBM_SOLVER_PSEUDO_PRESSURE) {
+            // F[i] (consumed above) <- G_i p; rhs += lambda dt B_i F[i]
+            twopoint_op<double> G;
+            ibm_grid dst;
+            IBM_TRY(step_status(ctx, c, make_twopoint(s->gp, i, 0, &G, &dst)));
+            IBM_TRY(step_status(ctx, c, launch_twopoint<double>(c, G, 1.0, st->p, 0.0, s->F[i])));
+            IBM_TRY(step_status(ctx, c, launch_ins_boundary(c, s->Bw[i], 0.5 * dt * nu * dt, s->F[i], 1.0,
+                                                             s->rhs[i])));
+        }
         extrapolate_kernel<<<blocks(n), 256, 0, c->stream>>>(n, s->warm_hist, s->du[i], s->du_prev[i]);
         IBM_LAUNCH_CHECK(c);
         if (fixed) {
@@ -463,6 +477,7 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         !(P.fscale >= 0))
         return fail(ctx, IBM_ERR_ARG, "bad solver parameters");
     if (P.precision != 64 && P.precision != 32) return fail(ctx, IBM_ERR_ARG, "precision must be 64 or 32");
+    if (P.flags & ~IBM_SOLVER_PSEUDO_PRESSURE) return fail(ctx, IBM_ERR_ARG, "unknown solver flags 0x%x", P.flags);
     auto* s = new ibm_solver;
     s->par = P;
     ibm_status st = ibm_mac_grids(domain, ndims, refinement, s->g);
@@ -481,7 +496,10 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         release(s);
         return fail(ctx, st, "could not build the grids");
     }
-    for (int i = 0; i < 3; ++i) s->nu[i] = (size_t) ibm_grid_size(&s->g[i]);
+    for (int i = 0; i < 3; ++i) {
+        s->nu[i] = (size_t) ibm_grid_size(&s->g[i]);
+        s->Bw[i] = make_ins_boundary(s->g[i], i, 2);
+    }
     s->np = (size_t) ibm_grid_size(&s->gp);
     s->dt = P.timestep;
     if (!(P.fscale > 0)) {

@@ -26,6 +26,7 @@
 #include "context.hpp"
 #include "deltas.hpp"
 
+
 namespace ibm {
 namespace {
 
@@ -178,7 +179,13 @@ __global__ void bin_count_kernel(marker_sets sets, kgrids g, bin_desc bd, int wi
         code[m] = c9;
         if (with_forces) {
             const double* f = sets.f[k];
-            double a = fmax(fabs(f[l]), fmax(fabs(f[n + l]), fabs(f[2 * n + l])));
+            const double f0 = fabs(f[l]), f1 = fabs(f[n + l]), f2 = fabs(f[2 * n + l]);
+            double a = f0 > f1 ? f0 : f1;
+            a = a > f2 ? a : f2;
+            // A non-finite force is recorded as the canonical NaN, whose bit
+            // pattern exceeds every finite and infinite one, so it survives
+            // the integer maxima below and poisons the spread result.
+            if (!(isfinite(f0) && isfinite(f1) && isfinite(f2))) a = __longlong_as_double(0x7ff8000000000000ll);
             fbits = (unsigned long long) __double_as_longlong(a);
         }
     }
@@ -458,8 +465,11 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
             for (int k = 0; k < n; ++k) pos_list[o + k] = b + k;
         }
     __syncthreads();
+    // A NaN or infinite force makes the bound non-finite: every point is then
+    // written as NaN (the accumulation below still runs, on a dummy quantum).
+    const bool poisoned = !(bound_s <= 1.7976931348623157e308);
     int e2 = 0;
-    if (bound_s > 0) frexp(bound_s, &e2);
+    if (!poisoned && bound_s > 0) frexp(bound_s, &e2);
     const double inv_q = ldexp(1.0, 62 - e2);
 
     const int wave = tid >> 6;
@@ -652,7 +662,7 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
     __syncthreads();
 
     // Each grid point belongs to exactly one brick: plain stores.
-    const double q = ldexp(1.0, e2 - 62);
+    const double q = poisoned ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, e2 - 62);
     double* fe[3] = {fe0, fe1, fe2};
     for (int i = tid; i < BRICK_PTS; i += SPREAD_THREADS) {
         int tx = i % BX, ty = (i / BX) % BY, tz = i / (BX * BY);
@@ -685,16 +695,16 @@ interp_kernel(marker_sets sets, kgrids g, int delta,
     const double* ue[3] = {u0, u1, u2};
     for (int c = 0; c < 3; ++c) {
         double w[3][4];
-        int ix[4], oy[4], oz[4];
+        size_t ix[4], oy[4], oz[4];
         const int* pt = g.points[c];
         for (int j = 0; j < 3; ++j) {
             stencil_axis st = axis_stencil(g.a[c][j], p[j]);
             delta4(delta, st.d, w[j]);
             for (int o = 0; o < 4; ++o) {
-                int idx = axis_interp_index(g.a[c][j], st.base + o - 1);
+                size_t idx = (size_t) axis_interp_index(g.a[c][j], st.base + o - 1);
                 if (j == 0) ix[o] = idx;
-                else if (j == 1) oy[o] = idx * pt[0];
-                else oz[o] = idx * pt[0] * pt[1];
+                else if (j == 1) oy[o] = idx * (size_t) pt[0];
+                else oz[o] = idx * (size_t) pt[0] * (size_t) pt[1];
             }
         }
         const double* u = ue[c];
@@ -742,6 +752,8 @@ ibm_status make_kgrids(ibm_ctx* ctx, const ibm_grid* grids, kgrids* out)
                 return fail(ctx, IBM_ERR_BAD_GRID, "empty grid axis");
         }
         out->res3[c] = r;
+        if ((long long) grids[c].axis[0].points * grids[c].axis[1].points * grids[c].axis[2].points > INT_MAX)
+            return fail(ctx, IBM_ERR_SIZE, "spread/interpolate grids hold at most %d points", INT_MAX);
     }
     // Along each dimension the component grids may use at most two distinct
     // discretisations (MAC: the face-aligned one and the centred one).
@@ -809,7 +821,7 @@ ibm_status make_sets(ibm_ctx* ctx, int nsets, const int* n, const double* const*
 bin_desc make_bins(const ibm_grid* grids)
 {
     bin_desc bd;
-    int nbins = 1;
+    long long nbins = 1;
     for (int j = 0; j < 3; ++j) {
         int fc = -1;
         for (int c = 0; c < 3; ++c)
@@ -834,64 +846,78 @@ bin_desc make_bins(const ibm_grid* grids)
         bd.hi[j] = hi;
         nbins *= bd.nb[j];
     }
-    bd.nbins = nbins;
+    // Grids are limited to INT_MAX points (make_kgrids), so this only fails
+    // with walls adding a bin row; callers check nbins > 0.
+    bd.nbins = nbins <= INT_MAX - 1 ? (int) nbins : -1;
     return bd;
 }
 
 int blocks_for(long long n, int threads) { return (int) ((n + threads - 1) / threads); }
 
-// Counting sort of all markers into bins; leaves `start` (nbins+1), `perm`
-// and sorted positions (and forces) in scratch.
+// Device buffers of one binning pass.
+struct bin_work {
+    int *count, *start, *key, *rank, *perm, *code;
+    long long* tmp;
+    double* sorted;          // 6 * total doubles: positions, then forces (SoA)
+    bin_stats* stats;
+    void* scan_tmp;
+    size_t scan_bytes;
+};
+
+// The binning buffers in the context's scratch slots (grown on demand).
+ibm_status ctx_bin_work(ibm_ctx* ctx, int total, int nbins, bin_work* w)
+{
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_BIN_COUNT, sizeof(int) * (size_t) (nbins + 1), &p)); w->count = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_BIN_START, sizeof(int) * (size_t) (nbins + 1), &p)); w->start = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_KEY, sizeof(int) * (size_t) total, &p)); w->key = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_RANK, sizeof(int) * (size_t) total, &p)); w->rank = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_TMP, sizeof(long long) * (size_t) total, &p)); w->tmp = (long long*) p;
+    IBM_TRY(scratch(ctx, SLOT_PERM, sizeof(int) * (size_t) total, &p)); w->perm = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_CODE, sizeof(int) * (size_t) total, &p)); w->code = (int*) p;
+    IBM_TRY(scratch(ctx, SLOT_SORTED, sizeof(double) * 6 * (size_t) total, &p)); w->sorted = (double*) p;
+    IBM_TRY(scratch(ctx, SLOT_FLAGS, sizeof(bin_stats), &p)); w->stats = (bin_stats*) p;
+    size_t tbytes = 0;
+    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, w->count, w->start, nbins + 1, ctx->stream));
+    IBM_TRY(scratch(ctx, SLOT_SCAN_TMP, tbytes, &p));
+    w->scan_tmp = p;
+    w->scan_bytes = tbytes;
+    return IBM_OK;
+}
+
+// Counting sort of all markers into bins: leaves the bin starts (nbins+1),
+// the permutation and sorted positions (and forces) in `w`.  Every buffer
+// that must start at zero is cleared here, by kernels (no memset), so the
+// pass can be recorded in a HIP graph and run on reused memory.
 ibm_status bin_markers(ibm_ctx* ctx, const marker_sets& sets, const kgrids& kg, const bin_desc& bd,
-                       int with_forces, int** start_out, int** perm_out, double** sorted_out,
-                       bin_stats** stats_out)
+                       int with_forces, const bin_work& w, bool timed)
 {
     const int total = sets.total;
     const int nbins = bd.nbins;
-    int *count, *start, *key, *rank, *perm, *code;
-    long long* tmp;
-    double* sorted;
-    void* p;
-    IBM_TRY(scratch(ctx, SLOT_BIN_COUNT, sizeof(int) * (size_t) (nbins + 1), &p)); count = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_BIN_START, sizeof(int) * (size_t) (nbins + 1), &p)); start = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_KEY, sizeof(int) * (size_t) total, &p)); key = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_RANK, sizeof(int) * (size_t) total, &p)); rank = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_TMP, sizeof(long long) * (size_t) total, &p)); tmp = (long long*) p;
-    IBM_TRY(scratch(ctx, SLOT_PERM, sizeof(int) * (size_t) total, &p)); perm = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_CODE, sizeof(int) * (size_t) total, &p)); code = (int*) p;
-    IBM_TRY(scratch(ctx, SLOT_SORTED, sizeof(double) * 6 * (size_t) total, &p)); sorted = (double*) p;
-    bin_stats* stats;
-    IBM_TRY(scratch(ctx, SLOT_FLAGS, sizeof(bin_stats), &p)); stats = (bin_stats*) p;
-
-    prof_scope prof(ctx, "bin");
+    prof_scope prof(ctx, timed ? "bin" : nullptr);
     {
         size_t words = (size_t) nbins + 1;
         zero_words_kernel<<<blocks_for((long) std::min<size_t>(words, 1u << 22), 256), 256, 0, ctx->stream>>>(
-            (unsigned int*) count, words);
+            (unsigned int*) w.count, words);
         IBM_LAUNCH_CHECK(ctx);
-        zero_words_kernel<<<1, 256, 0, ctx->stream>>>((unsigned int*) stats, sizeof(bin_stats) / 4);
+        zero_words_kernel<<<1, 256, 0, ctx->stream>>>((unsigned int*) w.stats, sizeof(bin_stats) / 4);
         IBM_LAUNCH_CHECK(ctx);
     }
     if (total > 0) {
-        bin_count_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, with_forces, key, rank,
-                                                                           code, count, stats);
+        bin_count_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, with_forces, w.key, w.rank,
+                                                                           w.code, w.count, w.stats);
         IBM_LAUNCH_CHECK(ctx);
     }
-    size_t tbytes = 0;
-    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, count, start, nbins + 1, ctx->stream));
-    IBM_TRY(scratch(ctx, SLOT_SCAN_TMP, tbytes, &p));
-    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(p, tbytes, count, start, nbins + 1, ctx->stream));
+    size_t tbytes = w.scan_bytes;
+    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tbytes, w.count, w.start, nbins + 1, ctx->stream));
     if (total > 0) {
-        bin_scatter_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(total, key, rank, code, start, tmp);
+        bin_scatter_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(total, w.key, w.rank, w.code, w.start,
+                                                                             w.tmp);
         IBM_LAUNCH_CHECK(ctx);
-        bin_canon_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, with_forces, key, start, tmp, perm,
-                                                                           sorted, stats);
+        bin_canon_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(sets, with_forces, w.key, w.start, w.tmp,
+                                                                           w.perm, w.sorted, w.stats);
         IBM_LAUNCH_CHECK(ctx);
     }
-    *start_out = start;
-    *perm_out = perm;
-    *sorted_out = sorted;
-    *stats_out = stats;
     return IBM_OK;
 }
 
@@ -916,13 +942,13 @@ ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
             if (!kg.a[c][j].solid && kg.a[c][j].cells < 4)
                 return fail(ctx, IBM_ERR_BAD_GRID, "periodic dimension needs at least 4 cells for a 4-point stencil");
     bin_desc bd = make_bins(grids);
+    if (bd.nbins <= 0) return fail(ctx, IBM_ERR_SIZE, "too many grid bins");
     for (int j = 0; j < 3; ++j)
         if (bd.hi[j] - bd.lo[j] > 1 || bd.hi[j] < bd.lo[j])
             return fail(ctx, IBM_ERR_BAD_GRID, "component grids are not a staggering of one cell grid");
-    int *start, *perm;
-    double* sorted;
-    bin_stats* stats;
-    IBM_TRY(bin_markers(ctx, sets, kg, bd, 1, &start, &perm, &sorted, &stats));
+    bin_work w;
+    IBM_TRY(ctx_bin_work(ctx, sets.total, bd.nbins, &w));
+    IBM_TRY(bin_markers(ctx, sets, kg, bd, 1, w, true));
     int maxp[3] = {0, 0, 0};
     for (int c = 0; c < 3; ++c)
         for (int j = 0; j < 3; ++j) maxp[j] = kg.points[c][j] > maxp[j] ? kg.points[c][j] : maxp[j];
@@ -947,7 +973,8 @@ ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
     prof_scope prof(ctx, "spread_brick", 48.0 * sets.total + 8.0 * grid_points);
     auto launch = [&](auto kernel) {
         kernel<<<(int) nbricks, SPREAD_THREADS, 0, ctx->stream>>>(
-            kg, bd, delta, sets.total, start, sorted, stats, fe[0], fe[1], fe[2], bx, by, (int) nbricks, xcd_chunk);
+            kg, bd, delta, sets.total, w.start, w.sorted, w.stats, fe[0], fe[1], fe[2], bx, by, (int) nbricks,
+            xcd_chunk);
     };
     if (mac && fast) launch(spread_brick_kernel<true, true>);
     else if (mac) launch(spread_brick_kernel<true, false>);

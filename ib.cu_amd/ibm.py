@@ -134,6 +134,7 @@ _SIGS = {
     "ibm_axpby": (_I, [_VP, ctypes.c_int64, _D, _VP, _D, _VP]),
     "ibm_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _I, _D, _VP, _D, _VP]),
     "ibm_mg_solve_fixed": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "ibm_ins_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _D, _VP, _D, _VP]),
     "ibm_mg_solve_atol": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(_I), _PD]),
     "ibm_solver_create": (_I, [_VP, ctypes.POINTER(ibm_dimension), _I, _I, _VP, _I, _VP, ctypes.POINTER(_VP)]),
     "ibm_solver_destroy": (_I, [_VP]),
@@ -415,9 +416,16 @@ def chebyshev_apply(ctx, grid, op, a, b, r):
 
 
 def dot(ctx, x, y):
+    """linalg::dot (linalg/blas.h:140-155): FP64, fixed summation order."""
     out = ctypes.c_double()
     _check(lib().ibm_dot(ctx.handle, x.numel(), _ptr(x), _ptr(y), ctypes.byref(out)), ctx)
     return out.value
+
+
+def axpby(ctx, a, x, b, y):
+    """y = a x + b y in place (linalg::axpy / scal, linalg/blas.h:63-98)."""
+    _check(lib().ibm_axpby(ctx.handle, y.numel(), a, _ptr(x), b, _ptr(y)), ctx)
+    return y
 
 
 class Multigrid:
@@ -606,6 +614,14 @@ def surface_geometry(ctx, ns, nc, y, t1, t2, w):
     return normal, sigma, area, volume
 
 
+def ins_boundary_apply(ctx, grid, dim, x, y, alpha=1.0, beta=1.0, order=0):
+    """y = alpha * ins::boundary(grid, dim) x + beta * y (order 2: scaled by
+    resolution^2 per axis)."""
+    _check(lib().ibm_ins_boundary_apply(ctx.handle, ctypes.byref(grid), dim, order, alpha, _ptr(x), beta,
+                                        _ptr(y)), ctx)
+    return y
+
+
 def boundary_apply(ctx, grid, view, side, y, value=0.0, values=None, alpha=1.0, order=2):
     """y += alpha * fd::{lower,upper}_boundary(grid, view, order) g (side 0 / 1)."""
     _check(lib().ibm_boundary_apply(ctx.handle, ctypes.byref(grid), view, side, order, alpha,
@@ -617,11 +633,12 @@ def boundary_apply(ctx, grid, view, side, y, value=0.0, values=None, alpha=1.0, 
 # Immersed-boundary timestep
 # ----------------------------------------------------------------------
 MAX_CELL_TYPES = 8
+PSEUDO_PRESSURE = 1       # IBM_SOLVER_PSEUDO_PRESSURE
 
 
 class ibm_solver_params(ctypes.Structure):
     _fields_ = [("timestep", _D), ("diffusivity", _D), ("density", _D), ("tolerance", _D),
-                ("delta", _I), ("precision", _I), ("max_iterations", _I), ("reserved", _I), ("fscale", _D)]
+                ("delta", _I), ("precision", _I), ("max_iterations", _I), ("flags", _I), ("fscale", _D)]
 
 
 class ibm_cell_type(ctypes.Structure):
@@ -692,11 +709,14 @@ class Solver:
     tether (CUDA tensor, ns x 3 cells), stiffness."""
 
     def __init__(self, ctx, domain, refinement, timestep, diffusivity, density=1.0, tolerance=1e-10,
-                 delta="cosine", precision=64, max_iterations=100, cell_types=(), fscale=0.0):
-        """With fscale > 0, `timestep` is the tracker's smallest step k_min."""
+                 delta="cosine", precision=64, max_iterations=100, cell_types=(), fscale=0.0,
+                 pseudo_pressure=False):
+        """With fscale > 0, `timestep` is the tracker's smallest step k_min;
+        pseudo_pressure selects the u* = g + dt grad p wall condition."""
         self.ctx = ctx
+        flags = PSEUDO_PRESSURE if pseudo_pressure else 0
         par = ibm_solver_params(timestep, diffusivity, density, tolerance, DELTA.get(delta, delta), precision,
-                                max_iterations, 0, fscale)
+                                max_iterations, flags, fscale)
         n = len(cell_types)
         arr = (ibm_cell_type * max(n, 1))()
         self._keep = []

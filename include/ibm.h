@@ -336,6 +336,17 @@ ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, doubl
 ibm_status ibm_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int view, int side, int order, double alpha,
                               const double* values, double value, double* y);
 
+/* y = alpha * B x + beta * y with B = ins::boundary(grid, dim)
+ * (ins/boundary.h:16-56): the operator that carries the pseudo-pressure wall
+ * condition w = u + k grad(phi) of the intermediate velocity.  B is diagonal:
+ * on each walled axis j != dim it holds the ghost-cell boundary coefficient
+ * (fd/discretization.h:56) on the axis's first and last rows, kron the
+ * zeroth-order identity of the other axes, summed over those axes.  order 0
+ * is the reference's matrix; order 2 scales axis j's entries by its
+ * resolution^2, the Laplacian's factor.  dim = -1 excludes no axis. */
+ibm_status ibm_ins_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int dim, int order, double alpha,
+                                  const double* x, double beta, double* y);
+
 /* ibm_mg_solve stopping at ||b - A x|| <= max(rtol ||b||, atol): for right-hand
  * sides whose norm is itself rounding noise (the divergence of a field that
  * is already divergence-free), atol carries the scale of the problem.  With
@@ -369,9 +380,17 @@ typedef struct {
     int delta;           /* ibm_delta of spread and interpolate */
     int precision;       /* multigrid V-cycles in 64 or 32 bit; defect correction is FP64 */
     int max_iterations;  /* per solve */
-    int reserved;
+    int flags;           /* IBM_SOLVER_* bits */
     double fscale;       /* > 0: step size from ins::tracker's force bound (ins/tracker.h:62-75) */
 } ibm_solver_params;
+
+/* Wall condition of the intermediate velocity u* in the Helmholtz solves:
+ * u* = g + dt grad(p_prev) on the walls tangential to each component (the
+ * w = u + k grad(phi) of ins/boundary.h:16-18, with the previous step's
+ * pressure), added to the right-hand side through ins::boundary.  After the
+ * projection the tangential wall slip is then dt |grad(p_prev - p)| instead
+ * of dt |grad p|.  Without the flag u* takes the wall data g. */
+#define IBM_SOLVER_PSEUDO_PRESSURE 1
 
 typedef struct {
     ibm_rbf* ops;            /* 3 operators data -> sample: position, d/dtheta, d/dphi */

@@ -74,24 +74,99 @@ inline void check(ibm_status s)
 
 } // namespace ibm
 
+// Units (units.h:16-276): every quantity is a double in library units,
+// 1 length unit = 10 um (1 cm = 1000), 1 mass unit = 1 ug (1 g = 10^6),
+// 1 time unit = 1 ms (1 s = 1000).  The reference's unit<d, m, t> wrapper
+// converts implicitly to and from double; here the types are doubles.
+namespace units {
+
+template <long dist, long mass, long time>
+using unit = double;
+using scalar = unit<0, 0, 0>;
+using length = unit<1, 0, 0>;
+using mass = unit<0, 1, 0>;
+using time = unit<0, 0, 1>;
+using force = unit<1, 1, -2>;
+using velocity = unit<1, 0, -1>;
+using acceleration = unit<1, 0, -2>;
+using hertz = unit<0, 0, -1>;
+using viscosity = unit<-1, 1, -1>;
+using volume = unit<3, 0, 0>;
+using area = unit<2, 0, 0>;
+using diffusivity = unit<2, 0, -1>;
+using density = unit<-3, 1, 0>;
+using work = unit<2, 1, -2>;
+using energy = work;
+
+// SI units in library units (m = 100 cm of 1000 units; N = kg m / s^2;
+// P = 0.1 kg / (m s); J = N m).
+inline constexpr double m = 1e5;
+inline constexpr double g = 1e6;
+inline constexpr double s = 1e3;
+inline constexpr double kg = 1e9;
+inline constexpr double N = 1e8;
+inline constexpr double L = 1e12;
+inline constexpr double mL = 1e9;
+inline constexpr double P = 1.0;
+inline constexpr double J = 1e13;
+
+namespace literals {
+#define IBM_UNIT_LITERAL(suffix, factor)                                                   \
+    constexpr double operator""_##suffix(long double v) { return (double) (v * (long double) (factor)); } \
+    constexpr double operator""_##suffix(unsigned long long v) { return (double) v * (factor); }
+IBM_UNIT_LITERAL(m, 1e5)
+IBM_UNIT_LITERAL(kg, 1e9)
+IBM_UNIT_LITERAL(s, 1e3)
+IBM_UNIT_LITERAL(N, 1e8)
+IBM_UNIT_LITERAL(cm, 1e3)
+IBM_UNIT_LITERAL(mm, 1e2)
+IBM_UNIT_LITERAL(um, 0.1)
+IBM_UNIT_LITERAL(nm, 1e-4)
+IBM_UNIT_LITERAL(L, 1e12)
+IBM_UNIT_LITERAL(mL, 1e9)
+IBM_UNIT_LITERAL(g, 1e6)
+IBM_UNIT_LITERAL(ms, 1.0)
+IBM_UNIT_LITERAL(us, 1e-3)
+IBM_UNIT_LITERAL(ns, 1e-6)
+IBM_UNIT_LITERAL(dyn, 1e3)
+IBM_UNIT_LITERAL(P, 1.0)
+IBM_UNIT_LITERAL(cP, 1e-2)
+IBM_UNIT_LITERAL(J, 1e13)
+IBM_UNIT_LITERAL(erg, 1e6)
+#undef IBM_UNIT_LITERAL
+} // namespace literals
+} // namespace units
+
+// As units.h does, the literals are visible everywhere the header is.
+using namespace units::literals;
+
 namespace fd {
 
+// Boundary kinds (fd/boundary.h:31-42): `periodic`, `dirichlet` and
+// `neumann` are objects, usable as `fd::boundary::periodic` or, like a
+// constructor call, `fd::boundary::periodic()`; robin(a, b) describes
+// a u + b du/dn = gamma.
 namespace boundary {
-struct periodic {};
 struct robin {
-    double a, b;   // a u + b du/dn = 0
+    double a, b;
+    constexpr robin(double alpha, double beta) : a(alpha), b(beta) {}
 };
-struct dirichlet : robin {
-    constexpr dirichlet() : robin{1.0, 0.0} {}
-};
-struct neumann : robin {
-    constexpr neumann() : robin{0.0, 1.0} {}
-};
+inline constexpr struct periodic_t {
+    constexpr periodic_t operator()() const { return *this; }
+} periodic{};
+inline constexpr struct dirichlet_t : robin {
+    constexpr dirichlet_t() : robin(1.0, 0.0) {}
+    constexpr dirichlet_t operator()() const { return *this; }
+} dirichlet{};
+inline constexpr struct neumann_t : robin {
+    constexpr neumann_t() : robin(0.0, 1.0) {}
+    constexpr neumann_t operator()() const { return *this; }
+} neumann{};
 } // namespace boundary
 
 struct dimension {
     ibm_dimension d;
-    constexpr dimension(double length, boundary::periodic) : d{length, 0, 0, {0, 0}, {0, 0}} {}
+    constexpr dimension(double length, boundary::periodic_t) : d{length, 0, 0, {0, 0}, {0, 0}} {}
     constexpr dimension(double length, boundary::robin both) : d{length, 1, 0, {both.a, both.b}, {both.a, both.b}} {}
     constexpr dimension(double length, boundary::robin lower, boundary::robin upper) :
         d{length, 1, 0, {lower.a, lower.b}, {upper.a, upper.b}} {}
@@ -100,7 +175,7 @@ struct dimension {
 
 class domain {
 public:
-    domain(dimension x, dimension y, dimension z) : dims_{x.d, y.d, z.d} {}
+    constexpr domain(dimension x, dimension y, dimension z) : dims_{x.d, y.d, z.d} {}
     double unit() const
     {
         double u = 0;
@@ -238,38 +313,45 @@ struct bspline<3> {
 };
 } // namespace delta
 
-// ib::spread{fd::mac{r}, domain, delta}: forces at n markers (SoA x, f) onto
-// the three component grids (ib/novel.h:197-250).
-template <typename Delta>
+// ib::spread{fd::mac{r}, domain[, delta]}: forces at n markers (SoA x, f)
+// onto the three component grids (ib/novel.h:197-250).  Constructible in
+// constant expressions, as the reference's is; the component grids are
+// derived (on the host) at each call.  The delta defaults to the cosine
+// kernel.
+template <typename Delta = delta::cosine>
 class spread {
 public:
-    spread(const fd::mac& m, const fd::domain& dom, Delta = {}) : grids_(fd::components(m, dom)) {}
+    constexpr spread(const fd::mac& m, const fd::domain& dom, Delta = {}) : mac_(m), dom_(dom) {}
     void operator()(int n, const double* x, const double* f, std::array<double*, 3> fe,
                     ibm::context& ctx = ibm::context::current()) const
     {
-        ibm_grid g[3] = {grids_[0].g, grids_[1].g, grids_[2].g};
+        const auto c = grids();
+        ibm_grid g[3] = {c[0].g, c[1].g, c[2].g};
         ctx.check(ibm_spread(ctx.get(), g, Delta::kind, n, x, f, fe.data()));
     }
-    const std::array<fd::grid, 3>& grids() const { return grids_; }
+    std::array<fd::grid, 3> grids() const { return fd::components(mac_, dom_); }
 
 private:
-    std::array<fd::grid, 3> grids_;
+    fd::mac mac_;
+    fd::domain dom_;
 };
 
-// ib::interpolate{fd::mac{r}, domain, delta} (ib/interpolate.h:92-138).
-template <typename Delta>
+// ib::interpolate{fd::mac{r}, domain[, delta]} (ib/interpolate.h:92-138).
+template <typename Delta = delta::cosine>
 class interpolate {
 public:
-    interpolate(const fd::mac& m, const fd::domain& dom, Delta = {}) : grids_(fd::components(m, dom)) {}
+    constexpr interpolate(const fd::mac& m, const fd::domain& dom, Delta = {}) : mac_(m), dom_(dom) {}
     void operator()(int n, const double* x, std::array<const double*, 3> ue, double* ul,
                     ibm::context& ctx = ibm::context::current()) const
     {
-        ibm_grid g[3] = {grids_[0].g, grids_[1].g, grids_[2].g};
+        const auto c = fd::components(mac_, dom_);
+        ibm_grid g[3] = {c[0].g, c[1].g, c[2].g};
         ctx.check(ibm_interpolate(ctx.get(), g, Delta::kind, n, x, ue.data(), ul));
     }
 
 private:
-    std::array<fd::grid, 3> grids_;
+    fd::mac mac_;
+    fd::domain dom_;
 };
 
 } // namespace ib

@@ -6,6 +6,7 @@ Restates, with scipy.sparse CSR matrices in FP64:
   1-D Laplacian          fd/laplacian.h:18-49 (3-point rows + boundary entries)
   1-D identity           fd/identity.h:25-51 (second-order corner corrections)
   boundary entries       fd/discretization.h:16-45,94-108 (single_entry, interior)
+  ins::boundary          ins/boundary.h:16-56 (pseudo-pressure wall operator)
   d-D assembly           fd/combine.h:10-49 (Kronecker sum/product, x fastest)
   1-D differential/avg   fd/differential.h:44-82, fd/average.h:44-81
   Gershgorin             algo/gershgorin.h:14-50
@@ -86,6 +87,31 @@ def boundary_op(axes, view, lower, order):
     for j, a in enumerate(axes):
         mats.append(a.resolution ** order * boundary1(a, lower) if j == view else id1(a, order))
     return kron_product(mats)
+
+
+def ins_boundary(axes, dim, order=0):
+    """ins::boundary(grid, dim) (ins/boundary.h:16-56): per axis the
+    discretization's boundary entries at (0, 0) and (n-1, n-1)
+    (fd/discretization.h:102-108, 138-141) with scale `solid`, none on the
+    axis `dim` (comp == dim) or on periodic axes; folded over the reversed
+    components as kron(lb, ri) + kron(li, rb) with the zeroth-order identity,
+    so x is fastest.  order 2 scales each axis's entries by resolution^2."""
+    parts = []
+    for j, a in enumerate(axes):
+        rows = a.points
+        if j == dim or not a.solid:
+            b = sp.csr_matrix((rows, rows))
+        else:
+            b = (single_entry(rows, rows, 1.0, False, False, a.boundary[0])
+                 + single_entry(rows, rows, 1.0, True, True, a.boundary[1]))
+            if order == 2:
+                b = b * a.resolution ** 2
+        parts.append((sp.csr_matrix(b), sp.identity(rows, format="csr")))
+    seq = list(reversed(parts))
+    lb, li = seq[0]
+    for rb, ri in seq[1:]:
+        lb, li = sp.kron(lb, ri, format="csr") + sp.kron(li, rb, format="csr"), sp.kron(li, ri, format="csr")
+    return sp.csr_matrix(lb)
 
 
 def shifted_axis(axis):

@@ -8,6 +8,9 @@ of the reference, so this follows the library's documented scheme:
   H = advection(u) (oracle/fd.py),
   rhs = dt I~ (F / rho - H) + dt nu (b + L u),  (I~ - dt nu / 2 L) du = rhs
   (ins/diffusion.h:46-56, sparse direct solves),
+  optionally (pseudo_pressure) rhs += dt nu / 2 dt B_i (G_i p) with
+  B_i = ins::boundary(grid_i, i) at order 2 and p the previous pressure
+  (the w = u + k grad phi wall data of ins/boundary.h:16-18),
   L_p phi = D u* / dt on the Neumann cell-centred grid (mean-free; pinned
   phi_0 = 0 then the mean removed), u = u* - dt G phi, p = phi,
   X += dt interpolate(u, X).
@@ -23,11 +26,12 @@ CENTRE = [0.5, 0.5, 0.5]
 class Step:
     """spec: [(length, solid, lower, upper)] x 3 as for orc.grid_axes."""
 
-    def __init__(self, orc, fd, rbf, spec, refinement, dt, nu, rho, delta, types):
+    def __init__(self, orc, fd, rbf, spec, refinement, dt, nu, rho, delta, types, pseudo_pressure=False):
         self.orc, self.fd, self.rbf = orc, fd, rbf
         self.spec, self.ref = spec, refinement
         self.dt, self.nu, self.rho, self.delta = dt, nu, rho, delta
         self.types = types                  # dicts: E, Et, Ep, nd, ns, nc, w, ref (3, nc, ns), k
+        self.pseudo_pressure = pseudo_pressure
         pspec = [(L, s, (0.0, 1.0) if s else lo, (0.0, 1.0) if s else up) for (L, s, lo, up) in spec]
         self.pspec = pspec
         self.axes_of = lambda align: self._axes(spec, align)
@@ -36,6 +40,7 @@ class Step:
         self.uaxes = [self.axes_of(a) for a in MAC]
         self.paxes = self.paxes_of(CENTRE)
         lam = dt * nu / 2
+        self.Bw = [fd.ins_boundary(ax, i, 2) for i, ax in enumerate(self.uaxes)]
         self.I = [fd.identity(ax, 2) for ax in self.uaxes]
         self.L = [fd.laplacian(ax) for ax in self.uaxes]
         self.helm = [spla.splu(sp.csc_matrix(fd.operator(ax, 1.0, -lam))) for ax in self.uaxes]
@@ -66,6 +71,7 @@ class Step:
             Fi = orc.spread(self.mac, self.delta, y.reshape(-1), f.reshape(-1))
             F = [a + b for a, b in zip(F, Fi)]
         H = fd.advection(self.axes_of, MAC, u)
+        Gp = fd.gradient(self.paxes_of, CENTRE, p) if self.pseudo_pressure else None
         us = []
         for i in range(3):
             acc = F[i] / self.rho - H[i]
@@ -74,6 +80,8 @@ class Step:
             if boundary is not None:
                 lu = lu + boundary[i]
             rhs = rhs + dt * nu * lu
+            if Gp is not None:
+                rhs = rhs + (dt * nu / 2) * dt * (self.Bw[i] @ Gp[i])
             us.append(u[i] + self.helm[i].solve(rhs))
         div = fd.divergence(self.axes_of, MAC, us) / dt
         div -= div.mean()

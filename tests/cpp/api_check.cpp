@@ -60,9 +60,9 @@ int main(int argc, char** argv)
         double* df = upload(f);
 
         constexpr double um = 0.1;
-        fd::domain domain{fd::dimension{16 * um, fd::boundary::periodic{}},
-                          fd::dimension{16 * um, fd::boundary::periodic{}},
-                          fd::dimension{16 * um, fd::boundary::periodic{}}};
+        fd::domain domain{fd::dimension{16 * um, fd::boundary::periodic},
+                          fd::dimension{16 * um, fd::boundary::periodic},
+                          fd::dimension{16 * um, fd::boundary::periodic}};
         constexpr fd::mac mac{32};
         auto grids = fd::components(mac, domain);
         std::array<double*, 3> fe;

@@ -12,6 +12,18 @@ pinned by what the mathematics fixes:
   chebyshev     1 - z p(z) = T4(s(z)) / T4(s(0)) (algo/chebyshev.h:9-41)
   multigrid     V-cycle contracts the error; solve == sparse direct solve;
                 reference transfer order == x-fastest order on centred grids
+  transfers     interp1 reproduces linear fields on interior rows of periodic,
+                Dirichlet and Neumann axes, face-aligned and centred; Dirichlet
+                wall rows reproduce fields vanishing at that wall, Neumann wall
+                rows reproduce constants; restrict1 = interp1^T / 2 except the
+                reference's Neumann face-aligned wall fold
+  smoother      interval [a + (b - a) prop, b] with prop and the Gershgorin
+                interval in closed form (cube: 1/8; 32 x 64 x 32 box: 1/16)
+  boundary_op   Couette identity L u_lin + B_lo g_lo + B_hi g_hi = 0 for linear
+                wall-to-wall profiles on centred and face-aligned walled axes
+  transfer order  on staggered grids the reference's kron order is not a
+                linear interpolation along x and z, the per-axis order is;
+                both V-cycles contract alike and stay within 1e-3 of each other
   chebpcg       == sparse direct solve
 """
 import math
@@ -195,3 +207,140 @@ def test_chebpcg_matches_direct(orc):
     x, it = fd.chebpcg(A, b, 1e-12)
     exact = spla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - exact) < 1e-10 * np.linalg.norm(exact)
+
+
+# ----------------------------------------------------------------------
+# Multigrid transfers (solvers/mg/interpolation.h:24-120)
+# ----------------------------------------------------------------------
+def _axis(orc, spec, ref, align):
+    return orc.grid_axes([spec] * 3, ref, [align] * 3)[0]
+
+
+def _coords1(a):
+    return (np.arange(a.points) + a.shift) / a.resolution
+
+
+@pytest.mark.parametrize("kind", ["periodic", "dirichlet", "neumann"])
+@pytest.mark.parametrize("align", [0.0, 0.5])
+def test_interpolation_reproduces_linear_fields(orc, kind, align):
+    spec = {"periodic": P(1.0), "dirichlet": D(1.0), "neumann": N(1.0)}[kind]
+    f, c = _axis(orc, spec, 16, align), _axis(orc, spec, 8, align)
+    Pm = fd.interp1(f, c).toarray()
+    xf, xc = _coords1(f), _coords1(c)
+    # interior rows: the 4-tap weights 1 - |dx|/2 interpolate linearly
+    err = Pm @ (2 * xc + 1) - (2 * xf + 1)
+    assert np.abs(err[1:-1]).max() < 1e-13
+    if kind == "dirichlet":
+        # the ghost value -u0 (centred) / the dropped wall point (face) is the
+        # homogeneous wall: fields vanishing at a wall are exact on its row
+        assert abs((Pm @ xc - xf)[0]) < 1e-13
+        assert abs((Pm @ (1 - xc) - (1 - xf))[-1]) < 1e-13
+        assert abs((Pm @ np.ones(c.points))[0] - 1) > 0.1     # constants are not
+    else:
+        # periodic wrap and Neumann ghost (= u0) keep constants on every row
+        assert np.abs(Pm @ np.ones(c.points) - 1).max() < 1e-13
+    if kind == "periodic":
+        # the wrap rows see the coarse field's other end
+        assert abs(err[-1]) > 0.1
+
+
+@pytest.mark.parametrize("kind", ["periodic", "dirichlet", "neumann"])
+@pytest.mark.parametrize("align", [0.0, 0.5])
+def test_restriction_is_half_interpolation_transpose(orc, kind, align):
+    spec = {"periodic": P(1.0), "dirichlet": D(1.0), "neumann": N(1.0)}[kind]
+    f, c = _axis(orc, spec, 16, align), _axis(orc, spec, 8, align)
+    Pm = fd.interp1(f, c).toarray()
+    R = fd.restrict1(f, c).toarray()
+    d = R - Pm.T / 2
+    if kind == "neumann" and align == 0.0:
+        # The reference folds the wall ghost into its interpolation with
+        # weights[3] = 1/2 but into its restriction with weights[0] / 2 = 0
+        # (solvers/mg/interpolation.h:79-80 vs 116-118): exactly the two
+        # corner entries differ, by 1/4.
+        bad = np.argwhere(np.abs(d) > 1e-15)
+        assert sorted(map(tuple, bad)) == [(0, 0), (c.points - 1, f.points - 1)]
+        assert np.allclose(np.abs(d[tuple(bad.T)]), 0.25)
+    else:
+        assert np.abs(d).max() < 1e-15
+
+
+def test_smoother_interval_closed_form(orc):
+    # cube, periodic: Gershgorin of I - lam L is [1, 1 + 12 lam r^2], prop 1/8
+    ax = axes(orc, [P(1.0)] * 3, 8, [0.5] * 3)
+    r2 = ax[0].resolution ** 2
+    lam = 0.4 / r2
+    a, b = fd.smoother_interval(ax, fd.operator(ax, 1.0, -lam))
+    lo, hi = 1.0, 1 + 12 * lam * r2
+    assert abs(a - (lo + (hi - lo) / 8)) < 1e-12 and abs(b - hi) < 1e-12
+    # 32 x 64 x 32 box (the C4 shape at 1/4 scale), y no-slip walls, cell
+    # centred in y: wall rows of I~ - lam L carry 3/4 + 6 lam r^2 on the
+    # diagonal and 4 lam r^2 off it, so lo = min(1, 3/4 + 2 lam r^2);
+    # prop = (16/32) (16/64) (16/32) = 1/16.
+    for lamr2 in (0.1, 0.4):
+        ax = axes(orc, [P(0.32), D(0.64), P(0.32)], 25, [0.0, 0.5, 0.5])
+        r2 = ax[1].resolution ** 2
+        lam = lamr2 / r2
+        A = fd.operator(ax, 1.0, -lam)
+        a, b = fd.smoother_interval(ax, A)
+        lo, hi = min(1.0, 0.75 + 2 * lamr2), 1 + 12 * lamr2
+        assert abs(b - hi) < 1e-12
+        assert abs(a - (lo + (hi - lo) / 16)) < 1e-12, (a, lo + (hi - lo) / 16)
+    # the Laplacian's interval [-12 r^2, 0] has |a| > |b|: the ends swap to
+    # (0, -12 r^2), so the smoothing interval is [-12 r^2 / 8, -12 r^2]
+    ax = axes(orc, [P(1.0)] * 3, 8, [0.5] * 3)
+    r2 = ax[0].resolution ** 2
+    a, b = fd.smoother_interval(ax, fd.laplacian(ax))
+    assert abs(b + 12 * r2) < 1e-9 * r2 and abs(a + 12 * r2 / 8) < 1e-9 * r2
+
+
+@pytest.mark.parametrize("align", [0.5, 0.0])
+def test_boundary_operators_couette_identity(orc, align):
+    # A linear profile between walls with values g_lo, g_hi is a steady state:
+    # L u + B_lo g_lo + B_hi g_hi = 0 on every row (fd/boundary_ops.h:29-71
+    # supplies the ghost-cell contribution A gamma r^2 the Laplacian's wall
+    # rows leave out).  y is cell-centred (u, w grids) or face-aligned with
+    # the wall points dropped (v grid).
+    ax = axes(orc, [P(0.5), D(1.0), P(0.5)], 10, [0.5, align, 0.5])
+    L = fd.laplacian(ax)
+    g_lo, g_hi = 0.3, 1.7
+    u = field(ax, lambda x, y, z: g_lo + (g_hi - g_lo) * y / 1.0 + 0 * x)
+    Blo = fd.boundary_op(ax, 1, True, 2)
+    Bhi = fd.boundary_op(ax, 1, False, 2)
+    nface = ax[0].points * ax[2].points
+    assert Blo.shape == (u.size, nface)
+    res = L @ u + Blo @ np.full(nface, g_lo) + Bhi @ np.full(nface, g_hi)
+    assert np.abs(res).max() < 1e-12 * ax[1].resolution ** 2
+    # without the boundary data the wall rows are off by A gamma r^2
+    assert np.abs(L @ u).max() > 0.1 * ax[1].resolution ** 2
+    # order 0 is the same operator without the r^2 factor
+    B0 = fd.boundary_op(ax, 1, False, 0)
+    assert abs(B0 * ax[1].resolution ** 2 - Bhi).max() < 1e-9
+
+
+def test_reference_transfer_order_on_staggered_grid(orc):
+    """The reference folds its 1-D transfers as kron(kron(Px, Py), Pz), which
+    applies Pz along the fastest (x) index of the grid vectors; on the u
+    grid (x face-aligned, z centred) that is no longer a linear interpolation
+    along x or z.  The matrix-free kernels apply each axis's own transfer
+    (the x-fastest fold).  Both V-cycles are convergent and agree closely."""
+    spec = [P(2.0), D(2.0), P(2.0)]
+    align = [0.0, 0.5, 0.5]
+    af = _axes_of(orc, spec, align)
+    f, c = af(16), af(8)
+    n = [a.points for a in f]
+    for order, exact in (("reference", False), ("x-fastest", True)):
+        Pm = fd.transfer(f, c, "interpolation", order)
+        for fn in (lambda x, y, z: x + 0 * y, lambda x, y, z: z + 0 * x):
+            e = (Pm @ field(c, fn) - field(f, fn)).reshape(n[2], n[1], n[0])[2:-2, 2:-2, 2:-2]
+            assert (np.abs(e).max() < 1e-13) == exact
+    lam = 0.4 / f[0].resolution ** 2
+    b = np.random.default_rng(0).standard_normal(np.prod(n))
+    A = fd.operator(f, 1.0, -lam)
+    exact = spla.spsolve(A.tocsc(), b)
+    v = {}
+    for order in ("reference", "x-fastest"):
+        mg = fd.Multigrid(af, 16, 1.0, -lam, order)
+        v[order] = mg.vcycle(b)
+        assert np.linalg.norm(exact - v[order]) < 2e-3 * np.linalg.norm(exact)
+    # measured 1.6e-4
+    assert np.linalg.norm(v["reference"] - v["x-fastest"]) < 1e-3 * np.linalg.norm(v["x-fastest"])
