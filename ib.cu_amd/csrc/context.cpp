@@ -123,6 +123,11 @@ ibm_status ibm_create(ibm_ctx** out, int device)
         return IBM_ERR_HIP;
     }
     ctx->scratch.resize(ibm::SLOT_COUNT);
+    if (ibm::static_scratch_init(ctx, device) != IBM_OK) {
+        hipHostFree(ctx->host_flag);
+        delete ctx;
+        return IBM_ERR_HIP;
+    }
     *out = ctx;
     return IBM_OK;
 }

@@ -52,6 +52,9 @@ ibm_status hip_fail(ibm_ctx* ctx, hipError_t e, const char* what);
 // Returns a device buffer of at least `bytes` for `slot`, reallocating when
 // the request grows.  Contents are not preserved across growth.
 ibm_status scratch(ibm_ctx* ctx, int slot, size_t bytes, void** out);
+// Looks up, once per device, the device address of the fixed workspace of
+// ibm_spread_sets_noalloc; `device` must be current.
+ibm_status static_scratch_init(ibm_ctx* ctx, int device);
 
 // Brackets a kernel launch with timing events when profiling is on.
 struct prof_scope {

@@ -23,9 +23,36 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <climits>
+#include <mutex>
 #include "context.hpp"
 #include "deltas.hpp"
 
+
+// ----------------------------------------------------------------------
+// Fixed workspace of ibm_spread_sets_noalloc: the buffers of one binning
+// pass for at most STATIC_MAX_MARKERS markers and STATIC_MAX_BINS bins
+// (a 128 x 129 x 128 cell grid), 36.6 MB per device, held from module load
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel: its device address (one per GPU) comes from hipGetSymbolAddress.
+// ----------------------------------------------------------------------
+namespace ibm_fixed {
+constexpr size_t STATIC_MAX_MARKERS = size_t(1) << 18;
+constexpr size_t STATIC_MAX_BINS = size_t(128) * 129 * 128;
+constexpr size_t STATIC_SCAN_BYTES = size_t(1) << 20;
+constexpr size_t STATIC_STATS_BYTES = 4096;
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t STATIC_BIN_WORDS = align256(sizeof(int) * (STATIC_MAX_BINS + 1));
+constexpr size_t STATIC_INT_ARRAY = align256(sizeof(int) * STATIC_MAX_MARKERS);
+constexpr size_t STATIC_SCRATCH_BYTES =
+    2 * STATIC_BIN_WORDS +                                   // count, start
+    4 * STATIC_INT_ARRAY +                                   // key, rank, perm, code
+    align256(sizeof(long long) * STATIC_MAX_MARKERS) +       // tmp
+    align256(sizeof(double) * 6 * STATIC_MAX_MARKERS) +      // sorted positions and forces
+    STATIC_STATS_BYTES + STATIC_SCAN_BYTES;
+} // namespace ibm_fixed
+
+static __device__ __align__(256) unsigned char scratch[ibm_fixed::STATIC_SCRATCH_BYTES];
 
 namespace ibm {
 namespace {
@@ -928,10 +955,62 @@ ibm_status check_delta(ibm_ctx* ctx, int delta)
     return IBM_OK;
 }
 
+using namespace ibm_fixed;
+static_assert(sizeof(bin_stats) <= STATIC_STATS_BYTES, "binning statistics exceed their fixed slot");
+constexpr int STATIC_MAX_DEVICES = 64;
+std::mutex static_scratch_mutex;
+void* static_scratch_ptr[STATIC_MAX_DEVICES] = {};
+
+// Carves the binning buffers out of the fixed workspace at `base` (a device
+// address of `scratch`).
+ibm_status static_bin_work(ibm_ctx* ctx, unsigned char* base, int total, int nbins, bin_work* w)
+{
+    if ((size_t) total > STATIC_MAX_MARKERS)
+        return fail(ctx, IBM_ERR_SIZE, "ibm_spread_sets_noalloc takes at most %zu markers (got %d)",
+                    STATIC_MAX_MARKERS, total);
+    if ((size_t) nbins > STATIC_MAX_BINS)
+        return fail(ctx, IBM_ERR_SIZE, "ibm_spread_sets_noalloc takes grids of at most %zu bins (got %d)",
+                    STATIC_MAX_BINS, nbins);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { unsigned char* p = base + off; off += align256(bytes); return p; };
+    w->count = (int*) take(STATIC_BIN_WORDS);
+    w->start = (int*) take(STATIC_BIN_WORDS);
+    w->key = (int*) take(STATIC_INT_ARRAY);
+    w->rank = (int*) take(STATIC_INT_ARRAY);
+    w->perm = (int*) take(STATIC_INT_ARRAY);
+    w->code = (int*) take(STATIC_INT_ARRAY);
+    w->tmp = (long long*) take(sizeof(long long) * STATIC_MAX_MARKERS);
+    w->sorted = (double*) take(sizeof(double) * 6 * STATIC_MAX_MARKERS);
+    w->stats = (bin_stats*) take(STATIC_STATS_BYTES);
+    w->scan_tmp = base + off;
+    size_t tbytes = 0;
+    // A size query: rocPRIM neither allocates nor launches anything here.
+    IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, w->count, w->start, nbins + 1, ctx->stream));
+    if (tbytes > STATIC_SCAN_BYTES)
+        return fail(ctx, IBM_ERR_SIZE, "scan workspace of %zu bytes exceeds the fixed %zu", tbytes, STATIC_SCAN_BYTES);
+    w->scan_bytes = tbytes;
+    return IBM_OK;
+}
+
 } // namespace
 
+// Resolves the device address of `scratch` on `device` (made current by the
+// caller) once per device; ibm_create calls it.
+ibm_status static_scratch_init(ibm_ctx* ctx, int device)
+{
+    if (device < 0 || device >= STATIC_MAX_DEVICES)
+        return fail(ctx, IBM_ERR_ARG, "device %d beyond the fixed-workspace table", device);
+    std::lock_guard<std::mutex> lock(static_scratch_mutex);
+    if (static_scratch_ptr[device]) return IBM_OK;
+    void* p = nullptr;
+    IBM_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    if (!p) return fail(ctx, IBM_ERR_HIP, "hipGetSymbolAddress returned a null workspace address");
+    static_scratch_ptr[device] = p;
+    return IBM_OK;
+}
+
 ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const marker_sets& sets,
-                       double* const* fe)
+                       double* const* fe, unsigned char* fixed_workspace = nullptr)
 {
     kgrids kg;
     IBM_TRY(make_kgrids(ctx, grids, &kg));
@@ -947,8 +1026,9 @@ ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
         if (bd.hi[j] - bd.lo[j] > 1 || bd.hi[j] < bd.lo[j])
             return fail(ctx, IBM_ERR_BAD_GRID, "component grids are not a staggering of one cell grid");
     bin_work w;
-    IBM_TRY(ctx_bin_work(ctx, sets.total, bd.nbins, &w));
-    IBM_TRY(bin_markers(ctx, sets, kg, bd, 1, w, true));
+    if (fixed_workspace) IBM_TRY(static_bin_work(ctx, fixed_workspace, sets.total, bd.nbins, &w));
+    else IBM_TRY(ctx_bin_work(ctx, sets.total, bd.nbins, &w));
+    IBM_TRY(bin_markers(ctx, sets, kg, bd, 1, w, fixed_workspace == nullptr));
     int maxp[3] = {0, 0, 0};
     for (int c = 0; c < 3; ++c)
         for (int j = 0; j < 3; ++j) maxp[j] = kg.points[c][j] > maxp[j] ? kg.points[c][j] : maxp[j];
@@ -970,7 +1050,7 @@ ibm_status spread_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
     double grid_points = 0;
     for (int c = 0; c < 3; ++c) grid_points += (double) kg.points[c][0] * kg.points[c][1] * kg.points[c][2];
     // positions + forces read once, three grids written once
-    prof_scope prof(ctx, "spread_brick", 48.0 * sets.total + 8.0 * grid_points);
+    prof_scope prof(ctx, fixed_workspace ? nullptr : "spread_brick", 48.0 * sets.total + 8.0 * grid_points);
     auto launch = [&](auto kernel) {
         kernel<<<(int) nbricks, SPREAD_THREADS, 0, ctx->stream>>>(
             kg, bd, delta, sets.total, w.start, w.sorted, w.stats, fe[0], fe[1], fe[2], bx, by, (int) nbricks,
@@ -1011,6 +1091,19 @@ ibm_status ibm_spread_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
     ibm::marker_sets s;
     IBM_TRY(ibm::make_sets(ctx, nsets, n, x, f, nullptr, &s));
     return ibm::spread_sets(ctx, grids, (int) delta, s, fe);
+}
+
+ibm_status ibm_spread_sets_noalloc(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nsets,
+                                   const int* n, const double* const* x, const double* const* f,
+                                   double* const* fe)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (nsets > 0 && !f) return ibm::fail(ctx, IBM_ERR_ARG, "null force arrays");
+    if (ctx->device < 0 || ctx->device >= ibm::STATIC_MAX_DEVICES || !ibm::static_scratch_ptr[ctx->device])
+        return ibm::fail(ctx, IBM_ERR_HIP, "the fixed spreading workspace of device %d is not set up", ctx->device);
+    ibm::marker_sets s;
+    IBM_TRY(ibm::make_sets(ctx, nsets, n, x, f, nullptr, &s));
+    return ibm::spread_sets(ctx, grids, (int) delta, s, fe, (unsigned char*) ibm::static_scratch_ptr[ctx->device]);
 }
 
 ibm_status ibm_spread(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int n,

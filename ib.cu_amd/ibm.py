@@ -83,7 +83,7 @@ _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _PD = ctypes.POINTER(ctypes.c_double)
-ABI_VERSION = 3          # IBM_ABI_VERSION of include/ibm.h this binding matches
+ABI_VERSION = 4          # IBM_ABI_VERSION of include/ibm.h this binding matches
 
 _SIGS = {
     "ibm_abi_version": (_I, []),
@@ -104,6 +104,8 @@ _SIGS = {
     "ibm_spread": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, _VP, ctypes.POINTER(_VP)]),
     "ibm_spread_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                              ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ibm_spread_sets_noalloc": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
+                                     ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "ibm_interpolate": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, ctypes.POINTER(_VP), _VP]),
     "ibm_interpolate_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                                   ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
@@ -343,6 +345,19 @@ def spread_sets(ctx, grids, delta, xs, fs, out=None):
         out = [torch.empty(grids[c].size, dtype=torch.float64, device=xs[0].device) for c in range(3)]
     code = lib().ibm_spread_sets(ctx.handle, grids, DELTA.get(delta, delta), len(xs),
                                  (ctypes.c_int * len(ns))(*ns), _ptrs(xs), _ptrs(fs), _ptrs(out))
+    _check(code, ctx)
+    return out
+
+
+def spread_sets_noalloc(ctx, grids, delta, xs, fs, out):
+    """ibm_spread_sets_noalloc: spread_sets into caller-given outputs with the
+    library's fixed per-device workspace (no device allocation)."""
+    ns = [x.numel() // 3 for x in xs]
+    for x, f, n in zip(xs, fs, ns):
+        _check_markers(x, n)
+        _check_markers(f, n)
+    code = lib().ibm_spread_sets_noalloc(ctx.handle, grids, DELTA.get(delta, delta), len(xs),
+                                         (ctypes.c_int * len(ns))(*ns), _ptrs(xs), _ptrs(fs), _ptrs(out))
     _check(code, ctx)
     return out
 

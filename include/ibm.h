@@ -27,7 +27,7 @@ extern "C" {
 
 /* Revision of the entry points' signatures and structures; bindings check it
  * against ibm_abi_version() before calling anything else. */
-#define IBM_ABI_VERSION 3
+#define IBM_ABI_VERSION 4
 int ibm_abi_version(void);
 
 /* ------------------------------------------------------------------ */
@@ -171,6 +171,21 @@ ibm_status ibm_spread(ibm_ctx* ctx, const ibm_grid* grids /* [3] */, ibm_delta d
 ibm_status ibm_spread_sets(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
                            int nsets, const int* n, const double* const* x,
                            const double* const* f, double* const* fe);
+
+/* ibm_spread_sets without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * binning buffers live in a fixed per-device workspace of the library
+ * (36.6 MB, held until the process ends) instead of the context's scratch,
+ * so at most 262,144 markers in total and grids of at most 2,113,536 bins
+ * (one bin per face-aligned cell; a 128 x 129 x 128 cell grid) are taken;
+ * larger inputs return IBM_ERR_SIZE before any work is queued.  Every call
+ * on one GPU shares that workspace: calls that overlap on one device (other
+ * streams, other contexts, other host threads) race and must be ordered by
+ * the caller.  The result is bitwise that of ibm_spread_sets. */
+ibm_status ibm_spread_sets_noalloc(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
+                                   int nsets, const int* n, const double* const* x,
+                                   const double* const* f, double* const* fe);
 
 /* Interpolates grid velocities to markers:
  *     U_c(m) = sum_g u_c(g) * prod_j phi((g_j - X_j(m)) / h)
