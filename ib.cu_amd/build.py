@@ -37,8 +37,19 @@ def sources():
     return out
 
 
-def _compile(src):
+def _headers_mtime():
+    latest = 0.0
+    for d in (CSRC, os.path.join(ROOT, "include")):
+        for name in os.listdir(d):
+            if name.endswith((".hpp", ".h")):
+                latest = max(latest, os.path.getmtime(os.path.join(d, name)))
+    return latest
+
+
+def _compile(src, incremental=False, headers=0.0):
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), headers):
+        return obj
     cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-x", "hip", *COMMON, "-c", src, "-o", obj]
@@ -59,12 +70,15 @@ def device_asm(src, out):
     return out
 
 
-def build(verbose=True):
+def build(verbose=True, incremental=False):
+    """Compiles every source (incremental: only those older than their
+    object or any header) and links libibm.so."""
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = sources()
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
+    headers = _headers_mtime()
     with ThreadPoolExecutor(max_workers=max(jobs, 1)) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental, headers), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -97,7 +111,7 @@ def build_api_check(verbose=True):
 
 if __name__ == "__main__":
     try:
-        build()
+        build(incremental="--incremental" in sys.argv)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

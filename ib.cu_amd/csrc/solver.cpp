@@ -691,6 +691,21 @@ ibm_status ibm_solver_cells(ibm_solver* s, int type, ibm_cell_view* out)
     return IBM_OK;
 }
 
+ibm_status ibm_solver_layout(ibm_solver* s, ibm_state_layout* out)
+{
+    if (!s || !out) return IBM_ERR_ARG;
+    *out = ibm_state_layout{};
+    out->dims = 3;
+    for (int i = 0; i < 3; ++i) out->u_size[i] = (long long) s->nu[i];
+    out->p_size = (long long) s->np;
+    out->ntypes = (int) s->types.size();
+    for (int k = 0; k < out->ntypes; ++k) {
+        out->rows[k] = s->types[k].nd;
+        out->cols[k] = 3 * s->types[k].nc;
+    }
+    return IBM_OK;
+}
+
 ibm_status ibm_solver_grids(ibm_solver* s, ibm_grid* velocity /* [3] */, ibm_grid* pressure)
 {
     if (!s) return IBM_ERR_ARG;

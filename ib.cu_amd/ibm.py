@@ -147,6 +147,14 @@ _SIGS = {
     "ibm_solver_info": (_I, [_VP, _VP, _VP]),
     "ibm_solver_cells": (_I, [_VP, _I, _VP]),
     "ibm_solver_grids": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(ibm_grid)]),
+    "ibm_solver_layout": (_I, [_VP, _VP]),
+    "ibm_io_write_dense": (_I, [_VP, ctypes.c_char_p, _I, _I, ctypes.c_longlong, ctypes.c_longlong, _VP]),
+    "ibm_io_read_dense": (_I, [_VP, ctypes.c_char_p, ctypes.c_longlong, _I, ctypes.c_longlong, ctypes.c_longlong,
+                               _VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "ibm_io_peek_dense": (_I, [_VP, ctypes.c_char_p, ctypes.c_longlong, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                               ctypes.POINTER(_I)]),
+    "ibm_state_write": (_I, [_VP, ctypes.c_char_p, _I, _VP, _VP]),
+    "ibm_state_read": (_I, [_VP, ctypes.c_char_p, _I, _VP, _VP, ctypes.POINTER(_I)]),
 }
 
 
@@ -665,6 +673,61 @@ class ibm_state(ctypes.Structure):
     _fields_ = [("t", _D), ("u", _VP * 3), ("p", _VP), ("x", _VP * MAX_CELL_TYPES)]
 
 
+class ibm_state_layout(ctypes.Structure):
+    _fields_ = [("dims", _I), ("ntypes", _I), ("u_size", ctypes.c_longlong * 3), ("p_size", ctypes.c_longlong),
+                ("rows", _I * MAX_CELL_TYPES), ("cols", _I * MAX_CELL_TYPES)]
+
+
+def state_layout(u_sizes, p_size, shapes):
+    """ibm_state_layout of velocity / pressure sizes and per-type (rows, cols)."""
+    L = ibm_state_layout()
+    L.dims = len(u_sizes)
+    L.ntypes = len(shapes)
+    for i, n in enumerate(u_sizes):
+        L.u_size[i] = n
+    L.p_size = p_size
+    for k, (r, c) in enumerate(shapes):
+        L.rows[k] = r
+        L.cols[k] = c
+    return L
+
+
+def state_write(ctx, path, st, layout, append=True):
+    """Appends (or writes) one ib::state record in the linalg binary format."""
+    _check(lib().ibm_state_write(ctx.handle, os.fsencode(path), int(append), ctypes.byref(st.c),
+                                 ctypes.byref(layout)), ctx)
+
+
+def state_read(ctx, path, st, layout, record=-1):
+    """Reads record `record` (-1: the last complete one) into st's buffers;
+    returns the number of complete records in the file."""
+    n = _I(0)
+    _check(lib().ibm_state_read(ctx.handle, os.fsencode(path), record, ctypes.byref(st.c) if st is not None else None,
+                                ctypes.byref(layout), ctypes.byref(n)), ctx)
+    return n.value
+
+
+def io_write_dense(ctx, path, a, is_matrix, rows=None, cols=None, append=True):
+    rows = a.numel() if rows is None else rows
+    cols = 1 if cols is None else cols
+    _check(lib().ibm_io_write_dense(ctx.handle, os.fsencode(path), int(append), int(is_matrix), rows, cols,
+                                    _ptr(a)), ctx)
+
+
+def io_read_dense(ctx, path, offset, out, is_matrix, rows, cols=1):
+    nxt = ctypes.c_longlong(0)
+    _check(lib().ibm_io_read_dense(ctx.handle, os.fsencode(path), offset, int(is_matrix), rows, cols, _ptr(out),
+                                   ctypes.byref(nxt)), ctx)
+    return nxt.value
+
+
+def io_peek_dense(ctx, path, offset):
+    m, r, c = _I(0), _I(0), _I(0)
+    _check(lib().ibm_io_peek_dense(ctx.handle, os.fsencode(path), offset, ctypes.byref(m), ctypes.byref(r),
+                                   ctypes.byref(c)), ctx)
+    return m.value & 0xffffffff, r.value, c.value
+
+
 class ibm_step_info(ctypes.Structure):
     _fields_ = [("iterations", _I * 4), ("residual", _D * 4), ("divergence", _D), ("timestep", _D),
                 ("exponent", _I), ("reserved", _I)]
@@ -776,6 +839,12 @@ class Solver:
 
     def replay(self, st):
         _check(lib().ibm_solver_replay(self.ctx.handle, self.handle, ctypes.byref(st.c)), self.ctx)
+
+    def layout(self):
+        """ibm_state_layout of this solver's states (checkpoint / resume)."""
+        L = ibm_state_layout()
+        _check(lib().ibm_solver_layout(self.handle, ctypes.byref(L)), self.ctx)
+        return L
 
     def info(self):
         i = ibm_step_info()

@@ -445,6 +445,59 @@ typedef struct {
     const double* velocity;      /* interpolated at the data sites: nd x 3 cells */
 } ibm_cell_view;
 
+/* Shapes of the objects of an ib::state record for checkpoint / resume:
+ * u[0..dims) and p are dense vectors of u_size[i] / p_size values, x[k] is
+ * a dense rows[k] x cols[k] matrix (data sites x 3 cells, column
+ * comp * cells + cell). */
+typedef struct {
+    int dims;
+    int ntypes;
+    long long u_size[3];
+    long long p_size;
+    int rows[IBM_MAX_CELL_TYPES];
+    int cols[IBM_MAX_CELL_TYPES];
+} ibm_state_layout;
+
+/* ---------------------------------------------------------------------
+ * Checkpoint / resume in the reference's linalg binary format
+ * (linalg/io.h:67-99 magic numbers, :416-531 writers and readers): a dense
+ * object is a 4-byte magic (vector "ve" 0x65760000 or matrix "ma"
+ * 0x616d0000, | dense 0x6400, | double 0x64), the int32 header
+ * {rows, cols, rows*cols} and the values (column-major, host byte order).
+ * Device data is staged through host memory; the calls synchronise the
+ * context stream.  Files written here are read by the reference's
+ * `f >> linalg::io::binary >> st` (test/ib-whole.cc:184-187) and by
+ * util/kaboose.cc, and files they write are read here.
+ * ------------------------------------------------------------------- */
+
+/* Appends (append != 0) or writes one dense double vector (is_matrix = 0,
+ * cols ignored) or matrix of device data: linalg::io::write(out, binary, v)
+ * (linalg/io.h:421-454). */
+ibm_status ibm_io_write_dense(ibm_ctx* ctx, const char* path, int append, int is_matrix, long long rows,
+                              long long cols, const double* values);
+/* Reads the dense object at byte `offset` into device memory, which must
+ * have the expected shape (IBM_ERR_SIZE otherwise; IBM_ERR_ARG for a wrong
+ * magic): linalg::io::read(in, binary, v) (linalg/io.h:485-518).  *next
+ * receives the offset after the object. */
+ibm_status ibm_io_read_dense(ibm_ctx* ctx, const char* path, long long offset, int is_matrix, long long rows,
+                             long long cols, double* values, long long* next);
+/* Header of the object at `offset`: its magic word and shape. */
+ibm_status ibm_io_peek_dense(ibm_ctx* ctx, const char* path, long long offset, int* magic, int* rows, int* cols);
+
+/* Writes (append = 0) or appends one ib::state record: t (8 raw bytes),
+ * u[0..dims), p, x[0..ntypes) -- the order of ins::state's writer
+ * (ins/state.h:48-68) extended by the cell matrices, as test/ib-whole.cc's
+ * binary_writer stores every step and util/kaboose.cc:21-31 reads it. */
+ibm_status ibm_state_write(ibm_ctx* ctx, const char* path, int append, const ibm_state* st,
+                           const ibm_state_layout* layout);
+/* Reads record `record` (0-based; negative counts from the end, so -1 is
+ * the last complete record, util/kaboose.cc:53-66) into the state's device
+ * buffers and sets st->t.  *records (if given) receives the number of
+ * complete records; with st == NULL only the records are counted.  A
+ * truncated final record (a run killed while writing) is not counted. */
+ibm_status ibm_state_read(ibm_ctx* ctx, const char* path, int record, ibm_state* st,
+                          const ibm_state_layout* layout, int* records);
+
 ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndims, int refinement,
                              const ibm_solver_params* params, int ntypes, const ibm_cell_type* types,
                              ibm_solver** out);
@@ -463,6 +516,8 @@ ibm_status ibm_solver_replay(ibm_ctx* ctx, ibm_solver* s, ibm_state* st);
 ibm_status ibm_solver_info(ibm_ctx* ctx, ibm_solver* s, ibm_step_info* out);
 ibm_status ibm_solver_cells(ibm_solver* s, int type, ibm_cell_view* out);
 ibm_status ibm_solver_grids(ibm_solver* s, ibm_grid* velocity /* [3] */, ibm_grid* pressure);
+/* Shapes of this solver's states, for ibm_state_write / ibm_state_read. */
+ibm_status ibm_solver_layout(ibm_solver* s, ibm_state_layout* out);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -548,6 +548,13 @@ public:
         ctx.check(ibm_solver_info(ctx.get(), s_.get(), &i));
         return i;
     }
+    // Shapes of this solver's states (checkpoint / resume).
+    ibm_state_layout layout() const
+    {
+        ibm_state_layout l{};
+        ibm::check(ibm_solver_layout(s_.get(), &l));
+        return l;
+    }
     ibm_solver* get() const { return s_.get(); }
 
 private:
@@ -558,3 +565,33 @@ private:
 };
 
 } // namespace ib
+
+// Checkpoint / resume in the linalg binary format (linalg/io.h).  The
+// reference streams a state with `f << linalg::io::binary << st` and reads
+// it back with `f >> linalg::io::binary >> st` (test/ib-whole.cc:184-187);
+// here the file is named by its path and the state's shapes come from
+// ib::solver::layout().
+namespace linalg {
+namespace io {
+
+inline constexpr struct binary_t {} binary{};
+
+// Appends (or, with append = false, writes) one state record.
+inline void write(const std::string& path, const ib::state& st, const ibm_state_layout& layout, bool append = true,
+                  ibm::context& ctx = ibm::context::current())
+{
+    ctx.check(ibm_state_write(ctx.get(), path.c_str(), append ? 1 : 0, &st, &layout));
+}
+
+// Reads record `record` (-1: the last complete one, util/kaboose.cc) into
+// the state's buffers; returns the number of complete records in the file.
+inline int read(const std::string& path, ib::state& st, const ibm_state_layout& layout, int record = -1,
+                ibm::context& ctx = ibm::context::current())
+{
+    int n = 0;
+    ctx.check(ibm_state_read(ctx.get(), path.c_str(), record, &st, &layout, &n));
+    return n;
+}
+
+} // namespace io
+} // namespace linalg

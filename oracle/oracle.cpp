@@ -264,57 +264,57 @@ void orc_spread(const orc_axis* grids, int kind, int n, const double* x, const d
         maxsize = std::max(maxsize, size[c]);
     }
     std::vector<double> buf(8 * maxsize, 0.0);     // 8 plane buffers (ib/novel.h:180-191)
-    std::vector<int> keys(n), perm(n);
-    std::vector<double> values(8 * (size_t) n);
+    std::vector<int> keys(n), perm(n), skeys(n), seg;
+    std::vector<double> dd(3 * (size_t) n), fv(n);
     for (int c = 0; c < 3; ++c) {
         const grid3& gc = g[c];
         double res = gc.a[0].res * (gc.a[1].res * (gc.a[2].res * 1));
         // index(): keys and identity permutation, then sort by key
+#pragma omp parallel for schedule(static)
         for (int t = 0; t < n; ++t) {
             double p[3] = {x[t], x[n + t], x[2 * n + t]};
             keys[t] = sort_key(gc, p);
             perm[t] = t;
         }
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return keys[a] < keys[b]; });
-        std::vector<int> skeys(n);
-        for (int t = 0; t < n; ++t) skeys[t] = keys[perm[t]];
-        // reduce(): 8 sweeps of 8 values each
-        for (int sweep = 0; sweep < 8; ++sweep) {
-            for (int t = 0; t < n; ++t) {
-                int j = perm[t];
-                double fv = f[(size_t) c * n + j];
-                double p[3] = {x[j], x[n + j], x[2 * n + j]};
-                double d[3];
-                difference(gc, p, d);
-                for (int i = 0; i < 8; ++i) {
-                    int s[3];
-                    pattern(8 * sweep + i, s);
-                    values[8 * (size_t) t + i] = weight_of(kind, d, s) * fv;
-                }
-            }
-            // reduce_by_key over equal consecutive keys, then write planes
-            int t = 0;
-            while (t < n) {
-                int key = skeys[t];
-                double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                for (int i = 0; i < 8; ++i) r[i] = values[8 * (size_t) t + i];
-                int u = t + 1;
-                while (u < n && skeys[u] == key) {
-                    for (int i = 0; i < 8; ++i) r[i] += values[8 * (size_t) u + i];
-                    ++u;
-                }
-                int dec[3];
-                decompose(gc, key, dec);
-                for (int i = 0; i < 8; ++i) {
-                    int s[3];
-                    pattern(8 * sweep + i, s);
+        seg.clear();
+        for (int t = 0; t < n; ++t) {
+            skeys[t] = keys[perm[t]];
+            if (t == 0 || skeys[t] != skeys[t - 1]) seg.push_back(t);
+        }
+        seg.push_back(n);
+        // sorted offsets and forces of this component
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < n; ++t) {
+            int j = perm[t];
+            double p[3] = {x[j], x[n + j], x[2 * n + j]};
+            difference(gc, p, &dd[3 * (size_t) t]);
+            fv[t] = f[(size_t) c * n + j];
+        }
+        // reduce(): 8 sweeps of 8 values each, reduce_by_key over equal
+        // consecutive keys, then the plane writes.  Plane i only ever
+        // receives value i of each sweep, so the planes are independent and
+        // each keeps the sequential summation order of the reference.
+        const int nseg = (int) seg.size() - 1;
+#pragma omp parallel for schedule(static, 1)
+        for (int i = 0; i < 8; ++i) {
+            double* plane = &buf[(size_t) i * size[c]];
+            for (int sweep = 0; sweep < 8; ++sweep) {
+                int s[3];
+                pattern(8 * sweep + i, s);
+                for (int q = 0; q < nseg; ++q) {
+                    int t = seg[q], e = seg[q + 1];
+                    double r = weight_of(kind, &dd[3 * (size_t) t], s) * fv[t];
+                    for (int u = t + 1; u < e; ++u) r += weight_of(kind, &dd[3 * (size_t) u], s) * fv[u];
+                    int dec[3];
+                    decompose(gc, skeys[t], dec);
                     int k = grid_index(gc, dec, s, false);
-                    if (k >= 0) buf[(size_t) i * size[c] + k] += res * r[i];
+                    if (k >= 0) plane[k] += res * r;
                 }
-                t = u;
             }
         }
         // combine planes (ib/novel.h:153-162)
+#pragma omp parallel for schedule(static)
         for (size_t k = 0; k < size[c]; ++k) {
             double acc = 0.0;
             for (int i = 0; i < 8; ++i) {
@@ -334,6 +334,7 @@ void orc_interp(const orc_axis* grids, int kind, int n, const double* x,
     for (int c = 0; c < 3; ++c) {
         grid3 g;
         for (int j = 0; j < 3; ++j) g.a[j] = from_orc(grids[3 * c + j]);
+#pragma omp parallel for schedule(static)
         for (int t = 0; t < n; ++t) {
             double p[3] = {x[t], x[n + t], x[2 * n + t]};
             int key = sort_key(g, p);
