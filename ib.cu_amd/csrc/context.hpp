@@ -75,6 +75,16 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed);
 
+// Membrane force passes (forces.hip) on `stream`.
+ibm_status membrane_table(ibm_ctx* ctx, hipStream_t stream, int ns, const double* geo, const double* w, double* table);
+ibm_status mean_curvature(ibm_ctx* ctx, hipStream_t stream, int nd, int nc, const double* geo, const double* h0,
+                          double* h);
+ibm_status membrane_forces(ibm_ctx* ctx, hipStream_t stream, const ibm_membrane* m, double tether, int ns, int nc,
+                           const double* geo, const double* normal, const double* sigma, const double* hd,
+                           const double* table, const double* y, const double* yref, double beta, double* f);
+// Number of operators an RBF object holds.
+int rbf_nops(const ibm_rbf* op);
+
 } // namespace ibm
 
 #define IBM_HIP(ctx, ...)                                                   \

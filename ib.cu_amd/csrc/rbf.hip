@@ -328,6 +328,8 @@ struct ibm_rbf {
 
 namespace ibm {
 
+int rbf_nops(const ibm_rbf* op) { return op ? op->nops : 0; }
+
 ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed)

@@ -6,9 +6,13 @@
 //      [y, y_theta, y_phi] = E^T X (one RBF launch for all types), unit
 //      normals, patch areas and per-cell area / volume in FP64
 //      (bases/geometry.h:74-123);
-//   2. membrane forces at the sample sites: a tether f = -k (y - y_ref) to
-//      the reference sample positions (the spring of test/ib-whole.cc:237-245;
-//      the reference's Skalak / bending models are outside this library);
+//   2. membrane forces at the sample sites: per cell type the tension law
+//      (Skalak or neo-Hookean, forces/tension.h) and bending
+//      (forces/bending.h) against the type's undeformed shape, whose
+//      mean curvature needs the data-site geometry (five data-to-data
+//      operators) and its derivatives at the sample sites (five sample
+//      operators on h), and / or a tether f = -k (y - y_ref) to reference
+//      sample positions (the spring of test/ib-whole.cc:237-245);
 //   3. F = spread(y, f) onto the MAC grid (ib/novel.h);
 //   4. H = ins::advection(u) (ins/advection.h:58-125);
 //   5. per component, ins::diffusion (ins/diffusion.h:46-56) with gamma = 1:
@@ -121,6 +125,14 @@ struct ibm_solver {
         int nd, ns, nc;
         double stiffness;
         double *w, *ref, *Y, *f, *normal, *sigma, *area, *volume, *U;
+        // membrane models: Y holds six blocks (position, x_u, x_v, x_uu,
+        // x_uv, x_vv); bending adds the data geometry Yd (five blocks of
+        // nd x 3 cells), h = H - H0 at the data sites and its sample-site
+        // derivatives hd (six ns x nc blocks, the first unused)
+        ibm_membrane mem;
+        bool membrane, bend;
+        ibm_rbf* dops;
+        double *table, *h0, *Yd, *h, *hd;
     };
     std::vector<type_t> types;
     double *F[3]{}, *H[3]{}, *rhs[3]{}, *du[3]{}, *div = nullptr;
@@ -281,24 +293,26 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             Y[count] = T.Y;
             ++count;
         }
-        // Positions (operator 0) first: the forces and the spreading need
-        // only them.
-        ibm_ctx* gc = concurrent ? s->aux[0] : ctx;
+        bool any_membrane = false;
+        for (int t = 0; t < nt; ++t) any_membrane = any_membrane || (s->types[t].membrane && s->types[t].nc);
+        // Positions (operator 0) first: tether forces and the spreading need
+        // only them.  Membrane forces need every operator, so with a
+        // membrane model the whole geometry stays on the caller's stream.
+        const bool split_geometry = concurrent && !any_membrane;
+        ibm_ctx* gc = split_geometry ? s->aux[0] : ctx;
         unsigned char* packed = nullptr;
         if (count) {
-            IBM_TRY(rbf_eval_range(ctx, ctx->stream, count, ops, N, X, Y, 0, 1, true, &packed));
-            if (concurrent) {
+            if (!split_geometry) {
+                IBM_TRY(rbf_eval_range(ctx, ctx->stream, count, ops, N, X, Y, 0, 8, true, &packed));
+            } else {
+                IBM_TRY(rbf_eval_range(ctx, ctx->stream, count, ops, N, X, Y, 0, 1, true, &packed));
                 IBM_HIP(ctx, hipEventRecord(s->geo_ev, ctx->stream));
                 IBM_HIP(ctx, hipStreamWaitEvent(gc->stream, s->geo_ev, 0));
-            }
-            // One tangent operator per launch when concurrent: half as many
-            // long-lived matrix-core blocks resident beside the force /
-            // binning / spreading chain.
-            if (concurrent) {
+                // One tangent operator per launch: half as many long-lived
+                // matrix-core blocks resident beside the force / binning /
+                // spreading chain.
                 IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 2, false, &packed)));
                 IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 2, 8, false, &packed)));
-            } else {
-                IBM_TRY(step_status(ctx, gc, rbf_eval_range(gc, gc->stream, count, ops, N, X, Y, 1, 8, false, &packed)));
             }
         }
         for (int t = 0; t < nt; ++t) {
@@ -307,9 +321,61 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             size_t blk = (size_t) T.ns * 3 * T.nc;
             IBM_TRY(step_status(ctx, gc, ibm_surface_geometry(gc, T.ns, T.nc, T.Y, T.Y + blk, T.Y + 2 * blk, T.w,
                                                                T.normal, T.sigma, T.area, T.volume)));
-            // 2. tether forces
-            tether_kernel<<<blocks(blk), 256, 0, ctx->stream>>>(blk, T.stiffness, T.Y, T.ref, T.f);
-            IBM_LAUNCH_CHECK(ctx);
+            // 2. tether forces (cell types without a membrane model)
+            if (!T.membrane) {
+                if (T.stiffness != 0.0) {
+                    tether_kernel<<<blocks(blk), 256, 0, ctx->stream>>>(blk, T.stiffness, T.Y, T.ref, T.f);
+                    IBM_LAUNCH_CHECK(ctx);
+                } else {
+                    IBM_TRY(launch_fill<double>(ctx, blk, 0.0, T.f));
+                }
+            }
+        }
+        // 2. membrane forces: bending's mean curvature at the data sites
+        // (data geometry of every bending type in one launch), its
+        // derivatives at the sample sites (one launch), then per type the
+        // tension + bending (+ tether) pass
+        if (any_membrane) {
+            ibm_rbf* dops[MAX_TYPES];
+            ibm_rbf* hops[MAX_TYPES];
+            int dN[MAX_TYPES], hN[MAX_TYPES];
+            const double* dX[MAX_TYPES];
+            const double* hX[MAX_TYPES];
+            double* dY[MAX_TYPES];
+            double* hY[MAX_TYPES];
+            int nb = 0;
+            for (int t = 0; t < nt; ++t) {
+                auto& T = s->types[t];
+                if (!T.nc || !T.bend) continue;
+                dops[nb] = T.dops;
+                dN[nb] = 3 * T.nc;
+                dX[nb] = st->x[t];
+                dY[nb] = T.Yd;
+                hops[nb] = T.ops;
+                hN[nb] = T.nc;
+                hX[nb] = T.h;
+                hY[nb] = T.hd;
+                ++nb;
+            }
+            if (nb) {
+                unsigned char* dpacked = nullptr;
+                IBM_TRY(rbf_eval_range(ctx, ctx->stream, nb, dops, dN, dX, dY, 0, 5, true, &dpacked));
+                for (int t = 0; t < nt; ++t) {
+                    auto& T = s->types[t];
+                    if (!T.nc || !T.bend) continue;
+                    IBM_TRY(mean_curvature(ctx, ctx->stream, T.nd, T.nc, T.Yd, T.mem.preferred ? T.h0 : nullptr, T.h));
+                }
+                unsigned char* hpacked = nullptr;
+                IBM_TRY(rbf_eval_range(ctx, ctx->stream, nb, hops, hN, hX, hY, 1, 6, true, &hpacked));
+            }
+            for (int t = 0; t < nt; ++t) {
+                auto& T = s->types[t];
+                if (!T.nc || !T.membrane) continue;
+                size_t blk = (size_t) T.ns * 3 * T.nc;
+                IBM_TRY(membrane_forces(ctx, ctx->stream, &T.mem, T.stiffness, T.ns, T.nc, T.Y + blk, T.normal,
+                                        T.sigma, T.bend ? T.hd + (size_t) T.ns * T.nc : nullptr, T.table, T.Y,
+                                        T.ref, 0.0, T.f));
+            }
         }
     }
     // 2b. step size (ins::tracker) from the force bound
@@ -552,7 +618,8 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     if (st == IBM_OK) st = alloc(ctx, s, REDUCE_PARTS, &s->partial);
     for (int t = 0; t < ntypes && st == IBM_OK; ++t) {
         const ibm_cell_type& c = types[t];
-        if (!c.ops || c.data_sites <= 0 || c.sample_sites <= 0 || c.cells < 0 || !c.weights || !c.tether) {
+        if (!c.ops || c.data_sites <= 0 || c.sample_sites <= 0 || c.cells < 0 || !c.weights ||
+            (c.stiffness != 0.0 && !c.tether)) {
             st = fail(ctx, IBM_ERR_ARG, "bad cell type %d", t);
             break;
         }
@@ -562,16 +629,71 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         T.ns = c.sample_sites;
         T.nc = c.cells;
         T.stiffness = c.stiffness;
+        T.mem = c.membrane;
+        T.membrane = c.membrane.tension != IBM_TENSION_NONE || c.membrane.bending != 0.0;
+        T.bend = c.membrane.bending != 0.0;
+        T.dops = c.data_ops;
+        if (c.membrane.tension != IBM_TENSION_NONE && c.membrane.tension != IBM_TENSION_SKALAK &&
+            c.membrane.tension != IBM_TENSION_NEOHOOKEAN) {
+            st = fail(ctx, IBM_ERR_ARG, "cell type %d: unknown tension law %d", t, c.membrane.tension);
+            break;
+        }
+        if (T.membrane && (rbf_nops(c.ops) < 6 || !c.reference)) {
+            st = fail(ctx, IBM_ERR_ARG, "cell type %d: membrane models need the six sample operators (position, two "
+                      "first and three second derivatives) and the reference shape", t);
+            break;
+        }
+        if (T.bend && rbf_nops(c.data_ops) < 5) {
+            st = fail(ctx, IBM_ERR_ARG, "cell type %d: bending needs the five data-site derivative operators", t);
+            break;
+        }
+        const int nblocks = T.membrane ? 6 : 3;
         size_t blk = (size_t) T.ns * 3 * T.nc;
-        if ((st = alloc(ctx, s, T.ns, &T.w)) || (st = alloc(ctx, s, blk, &T.ref)) || (st = alloc(ctx, s, 3 * blk, &T.Y)) ||
+        if ((st = alloc(ctx, s, T.ns, &T.w)) || (st = alloc(ctx, s, (size_t) nblocks * blk, &T.Y)) ||
             (st = alloc(ctx, s, blk, &T.f)) || (st = alloc(ctx, s, blk, &T.normal)) ||
             (st = alloc(ctx, s, (size_t) T.ns * T.nc, &T.sigma)) || (st = alloc(ctx, s, T.nc, &T.area)) ||
             (st = alloc(ctx, s, T.nc, &T.volume)) || (st = alloc(ctx, s, (size_t) 3 * T.nd * T.nc, &T.U)))
             break;
-        if (hipMemcpyAsync(T.w, c.weights, sizeof(double) * T.ns, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
-            hipMemcpyAsync(T.ref, c.tether, sizeof(double) * blk, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        if (hipMemcpyAsync(T.w, c.weights, sizeof(double) * T.ns, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
             st = fail(ctx, IBM_ERR_HIP, "copying cell type %d", t);
             break;
+        }
+        if (c.stiffness != 0.0) {
+            if ((st = alloc(ctx, s, blk, &T.ref))) break;
+            if (hipMemcpyAsync(T.ref, c.tether, sizeof(double) * blk, hipMemcpyDeviceToDevice, ctx->stream) !=
+                hipSuccess) {
+                st = fail(ctx, IBM_ERR_HIP, "copying cell type %d", t);
+                break;
+            }
+        }
+        if (T.membrane) {
+            // The undeformed shape's sample geometry (one cell) and its table
+            // of metric terms, areas and mean curvatures.
+            double* refY = nullptr;
+            if ((st = alloc(ctx, s, 6 * (size_t) T.ns * 3, &refY)) || (st = alloc(ctx, s, 14 * (size_t) T.ns, &T.table)))
+                break;
+            int three = 3;
+            const double* rx = c.reference;
+            unsigned char* packed = nullptr;
+            if ((st = rbf_eval_range(ctx, ctx->stream, 1, &T.ops, &three, &rx, &refY, 0, 6, true, &packed))) break;
+            if ((st = membrane_table(ctx, ctx->stream, T.ns, refY + (size_t) T.ns * 3, T.w, T.table))) break;
+        }
+        if (T.bend) {
+            if ((st = alloc(ctx, s, 5 * (size_t) T.nd * 3 * T.nc, &T.Yd)) ||
+                (st = alloc(ctx, s, (size_t) T.nd * T.nc, &T.h)) ||
+                (st = alloc(ctx, s, 6 * (size_t) T.ns * T.nc, &T.hd)))
+                break;
+            if (T.mem.preferred) {
+                // H of the undeformed shape at the data sites
+                double* refYd = nullptr;
+                if ((st = alloc(ctx, s, 5 * (size_t) T.nd * 3, &refYd)) || (st = alloc(ctx, s, T.nd, &T.h0))) break;
+                int three = 3;
+                const double* rx = c.reference;
+                unsigned char* packed = nullptr;
+                if ((st = rbf_eval_range(ctx, ctx->stream, 1, &T.dops, &three, &rx, &refYd, 0, 5, true, &packed)))
+                    break;
+                if ((st = mean_curvature(ctx, ctx->stream, T.nd, 1, refYd, nullptr, T.h0))) break;
+            }
         }
         s->types.push_back(T);
     }

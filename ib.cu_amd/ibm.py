@@ -148,6 +148,9 @@ _SIGS = {
     "ibm_solver_cells": (_I, [_VP, _I, _VP]),
     "ibm_solver_grids": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(ibm_grid)]),
     "ibm_solver_layout": (_I, [_VP, _VP]),
+    "ibm_membrane_reference": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "ibm_mean_curvature": (_I, [_VP, _I, _I, _VP, _VP, _VP]),
+    "ibm_membrane_forces": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
     "ibm_io_write_dense": (_I, [_VP, ctypes.c_char_p, _I, _I, ctypes.c_longlong, ctypes.c_longlong, _VP]),
     "ibm_io_read_dense": (_I, [_VP, ctypes.c_char_p, ctypes.c_longlong, _I, ctypes.c_longlong, ctypes.c_longlong,
                                _VP, ctypes.POINTER(ctypes.c_longlong)]),
@@ -664,9 +667,23 @@ class ibm_solver_params(ctypes.Structure):
                 ("delta", _I), ("precision", _I), ("max_iterations", _I), ("flags", _I), ("fscale", _D)]
 
 
+TENSION = {"none": 0, "skalak": 1, "neohookean": 2}
+
+
+class ibm_membrane(ctypes.Structure):
+    _fields_ = [("tension", _I), ("preferred", _I), ("shear", _D), ("bulk", _D), ("bending", _D)]
+
+
+def membrane(tension="none", shear=0.0, bulk=0.0, bending=0.0, preferred=False):
+    """ibm_membrane: a tension law ("skalak", "neohookean") with its moduli
+    and a bending modulus (forces/skalak.h, neohookean.h, bending.h)."""
+    return ibm_membrane(TENSION.get(tension, tension), int(bool(preferred)), shear, bulk, bending)
+
+
 class ibm_cell_type(ctypes.Structure):
     _fields_ = [("ops", _VP), ("data_sites", _I), ("sample_sites", _I), ("cells", _I), ("reserved", _I),
-                ("weights", _VP), ("tether", _VP), ("stiffness", _D)]
+                ("weights", _VP), ("tether", _VP), ("stiffness", _D), ("membrane", ibm_membrane),
+                ("data_ops", _VP), ("reference", _VP)]
 
 
 class ibm_state(ctypes.Structure):
@@ -690,6 +707,35 @@ def state_layout(u_sizes, p_size, shapes):
         L.rows[k] = r
         L.cols[k] = c
     return L
+
+
+def membrane_reference(ctx, ns, geometry, weights, out=None):
+    """ibm_membrane_reference: the 14 x ns table of an undeformed shape."""
+    import torch
+    if out is None:
+        out = torch.empty(14 * ns, dtype=torch.float64, device=weights.device)
+    _check(lib().ibm_membrane_reference(ctx.handle, ns, _ptr(geometry), _ptr(weights), _ptr(out)), ctx)
+    return out
+
+
+def mean_curvature(ctx, nd, nc, geometry, h0=None, out=None):
+    import torch
+    if out is None:
+        out = torch.empty(nd * nc, dtype=torch.float64, device=geometry.device)
+    _check(lib().ibm_mean_curvature(ctx.handle, nd, nc, _ptr(geometry), _ptr(h0) if h0 is not None else None,
+                                    _ptr(out)), ctx)
+    return out
+
+
+def membrane_forces(ctx, model, ns, nc, geometry, normal, sigma, hd, table, out=None, beta=0.0):
+    import torch
+    if out is None:
+        out = torch.zeros(3 * ns * nc, dtype=torch.float64, device=geometry.device)
+    _check(lib().ibm_membrane_forces(ctx.handle, ctypes.byref(model), ns, nc, _ptr(geometry),
+                                     _ptr(normal) if normal is not None else None,
+                                     _ptr(sigma) if sigma is not None else None,
+                                     _ptr(hd) if hd is not None else None, _ptr(table), beta, _ptr(out)), ctx)
+    return out
 
 
 def state_write(ctx, path, st, layout, append=True):
@@ -783,8 +829,11 @@ class Solver:
     """The immersed-boundary timestep (ibm_solver_*).
 
     cell_types: list of dicts with keys ops (RbfOperator with 3 operators:
-    position, d/dtheta, d/dphi), cells, weights (CUDA tensor, ns),
-    tether (CUDA tensor, ns x 3 cells), stiffness."""
+    position, d/dtheta, d/dphi, or 6 with the second derivatives), cells,
+    weights (CUDA tensor, ns), and the forces: tether (CUDA tensor,
+    ns x 3 cells) with stiffness, and / or membrane (ibm.membrane(...)) with
+    reference (CUDA tensor, the undeformed data sites nd x 3) and, for
+    bending, data_ops (RbfOperator, 5 data-to-data derivative operators)."""
 
     def __init__(self, ctx, domain, refinement, timestep, diffusivity, density=1.0, tolerance=1e-10,
                  delta="cosine", precision=64, max_iterations=100, cell_types=(), fscale=0.0,
@@ -800,8 +849,14 @@ class Solver:
         self._keep = []
         for k, c in enumerate(cell_types):
             op = c["ops"]
+            tether = c.get("tether")
+            mem = c.get("membrane") or ibm_membrane()
+            dops = c.get("data_ops")
+            ref = c.get("reference")
             arr[k] = ibm_cell_type(op.handle.value, op.K, op.M, c["cells"], 0, c["weights"].data_ptr(),
-                                   c["tether"].data_ptr(), c["stiffness"])
+                                   tether.data_ptr() if tether is not None else None, c.get("stiffness", 0.0),
+                                   mem, dops.handle.value if dops is not None else None,
+                                   ref.data_ptr() if ref is not None else None)
             self._keep.append(c)
         dims, nd = _dims(domain)
         h = _VP()

@@ -340,6 +340,51 @@ ibm_status ibm_surface_geometry(ibm_ctx* ctx, int ns, int nc, const double* y, c
 ibm_status ibm_axpby(ibm_ctx* ctx, long long n, double a, const double* x, double b, double* y);
 
 /* ---------------------------------------------------------------------
+ * Membrane forces (include/forces/)
+ *
+ * Geometry blocks follow bases::geometry: for rows sites x nc cells,
+ * component i of point k = cell * rows + site is at [i * P + k],
+ * P = rows * nc; a "geometry" argument holds the five derivative blocks
+ * x_u, x_v, x_uu, x_uv, x_vv (u = theta, v = phi; the order of
+ * bases::operators' first and second derivatives, bases/operators.h:
+ * 168-190), 3P apart.  Scalar fields are P-long blocks.
+ * ------------------------------------------------------------------- */
+enum {
+    IBM_TENSION_NONE = 0,
+    IBM_TENSION_SKALAK = 1,       /* forces::skalak      (forces/skalak.h:11-36)      */
+    IBM_TENSION_NEOHOOKEAN = 2    /* forces::neohookean  (forces/neohookean.h:10-34)  */
+};
+
+typedef struct {
+    int tension;      /* IBM_TENSION_*                                                 */
+    int preferred;    /* forces::bending's `preferred`: bend relative to the reference */
+    double shear;     /* tension moduli, energy per area                               */
+    double bulk;
+    double bending;   /* forces::bending modulus (energy); 0 = no bending              */
+} ibm_membrane;
+
+/* Per-site table of the undeformed shape (one cell, ns sites): its metric
+ * terms, patch areas w |x_u x x_v| and mean curvature, 14 x ns doubles; the
+ * reference geometry every cell's forces compare against
+ * (object.geometry(reference) of forces/tension.h:56-60). */
+ibm_status ibm_membrane_reference(ibm_ctx* ctx, int ns, const double* geometry, const double* weights,
+                                  double* table);
+/* Mean curvature at the data sites, h = H (minus h0[site] when h0 is given:
+ * the preferred curvature), from the data-site geometry (nd x nc cells),
+ * forces/bending.h:142-158. */
+ibm_status ibm_mean_curvature(ibm_ctx* ctx, int nd, int nc, const double* geometry, const double* h0, double* h);
+/* f = beta f + tension + bending at the sample sites (ns x nc cells):
+ * forces/tension.h:50-99 with the model's law, plus
+ * -4 kappa s (Lap(H - H0) + 2 (H - H0)(H^2 - K + H H0)) n of
+ * forces/bending.h:160-223, whose Laplacian takes hd = the five parameter
+ * derivatives (hu, hv, huu, huv, hvv) of h at the sample sites (P-long
+ * blocks; unused without bending).  normal and sigma are the current
+ * geometry's (ibm_surface_geometry). */
+ibm_status ibm_membrane_forces(ibm_ctx* ctx, const ibm_membrane* model, int ns, int nc, const double* geometry,
+                               const double* normal, const double* sigma, const double* hd, const double* table,
+                               double beta, double* f);
+
+/* ---------------------------------------------------------------------
  * Boundary data and fixed-count solves
  * ------------------------------------------------------------------- */
 /* y += alpha * B g with B = fd::lower_boundary (side 0) / fd::upper_boundary
@@ -408,12 +453,18 @@ typedef struct {
 #define IBM_SOLVER_PSEUDO_PRESSURE 1
 
 typedef struct {
-    ibm_rbf* ops;            /* 3 operators data -> sample: position, d/dtheta, d/dphi */
+    ibm_rbf* ops;            /* operators data -> sample: position, d/dtheta, d/dphi, and with a
+                                membrane model also d2/dtheta2, d2/dtheta dphi, d2/dphi2 (6) */
     int data_sites, sample_sites, cells;
     int reserved;
     const double* weights;   /* sample-site quadrature weights (device; copied) */
-    const double* tether;    /* reference sample positions, ns x 3 cells (device; copied) */
+    const double* tether;    /* null, or reference sample positions, ns x 3 cells (device; copied) */
     double stiffness;        /* tether force f = -stiffness (y - tether) */
+    ibm_membrane membrane;   /* tension / bending models (all zero: none) */
+    ibm_rbf* data_ops;       /* bending: 5 operators data -> data: d/dtheta, d/dphi, d2/dtheta2,
+                                d2/dtheta dphi, d2/dphi2 (the data geometry of forces/bending.h:146) */
+    const double* reference; /* membrane models: the undeformed shape's data sites, nd x 3 (one cell;
+                                device; used at creation) */
 } ibm_cell_type;
 
 /* ib::state (test/ib-whole.cc:159,180-181): caller-owned device buffers. */

@@ -3,7 +3,10 @@ the oracle's pieces (test infrastructure only).
 
 The reference's own step (ib::solver, test/ib-whole.cc:226-275) is not part
 of the reference, so this follows the library's documented scheme:
-  y = E^T X (oracle/rbf.py), f = -k (y - y_ref),
+  y = E^T X (oracle/rbf.py), f = -k (y - y_ref) and / or the membrane
+  models of oracle/forces.py (tension against the undeformed shape's
+  geometry; bending from H at the data sites, differentiated at the sample
+  sites by the sample derivative operators),
   F = spread(y, f) (oracle/orc.py; two cell types spread separately and added),
   H = advection(u) (oracle/fd.py),
   rhs = dt I~ (F / rho - H) + dt nu (b + L u),  (I~ - dt nu / 2 L) du = rhs
@@ -26,11 +29,15 @@ CENTRE = [0.5, 0.5, 0.5]
 class Step:
     """spec: [(length, solid, lower, upper)] x 3 as for orc.grid_axes."""
 
-    def __init__(self, orc, fd, rbf, spec, refinement, dt, nu, rho, delta, types, pseudo_pressure=False):
+    def __init__(self, orc, fd, rbf, spec, refinement, dt, nu, rho, delta, types, pseudo_pressure=False,
+                 forces=None):
         self.orc, self.fd, self.rbf = orc, fd, rbf
+        self.forces_mod = forces
         self.spec, self.ref = spec, refinement
         self.dt, self.nu, self.rho, self.delta = dt, nu, rho, delta
-        self.types = types                  # dicts: E, Et, Ep, nd, ns, nc, w, ref (3, nc, ns), k
+        self.types = types                  # dicts: E, Et, Ep, nd, ns, nc, w, ref (3, nc, ns), k; membrane
+                                            # types add seconds (Euu, Euv, Evv), data_ops (5), reference (3, nd),
+                                            # membrane {tension, shear, bulk, bending, preferred}
         self.pseudo_pressure = pseudo_pressure
         pspec = [(L, s, (0.0, 1.0) if s else lo, (0.0, 1.0) if s else up) for (L, s, lo, up) in spec]
         self.pspec = pspec
@@ -62,12 +69,68 @@ class Step:
             out.append((y, t1, t2))
         return out
 
+    def forces(self, T, X, geo):
+        """Forces (3, nc, ns) of one cell type at geometry geo = (y, t1, t2)."""
+        y = geo[0]
+        f = np.zeros_like(y)
+        if T.get("k", 0.0):
+            f = f - T["k"] * (y - T["ref"])
+        m = T.get("membrane")
+        if not m:
+            return f
+        fo = self.forces_mod
+        nd, ns, nc = T["nd"], T["ns"], T["nc"]
+        P = ns * nc
+        ev = lambda M, Z, K=nd: self.rbf.evaluate(M, K, Z)
+
+        def blocks(mats, Z, rows):
+            # (5, 3, rows * cells) from operators applied to Z (3 * cells, nd)
+            cells = Z.shape[0] // 3
+            return np.stack([ev(M, Z).reshape(3, cells * rows) for M in mats])
+
+        sample_ops = [T["Et"], T["Ep"]] + list(T["seconds"])
+        Xm = X.reshape(3 * nc, nd)
+        g = blocks(sample_ops, Xm, ns)
+        t, tt = g[:2], g[2:]
+        ref = T["reference"].reshape(3, nd)
+        g0 = blocks(sample_ops, ref, ns)
+        t0, tt0 = g0[:2], g0[2:]
+        c0 = np.cross(t0[0], t0[1], axis=0)
+        s0 = T["w"] * np.sqrt((c0 * c0).sum(0))
+        out = np.zeros((3, P))
+        if m.get("tension", "none") != "none":
+            law = getattr(fo, m["tension"])(m["shear"], m["bulk"])
+            out += fo.tension(law, t, tt, t0, tt0, s0)
+        if m.get("bending", 0.0):
+            cr = np.cross(t[0], t[1], axis=0)
+            det = np.sqrt((cr * cr).sum(0))
+            n, s = cr / det, np.tile(T["w"], nc) * det
+            gd = blocks(T["data_ops"], Xm, nd)
+            nd_ = np.cross(gd[0], gd[1], axis=0)
+            nd_ /= np.sqrt((nd_ * nd_).sum(0))
+            if m.get("preferred"):
+                gd0 = blocks(T["data_ops"], ref, nd)
+                n0 = np.cross(gd0[0], gd0[1], axis=0)
+                n0 /= np.sqrt((n0 * n0).sum(0))
+                h = fo.mean_curvature_sites(gd[:2], gd[2:], nd_, gd0[:2], gd0[2:], n0)
+                n0s = np.cross(t0[0], t0[1], axis=0)
+                n0s /= np.sqrt((n0s * n0s).sum(0))
+                ref_geo = (t0, tt0, n0s)
+            else:
+                h = fo.mean_curvature_sites(gd[:2], gd[2:], nd_)
+                ref_geo = (None, None, None)
+            hm = h.reshape(nc, nd)
+            hd = np.stack([ev(M, hm).reshape(P) for M in sample_ops])
+            out += fo.bending(m["bending"], t, tt, n, s, hd, *ref_geo)
+        return f + out.reshape(3, nc, ns)
+
     def step(self, u, p, xs, boundary=None):
         fd, orc, dt, nu = self.fd, self.orc, self.dt, self.nu
         geo = self.geometry(xs)
         F = [np.zeros(self.I[i].shape[0]) for i in range(3)]
-        for T, (y, _, _) in zip(self.types, geo):
-            f = -T["k"] * (y - T["ref"])
+        for T, X, g in zip(self.types, xs, geo):
+            y = g[0]
+            f = self.forces(T, X, g)
             Fi = orc.spread(self.mac, self.delta, y.reshape(-1), f.reshape(-1))
             F = [a + b for a, b in zip(F, Fi)]
         H = fd.advection(self.axes_of, MAC, u)

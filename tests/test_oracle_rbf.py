@@ -83,3 +83,45 @@ def test_long_double_evaluation_is_exact_on_integers():
     Et = rng.integers(-5, 6, (37, 20)).astype(np.float64)
     X = rng.integers(-5, 6, (4, 17)).astype(np.float64)
     assert np.array_equal(O.evaluate(Et, 17, X), X @ Et[:, :17].T)
+
+
+def _rbc_symbolic_derivs():
+    """Exact parameter derivatives of the RBC map (sympy), keyed like the
+    operators: 0, 1, (0, 0), (0, 1), (1, 1)."""
+    sp = pytest.importorskip("sympy")
+    t, p = sp.symbols("t p", real=True)
+    x, y, z0 = sp.cos(t) * sp.cos(p), sp.sin(t) * sp.cos(p), sp.sin(p)
+    r2 = x * x + y * y
+    z = sp.Rational(1, 2) * z0 * (sp.Rational(21, 100) + 2 * r2 - sp.Rational(112, 100) * r2 * r2)
+    X = sp.Matrix([I.RBC_RADIUS * x, I.RBC_RADIUS * z, I.RBC_RADIUS * y])
+    keys = {0: X.diff(t), 1: X.diff(p), (0, 0): X.diff(t, 2), (0, 1): X.diff(t).diff(p), (1, 1): X.diff(p, 2)}
+    return {k: sp.lambdify((t, p), list(v), "numpy") for k, v in keys.items()}
+
+
+def _call(f, th, ph):
+    return np.array([np.broadcast_to(np.asarray(c, dtype=float), th.shape) for c in f(th, ph)])
+
+
+def test_second_derivative_operators_differentiate_the_rbc_shape_exactly():
+    """The RBC map is a degree-5 polynomial of the unit-sphere embedding, so
+    the interpolant reproduces it and every derivative operator returns its
+    exact parameter derivatives (bases/operators.h:168-190)."""
+    dp = I.bauer_params(300)
+    sp_ = I.bauer_params(700)
+    ops = S.operators(dp, sp_, 5, seconds=True)
+    assert len(ops) == 6
+    X = I.rbc_shape(*dp)
+    exact = _rbc_symbolic_derivs()
+    for op, key in zip(ops[1:], [0, 1, (0, 0), (0, 1), (1, 1)]):
+        want = _call(exact[key], *sp_)
+        got = O.evaluate(op, 300, X)
+        assert np.abs(got - want).max() < 1e-9 * np.abs(want).max(), key
+    # data-to-data derivative operators (the data geometry of bending)
+    dops = S.data_operators(dp, 5)
+    for op, key in zip(dops, [0, 1, (0, 0), (0, 1), (1, 1)]):
+        want = _call(exact[key], *dp)
+        got = O.evaluate(op, 300, X)
+        assert np.abs(got - want).max() < 1e-8 * np.abs(want).max(), key
+    # derivatives annihilate constants
+    for op in ops[1:] + dops:
+        assert np.abs(op[:, :300].sum(1)).max() < 1e-9
