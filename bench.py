@@ -5,11 +5,16 @@ Workload (C4): 200 RBCs (2210 data sites, 8832 sample sites each) and 400
 platelets (605 / 1552) in a 32 x 64 x 32 um box, periodic in x and z, no-slip
 walls in y with the top wall moving at 1e-3 m/s, fd::mac{128} ->
 128 x 256 x 128 cells; cosine delta; water (nu = 1e-6 m^2/s, rho = 1 g/mL);
-dt = 0.1 us.  Cells are tethered to a 2% larger copy of their shape, so
-every step spreads nonzero forces.
+dt = 0.1 us.  The cells carry the membrane models of test/ib-whole.cc:208-215:
+RBCs Skalak tension (2.5e-3, 2.5e-1 dyn/cm) with bending (2e-12 erg),
+platelets neo-Hookean tension (0.1, 1 dyn/cm) with preferred-curvature
+bending (2e-11 erg); every cell starts inflated by 1% against its undeformed
+shape, so tension and bending forces are both active.
 
 A step is the library's whole IB timestep (ibm_solver_step_fixed): RBF
-geometry of all cells (one matrix-core launch), tether forces, spread of
+geometry of all cells (position, two first and three second derivatives:
+one matrix-core launch), the data-site geometry and mean curvature, its
+sample-site derivatives (two more launches), the membrane forces, spread of
 2,387,200 sample-site forces, advection, three Helmholtz solves and the
 pressure projection (multigrid), the FP64 divergence check, interpolation at
 684,000 data sites and their update.  The headline form ("mixed") runs the
@@ -36,8 +41,10 @@ NU = 10.0                # 1e-6 m^2/s
 RHO = 1e-3               # 1 g/mL
 DT = 1e-4                # 0.1 us
 WALL = 0.1               # 1e-3 m/s
-STIFFNESS = 0.02
-TETHER_SCALE = 1.02
+INFLATE = 1.01           # initial cells against their undeformed shapes
+# membrane moduli in library units (1 dyn/cm = 1, 1 erg = 1e6), test/ib-whole.cc:208-215
+RBC_MEMBRANE = ("skalak", 2.5e-3, 2.5e-1, 2e-6, False)
+PLT_MEMBRANE = ("neohookean", 1e-1, 1.0, 2e-5, True)
 TOL = 1e-10
 FRAC = 0.3               # share of the cells the CPU baseline times
 ADAPTIVE_STEPS = 8       # adaptive steps before the capture
@@ -58,13 +65,12 @@ def c4_cells(inputs, surfaces):
     L = (32 * UM, 64 * UM, 32 * UM)
     rp, pp = inputs.cell_poses(*L, 200, 400, 4)
     types = []
-    for shape, nd, ns, deg, poses in ((inputs.rbc_shape, 2210, 8832, 5, rp),
-                                      (inputs.platelet_shape, 605, 1552, 0, pp)):
-        ct = surfaces.CellType(shape, nd, ns, deg, inputs.bauer_params)
-        dref, sref = ct.data_reference(), ct.sample_reference()
-        X = np.stack([inputs.posed(dref, q) for q in poses], axis=1)                      # (3, nc, nd)
-        tether = np.stack([inputs.posed(sref, q, TETHER_SCALE) for q in poses], axis=1)   # (3, nc, ns)
-        types.append({"ct": ct, "X": X, "tether": tether, "nc": len(poses)})
+    for shape, nd, ns, deg, poses, mem in ((inputs.rbc_shape, 2210, 8832, 5, rp, RBC_MEMBRANE),
+                                           (inputs.platelet_shape, 605, 1552, 0, pp, PLT_MEMBRANE)):
+        ct = surfaces.CellType(shape, nd, ns, deg, inputs.bauer_params, membrane=True)
+        dref = ct.data_reference()
+        X = np.stack([inputs.posed(dref, q, INFLATE) for q in poses], axis=1)                 # (3, nc, nd)
+        types.append({"ct": ct, "X": X, "nc": len(poses), "membrane": mem, "reference": dref})
     return L, types
 
 
@@ -74,11 +80,13 @@ def make_solver(ibm, ctx, L, types, precision):
     cell_types = []
     for T in types:
         ct = T["ct"]
-        mats = [torch.from_numpy(m).to(dev) for m in (ct.E, ct.Et, ct.Ep)]
+        mats = [torch.from_numpy(m).to(dev) for m in (ct.E, ct.Et, ct.Ep, *ct.seconds)]
         op = ibm.RbfOperator(ctx, "f64", mats, ct.nd)
-        cell_types.append({"ops": op, "cells": T["nc"], "weights": torch.from_numpy(ct.w).to(dev),
-                           "tether": torch.from_numpy(np.ascontiguousarray(T["tether"].reshape(-1))).to(dev),
-                           "stiffness": STIFFNESS})
+        dop = ibm.RbfOperator(ctx, "f64", [torch.from_numpy(m).to(dev) for m in ct.data_ops], ct.nd)
+        cell_types.append({"ops": op, "data_ops": dop, "cells": T["nc"],
+                           "weights": torch.from_numpy(ct.w).to(dev),
+                           "membrane": ibm.membrane(*T["membrane"]),
+                           "reference": torch.from_numpy(np.ascontiguousarray(T["reference"].reshape(-1))).to(dev)})
     dom = [ibm.periodic(L[0]), ibm.dirichlet(L[1]), ibm.periodic(L[2])]
     s = ibm.Solver(ctx, dom, 128, DT, NU, density=RHO, tolerance=TOL, precision=precision, max_iterations=60,
                    cell_types=cell_types)
@@ -89,7 +97,7 @@ def make_solver(ibm, ctx, L, types, precision):
     return s, st, ub, cell_types
 
 
-KERNELS = ("spread_brick", "bin", "interp", "rbf_gemm", "geometry", "stencil_f64", "stencil_f32", "restrict",
+KERNELS = ("spread_brick", "bin", "interp", "rbf_gemm", "geometry", "membrane", "stencil_f64", "stencil_f32", "restrict",
            "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64", "residual_restrict",
            "prolong_residual")
 
@@ -144,64 +152,94 @@ def run_form(ibm, ctx, stream, L, types, precision, steps, warmup):
     return res
 
 
-def cpu_baseline(inputs, orc, fd, rbf, L, types, gpu_iters):
-    """The oracle (single-threaded restatement of the reference algorithms) on
-    bounded samples of one C4 step, scaled to a whole step."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(orc, fd, rbf, forces, step_mod, L, types, gpu_iters, repeats=5):
+    """The oracle (the reference algorithms restated on the CPU) on bounded
+    samples of one C4 step, each timed `repeats` times (median) on all the
+    host threads OMP_NUM_THREADS allows, and scaled to the whole step:
+    geometry (FP64 BLAS) and membrane forces of 10% of the cells, spread of
+    their sample sites and interpolation at their data sites (OpenMP), one
+    CSR multigrid V-cycle and one advection on a 1/8 sub-box (scipy, one
+    thread) times the GPU's V-cycle counts."""
+    from types import SimpleNamespace
     from threadpoolctl import threadpool_limits
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     spec = [(L[0], 0, (0.0, 0.0), (0.0, 0.0)), (L[1], 1, (1.0, 0.0), (1.0, 0.0)), (L[2], 0, (0.0, 0.0), (0.0, 0.0))]
     axes = orc.mac_axes(spec, 128)
+    frac = 0.1
+
+    def median_time(fn):
+        ts = []
+        out = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+
+    holder = SimpleNamespace(rbf=SimpleNamespace(evaluate=lambda M, K, Z: rbf.evaluate(M, K, Z, dtype=np.float64)),
+                             forces_mod=forces)
     parts = {}
-    with threadpool_limits(limits=1):
-        # (a) spread of 30% of the sample sites, interpolation at 30% of the data sites
-        xs, fs, xd = [], [], []
+    with threadpool_limits(limits=threads):
+        samples = []
         for T in types:
-            ct, nc = T["ct"], T["nc"]
-            k = max(int(round(nc * FRAC)), 1)
-            y = rbf.evaluate(ct.E, ct.nd, T["X"][:, :k].reshape(3 * k, ct.nd), dtype=np.float64)
-            xs.append(y.reshape(3, -1))
-            fs.append((-STIFFNESS * (y.reshape(3, k, -1) - T["tether"][:, :k])).reshape(3, -1))
-            xd.append(T["X"][:, :k].reshape(3, -1))
-        x = np.ascontiguousarray(np.concatenate(xs, axis=1).reshape(-1))
-        f = np.ascontiguousarray(np.concatenate(fs, axis=1).reshape(-1))
-        t0 = time.perf_counter()
-        fe = orc.spread(axes, "cosine", x, f)
-        t1 = time.perf_counter()
-        orc.interpolate(axes, "cosine", np.ascontiguousarray(np.concatenate(xd, axis=1).reshape(-1)), fe)
-        t2 = time.perf_counter()
-        parts["spread"] = (t1 - t0) / FRAC
-        parts["interpolate"] = (t2 - t1) / FRAC
-        # (b) RBF geometry (3 operators) for 30% of the cells, FP64 BLAS on one thread
-        t3 = time.perf_counter()
-        for T in types:
-            ct, k = T["ct"], max(int(round(T["nc"] * FRAC)), 1)
-            Xk = T["X"][:, :k].reshape(3 * k, ct.nd)
-            for M in (ct.E, ct.Et, ct.Ep):
-                rbf.evaluate(M, ct.nd, Xk, dtype=np.float64)
-        parts["rbf"] = (time.perf_counter() - t3) / FRAC
-        # (c) one V-cycle and one advection on a 1/8 sub-box (64 x 128 x 64), scaled x8
+            ct, k = T["ct"], max(int(round(T["nc"] * frac)), 1)
+            tension, shear, bulk, bending, preferred = T["membrane"]
+            Ts = {"E": ct.E, "Et": ct.Et, "Ep": ct.Ep, "seconds": ct.seconds, "data_ops": ct.data_ops,
+                  "nd": ct.nd, "ns": ct.ns, "nc": k, "w": ct.w, "k": 0.0, "reference": T["reference"].reshape(-1),
+                  "membrane": {"tension": tension, "shear": shear, "bulk": bulk, "bending": bending,
+                               "preferred": preferred}}
+            samples.append((Ts, np.ascontiguousarray(T["X"][:, :k].reshape(-1))))
+
+        def geometry_and_forces():
+            out = []
+            for Ts, X in samples:
+                y = holder.rbf.evaluate(Ts["E"], Ts["nd"], X.reshape(3 * Ts["nc"], Ts["nd"]))
+                f = step_mod.Step.forces(holder, Ts, X, (y.reshape(3, Ts["nc"], Ts["ns"]),))
+                out.append((y.reshape(3, -1), f.reshape(3, -1)))
+            return out
+        parts["geometry_forces"], yf = median_time(geometry_and_forces)
+        x = np.ascontiguousarray(np.concatenate([a for a, _ in yf], axis=1).reshape(-1))
+        f = np.ascontiguousarray(np.concatenate([b for _, b in yf], axis=1).reshape(-1))
+        xd = np.ascontiguousarray(np.concatenate([X.reshape(3, -1) for _, X in samples], axis=1).reshape(-1))
+        parts["spread"], fe = median_time(lambda: orc.spread(axes, "cosine", x, f))
+        parts["interpolate"], _ = median_time(lambda: orc.interpolate(axes, "cosine", xd, fe))
+        for key in ("geometry_forces", "spread", "interpolate"):
+            parts[key] /= frac
         sub = [(L[0] / 2, 0, (0.0, 0.0), (0.0, 0.0)), (L[1] / 2, 1, (1.0, 0.0), (1.0, 0.0)),
                (L[2] / 2, 0, (0.0, 0.0), (0.0, 0.0))]
         af = lambda r, al=(0.0, 0.5, 0.5): [orc.grid_axes(sub, r, list(al))[j] for j in range(3)]
         mg = fd.Multigrid(af, 64, 1.0, -DT * NU / 2, order="x-fastest")
         b = np.random.default_rng(0).standard_normal(mg.levels[0]["A"].shape[0])
-        t4 = time.perf_counter()
-        mg.vcycle(b)
-        parts["v_cycle"] = (time.perf_counter() - t4) * 8
+        t_v, _ = median_time(lambda: mg.vcycle(b))
+        parts["v_cycle"] = 8 * t_v
         MAC = [[0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]]
         axf = lambda al: [orc.grid_axes(sub, 64, list(al))[j] for j in range(3)]
         u = [np.random.default_rng(1).standard_normal(np.prod([a.points for a in axf(al)])) for al in MAC]
-        t5 = time.perf_counter()
-        fd.advection(axf, MAC, u)
-        parts["advection"] = (time.perf_counter() - t5) * 8
+        t_a, _ = median_time(lambda: fd.advection(axf, MAC, u))
+        parts["advection"] = 8 * t_a
     cycles = sum(gpu_iters)
-    total = parts["spread"] + parts["interpolate"] + parts["rbf"] + parts["advection"] + cycles * parts["v_cycle"]
-    return {"value": 1.0 / total, "unit": "timesteps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle timed on samples of one C4 step and scaled to the whole step: spread of 30% of the "
-                       f"sample sites ({parts['spread']:.1f} s scaled), interpolation at 30% of the data sites "
-                       f"({parts['interpolate']:.1f} s), FP64 RBF geometry of 30% of the cells "
-                       f"({parts['rbf']:.1f} s), advection and one CSR multigrid V-cycle on a 64x128x64 sub-box "
-                       f"x8 ({parts['advection']:.1f} s, {parts['v_cycle']:.1f} s per cycle x {cycles} cycles, "
-                       f"the GPU's counts); total {total:.0f} s per step")}
+    total = (parts["geometry_forces"] + parts["spread"] + parts["interpolate"] + parts["advection"] +
+             cycles * parts["v_cycle"])
+    return {"value": 1.0 / total, "unit": "timesteps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu": _cpu_model(), "repeats": repeats,
+            "sample": (f"oracle on {threads} threads, median of {repeats} runs per part, scaled to one C4 step: "
+                       f"RBF geometry and membrane forces of 10% of the cells "
+                       f"({parts['geometry_forces']:.2f} s scaled), spread of their sample sites "
+                       f"({parts['spread']:.2f} s) and interpolation at their data sites "
+                       f"({parts['interpolate']:.2f} s), advection and one CSR multigrid V-cycle on a 64x128x64 "
+                       f"sub-box x8 ({parts['advection']:.2f} s, {parts['v_cycle']:.2f} s per cycle x {cycles} "
+                       f"cycles, the GPU's counts; scipy, one thread); total {total:.1f} s per step")}
 
 
 # Kernel-name prefixes of the counter files for each timed kernel group.
@@ -275,7 +313,38 @@ def recorded_traffic(kernel):
     return None
 
 
+def touched_points(L, types):
+    """Sum over the three MAC components of the grid points any data-site
+    stencil touches (T_c of the interpolation's algorithmic bytes
+    48 n + 8 sum_c T_c), from the initial data sites: stencil base
+    floor(x / h - a) per axis (a = 0 on the face axis, 1/2 elsewhere), four
+    points from base - 1, periodic x and z wrapped, walled y clamped."""
+    h = L[0] / 128
+    cells = [int(round(L[j] / h)) for j in range(3)]
+    x = np.concatenate([T["X"].reshape(3, -1) for T in types], axis=1)
+    total = 0
+    for c in range(3):
+        pts = [cells[j] - (1 if (j == 1 and c == 1) else 0) for j in range(3)]
+        idx = []
+        for j in range(3):
+            a = 0.0 if j == c else 0.5
+            base = np.floor(x[j] / h - a).astype(np.int64)
+            if j == 1 and c == 1:
+                base -= 1          # the v grid's first point is the first interior face
+            cols = base[:, None] + np.arange(-1, 3)[None, :]
+            cols = np.clip(cols, 0, pts[j] - 1) if j == 1 else np.mod(cols, pts[j])
+            idx.append(cols)
+        lin = (idx[0][:, None, None, :] + pts[0] * (idx[1][:, None, :, None] + pts[1] * idx[2][:, :, None, None]))
+        mask = np.zeros(pts[0] * pts[1] * pts[2], dtype=bool)
+        mask[lin.reshape(-1)] = True
+        total += int(mask.sum())
+    return total
+
+
 def main():
+    # The CPU oracle's OpenMP threads sleep between parallel regions instead
+    # of spinning beside the single-threaded scipy parts of the baseline.
+    os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -346,16 +415,38 @@ def main():
 
     n_sample = sum(T["ct"].ns * T["nc"] for T in types)
     n_data = sum(T["ct"].nd * T["nc"] for T in types)
-    t_si = sum(kern[k]["ms_per_step"] for k in ("spread_brick", "interp") if k in kern)
-    t_si += kern.get("bin", {"ms_per_step": 0})["ms_per_step"]
-    gpts = 3 * 64 * (n_sample + n_data) / (t_si * 1e-3) / 1e9
+    t_spread = kern.get("spread_brick", {"ms_per_step": 0})["ms_per_step"] + kern.get("bin", {"ms_per_step": 0})["ms_per_step"]
+    t_interp = kern.get("interp", {"ms_per_step": 0})["ms_per_step"]
+    touched = touched_points(L, types)
+    interp_bytes = 48.0 * n_data + 8.0 * touched
+    rates = {
+        "spread_gpts": 3 * 64 * n_sample / (t_spread * 1e-3) / 1e9 if t_spread else None,
+        "interp_gpts": 3 * 64 * n_data / (t_interp * 1e-3) / 1e9 if t_interp else None,
+        "interp_GB/s": interp_bytes / (t_interp * 1e-3) / 1e9 if t_interp else None,
+        "interp_algorithmic_bytes": interp_bytes,
+        "interp_touched_points": touched,
+    }
+    if "interp" in per_kernel and t_interp:
+        per_kernel["interp"]["GB/s"] = rates["interp_GB/s"]
+    errors = None
+    if plain:
+        a, b = mixed["state"], plain["state"]
+
+        def rel(x, y):
+            x, y = x.double(), y.double()
+            return float((x - y).abs().max() / y.abs().max().clamp_min(1e-300))
+        errors = {"velocity": max(rel(a.u[i], b.u[i]) for i in range(3)), "pressure": rel(a.p, b.p),
+                  "positions": max(rel(a.xs[k], b.xs[k]) for k in range(len(a.xs))),
+                  "steps": ADAPTIVE_STEPS + args.warmup + args.steps + 3}
 
     cpu = None
     if not args.no_cpu_baseline:
         orc = _load("orc", os.path.join(ROOT, "oracle", "orc.py"))
         fd = _load("orc_fd", os.path.join(ROOT, "oracle", "fd.py"))
         rbf = _load("orc_rbf", os.path.join(ROOT, "oracle", "rbf.py"))
-        cpu = cpu_baseline(inputs, orc, fd, rbf, L, types, mixed["info"]["iterations"])
+        forces = _load("orc_forces", os.path.join(ROOT, "oracle", "forces.py"))
+        step_mod = _load("orc_step", os.path.join(ROOT, "oracle", "step.py"))
+        cpu = cpu_baseline(orc, fd, rbf, forces, step_mod, L, types, mixed["info"]["iterations"])
 
     line = {
         "metric": "IB timesteps/s at 128³ + 600 cells; spread/interp Gpts/s; % HBM peak",
@@ -378,7 +469,8 @@ def main():
         "speedup_vs_plain_fp64": plain["ms_graph"] / mixed["ms_graph"] if plain else None,
         "eager_ms_per_step": mixed["eager_ms"],
         "solves": mixed["info"],
-        "spread_interp_gpts": gpts,
+        "spread_interp": rates,
+        "mixed_vs_fp64_relative_error": errors,
         "kernels": per_kernel,
         "roofline": roofline,
         "cpu_baseline": cpu,
