@@ -74,7 +74,7 @@ def c4_cells(inputs, surfaces):
     return L, types
 
 
-def make_solver(ibm, ctx, L, types, precision):
+def make_solver(ibm, ctx, L, types, precision, tol=TOL):
     import torch
     dev = torch.device("cuda", 0)
     cell_types = []
@@ -88,7 +88,7 @@ def make_solver(ibm, ctx, L, types, precision):
                            "membrane": ibm.membrane(*T["membrane"]),
                            "reference": torch.from_numpy(np.ascontiguousarray(T["reference"].reshape(-1))).to(dev)})
     dom = [ibm.periodic(L[0]), ibm.dirichlet(L[1]), ibm.periodic(L[2])]
-    s = ibm.Solver(ctx, dom, 128, DT, NU, density=RHO, tolerance=TOL, precision=precision, max_iterations=60,
+    s = ibm.Solver(ctx, dom, 128, DT, NU, density=RHO, tolerance=tol, precision=precision, max_iterations=60,
                    cell_types=cell_types)
     g, _ = s.grids()
     ub = [torch.zeros(g[i].size, dtype=torch.float64, device=dev) for i in range(3)]
@@ -97,7 +97,7 @@ def make_solver(ibm, ctx, L, types, precision):
     return s, st, ub, cell_types
 
 
-KERNELS = ("spread_brick", "bin", "interp", "rbf_gemm", "geometry", "membrane", "stencil_f64", "stencil_f32", "restrict",
+KERNELS = ("spread_brick", "spread_tile", "spread_merge", "bin", "interp", "rbf_gemm", "geometry", "membrane", "stencil_f64", "stencil_f32", "restrict",
            "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64", "residual_restrict",
            "prolong_residual")
 
@@ -248,7 +248,8 @@ PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "che
                "restrict": "::restrict_kernel<", "residual_restrict": "residual_restrict_kernel",
                "prolong": "::prolong_add_kernel", "prolong_residual": "prolong_residual_kernel",
                "rbf_gemm": "rbf_gemm_kernel",
-               "spread_brick": "spread_brick_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
+               "spread_brick": "spread_brick_kernel", "spread_tile": "tile_spread_kernel",
+               "spread_merge": "tile_merge_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
 COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
 
 
@@ -415,7 +416,8 @@ def main():
 
     n_sample = sum(T["ct"].ns * T["nc"] for T in types)
     n_data = sum(T["ct"].nd * T["nc"] for T in types)
-    t_spread = kern.get("spread_brick", {"ms_per_step": 0})["ms_per_step"] + kern.get("bin", {"ms_per_step": 0})["ms_per_step"]
+    t_spread = sum(kern.get(k, {"ms_per_step": 0})["ms_per_step"]
+                   for k in ("spread_brick", "spread_tile", "spread_merge", "bin"))
     t_interp = kern.get("interp", {"ms_per_step": 0})["ms_per_step"]
     touched = touched_points(L, types)
     interp_bytes = 48.0 * n_data + 8.0 * touched

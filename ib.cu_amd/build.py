@@ -80,7 +80,8 @@ def build(verbose=True, incremental=False):
     with ThreadPoolExecutor(max_workers=max(jobs, 1)) as ex:
         objs = list(ex.map(lambda s: _compile(s, incremental, headers), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

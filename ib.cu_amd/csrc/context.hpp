@@ -43,6 +43,9 @@ enum scratch_slot {
     SLOT_FLAGS,
     SLOT_REDUCE,        // reduction partials of the solvers (REDUCE_PARTS + 8 doubles)
     SLOT_RBF_X,         // packed data-site coordinates of an RBF evaluation
+    SLOT_TILES,         // fixed-point tiles of the tile spread (3 x 12^3 per brick)
+    SLOT_TILE_FLAGS,    // per-brick "has markers" flags and the tile scale
+    SLOT_BOX_OFF,       // marker offsets of the boxes of a batched spread / interpolation
     SLOT_STEP_BASE,     // timestep workspaces; slots from here on belong to the timestep solver
     SLOT_COUNT = 64
 };
@@ -84,6 +87,12 @@ ibm_status membrane_forces(ibm_ctx* ctx, hipStream_t stream, const ibm_membrane*
                            const double* table, const double* y, const double* yref, double beta, double* f);
 // Number of operators an RBF object holds.
 int rbf_nops(const ibm_rbf* op);
+
+// Whether the tile spread takes these MAC grids, and the timestep's spread
+// of nsets marker sets by the tile (tiles = true) or the brick kernel.
+bool tiles_usable(ibm_ctx* ctx, const ibm_grid* grids);
+ibm_status spread_sets_step(ibm_ctx* ctx, const ibm_grid* grids, int delta, bool tiles, int nsets, const int* n,
+                            const double* const* x, const double* const* f, double* const* fe);
 
 } // namespace ibm
 

@@ -128,6 +128,37 @@ ins_boundary_op make_ins_boundary(const ibm_grid& g, int dim, int order);
 // The differential (kind 0) or average (kind 1) of grid `src` along `view`.
 ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst);
 
+// Block targets of the z-marching kernels (smoothing, residual + restriction,
+// prolongation + residual) per grid: the number of blocks the z chunking aims
+// at.  0 means the built-in default; the multigrid's autotuner times the
+// candidates on each level and records the fastest here (process-wide, keyed
+// by the fine grid's shape, periodicity and element size).  The tile (wide
+// for grids of MARCH_WIDE_POINTS points or more, narrow below) stays fixed
+// by the grid size: a target only moves chunk boundaries, so every point is
+// computed by the same code either way and results do not depend on tuning.
+enum { TUNE_CHEB = 0, TUNE_RR = 1, TUNE_PR = 2 };
+int tuned_target(int kind, int elem, int nx, int ny, int nz, int per);
+void set_tuned_target(int kind, int elem, int nx, int ny, int nz, int per, int target);
+bool target_tuned(int kind, int elem, int nx, int ny, int nz, int per);
+void clear_tuned_targets();
+// Whether launches on an nx x ny x nz grid use the wide tile.
+bool march_wide(size_t points);
+// The built-in block targets.
+constexpr int MARCH_DEFAULT_TARGET = 512;
+constexpr int TRANSFER_DEFAULT_TARGET = 1024;
+
+// Batched forms over nbatch copies of one grid stored back to back.
+template <typename T>
+ibm_status launch_batch_stencil(ibm_ctx* ctx, const op3<T>& A, int nbatch, T alpha, const T* x, T beta, const T* y,
+                                T gamma, const T* z, T* out);
+template <typename T>
+ibm_status launch_batch_restrict(ibm_ctx* ctx, const transfer3<T>& R, int nbatch, const T* fine, T* coarse);
+template <typename T>
+ibm_status launch_batch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, int nbatch, const T* coarse, T* fine);
+// out[b] = sum of copy b of x (times y when given), FP64, fixed order
+template <typename T>
+ibm_status launch_batch_dot(ibm_ctx* ctx, size_t n, int nbatch, const T* x, const T* y, double* out);
+
 constexpr int REDUCE_PARTS = 1024;
 
 } // namespace ibm

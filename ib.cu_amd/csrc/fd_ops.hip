@@ -7,6 +7,8 @@
 // stencil pass moves only the vectors (16 B per point in FP64, 8 in FP32,
 // +8/+4 per extra input vector) instead of ~100 B of CSR data per row.
 #include <hip/hip_runtime.h>
+#include <map>
+#include <mutex>
 #include <mutex>
 #include <algorithm>
 #include <utility>
@@ -196,8 +198,53 @@ constexpr size_t MARCH_WIDE_POINTS = (size_t) 1 << 19;   // grids from 512K poin
 constexpr int MARCH_NV = 8;      // v planes in the ring (7 alive in a phase)
 constexpr int MARCH_NS = 4;      // s1 (and s2) planes alive in a phase
 constexpr int MARCH_LEAD = 9;    // phases between loading a v plane and storing its output
-constexpr int MARCH_TARGET_BLOCKS = 512;
+constexpr int MARCH_TARGET_BLOCKS = MARCH_DEFAULT_TARGET;
 constexpr int MARCH_PF = 2;      // phases between loading a plane's operands and using them
+
+namespace {
+struct shape_key {
+    int kind, elem, nx, ny, nz, per;
+    bool operator<(const shape_key& o) const
+    {
+        const int a[6] = {kind, elem, nx, ny, nz, per}, b[6] = {o.kind, o.elem, o.nx, o.ny, o.nz, o.per};
+        for (int i = 0; i < 6; ++i)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+std::mutex tune_lock;
+std::map<shape_key, int> tune_table;
+} // namespace
+
+int tuned_target(int kind, int elem, int nx, int ny, int nz, int per)
+{
+    std::lock_guard<std::mutex> guard(tune_lock);
+    auto it = tune_table.find(shape_key{kind, elem, nx, ny, nz, per});
+    return it == tune_table.end() ? 0 : it->second;
+}
+
+void set_tuned_target(int kind, int elem, int nx, int ny, int nz, int per, int target)
+{
+    std::lock_guard<std::mutex> guard(tune_lock);
+    tune_table[shape_key{kind, elem, nx, ny, nz, per}] = target;
+}
+
+void clear_tuned_targets()
+{
+    std::lock_guard<std::mutex> guard(tune_lock);
+    tune_table.clear();
+}
+
+bool target_tuned(int kind, int elem, int nx, int ny, int nz, int per)
+{
+    std::lock_guard<std::mutex> guard(tune_lock);
+    return tune_table.count(shape_key{kind, elem, nx, ny, nz, per}) != 0;
+}
+
+bool march_wide(size_t points) { return points >= MARCH_WIDE_POINTS; }
+
+static int per_mask(const op3<float>& A) { return (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0); }
+static int per_mask(const op3<double>& A) { return (A.a[0].periodic ? 1 : 0) | (A.a[1].periodic ? 2 : 0) | (A.a[2].periodic ? 4 : 0); }
 static_assert(MARCH_NV % MARCH_NS == 0 && MARCH_NV % MARCH_PF == 0, "ring periods must divide the unroll");
 
 // Global index of coordinate o - h + c along an axis: wrapped on a periodic
@@ -664,7 +711,7 @@ struct rr_tile {
     static constexpr int CX = WIDE ? 32 : 8, CY = 4;
     static constexpr int THREADS = WIDE ? 256 : 128;
 };
-constexpr int RR_TARGET_BLOCKS = 1024;
+constexpr int RR_TARGET_BLOCKS = TRANSFER_DEFAULT_TARGET;
 
 // Fine tap of coarse index c along an axis: region offset from fine
 // coordinate o (unwrapped on periodic axes, folded on walled ones) and weight.
@@ -905,7 +952,7 @@ struct pr_tile {
     static constexpr int X = WIDE ? 64 : 16, Y = 8;
     static constexpr int THREADS = WIDE ? 256 : 128;
 };
-constexpr int PR_TARGET_BLOCKS = 1024;
+constexpr int PR_TARGET_BLOCKS = TRANSFER_DEFAULT_TARGET;
 
 // Coarse taps of fine index j along an axis (prolong_at's ca, cb), as region
 // offsets from coarse coordinate o (unwrapped on periodic axes, folded on
@@ -1590,14 +1637,15 @@ static void cheb_launch(int per, int blocks, hipStream_t stream, const op3<T>& A
 
 template <typename T, bool WIDE>
 static void cheb_configure(const op3<T>& A, const double w[4], const T* v, T* out, const T* addsrc, bool add,
-                           double* acc, hipStream_t stream)
+                           double* acc, hipStream_t stream, int target)
 {
     constexpr int TX = march_tile<T, WIDE>::X, TY = march_tile<T, WIDE>::Y;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
-    // Enough z chunks for about MARCH_TARGET_BLOCKS blocks; each chunk costs
-    // MARCH_LEAD phases of pipeline fill.
-    int zchunk = (int) (((long long) nz * tiles_xy + MARCH_TARGET_BLOCKS - 1) / MARCH_TARGET_BLOCKS);
+    // Enough z chunks for about `target` blocks (MARCH_TARGET_BLOCKS unless
+    // tuned); each chunk costs MARCH_LEAD phases of pipeline fill.
+    if (target <= 0) target = MARCH_TARGET_BLOCKS;
+    int zchunk = (int) (((long long) nz * tiles_xy + target - 1) / target);
     zchunk = std::min(nz, std::max(zchunk, march_tile<T, WIDE>::MIN_Z));
     const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
     const int blocks = (nblocks + 7) / 8 * 8;
@@ -1632,8 +1680,9 @@ ibm_status launch_cheb_fused(ibm_ctx* ctx, const op3<T>& A, const double w[4], c
     // v read, out written (read too when adding; FP64 read-modify-write with acc)
     const double bytes = n * (sizeof(T) + (acc ? 16.0 + sizeof(T) : (add ? 2.0 : 1.0) * sizeof(T)));
     prof_scope prof(ctx, sizeof(T) == 8 ? "chebyshev_f64" : "chebyshev_f32", bytes);
-    if ((size_t) nx * ny * nz >= MARCH_WIDE_POINTS) cheb_configure<T, true>(A, w, v, out, addsrc, add, acc, ctx->stream);
-    else cheb_configure<T, false>(A, w, v, out, addsrc, add, acc, ctx->stream);
+    const int target = tuned_target(TUNE_CHEB, (int) sizeof(T), nx, ny, nz, per_mask(A));
+    if (march_wide((size_t) nx * ny * nz)) cheb_configure<T, true>(A, w, v, out, addsrc, add, acc, ctx->stream, target);
+    else cheb_configure<T, false>(A, w, v, out, addsrc, add, acc, ctx->stream, target);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -1669,12 +1718,13 @@ ibm_status launch_restrict(ibm_ctx* ctx, const transfer3<T>& R, const T* fine, T
 
 template <typename T, bool WIDE>
 static void rr_configure(const op3<T>& A, const transfer3<T>& R, const T* x, const T* b, T* coarse,
-                         hipStream_t stream)
+                         hipStream_t stream, int target)
 {
     constexpr int CX = rr_tile<T, WIDE>::CX, CY = rr_tile<T, WIDE>::CY;
     const int ncx = R.a[0].nc, ncy = R.a[1].nc, ncz = R.a[2].nc;
     const int tiles_x = (ncx + CX - 1) / CX, tiles_xy = tiles_x * ((ncy + CY - 1) / CY);
-    int kchunk = (int) (((long long) ncz * tiles_xy + RR_TARGET_BLOCKS - 1) / RR_TARGET_BLOCKS);
+    if (target <= 0) target = RR_TARGET_BLOCKS;
+    int kchunk = (int) (((long long) ncz * tiles_xy + target - 1) / target);
     kchunk = std::min(ncz, std::max(kchunk, 2));
     const int nblocks = tiles_xy * ((ncz + kchunk - 1) / kchunk);
     const int blocks = (nblocks + 7) / 8 * 8;
@@ -1712,20 +1762,22 @@ ibm_status launch_residual_restrict(ibm_ctx* ctx, const op3<T>& A, const transfe
             return fail(ctx, IBM_ERR_ARG, "residual/restriction: operator and transfer grids differ");
     // x and b read, coarse written
     prof_scope prof(ctx, "residual_restrict", (double) sizeof(T) * (2 * nf + nc));
-    if (nf >= MARCH_WIDE_POINTS) rr_configure<T, true>(A, R, x, b, coarse, ctx->stream);
-    else rr_configure<T, false>(A, R, x, b, coarse, ctx->stream);
+    const int target = tuned_target(TUNE_RR, (int) sizeof(T), A.a[0].n, A.a[1].n, A.a[2].n, per_mask(A));
+    if (march_wide(nf)) rr_configure<T, true>(A, R, x, b, coarse, ctx->stream, target);
+    else rr_configure<T, false>(A, R, x, b, coarse, ctx->stream, target);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
 
 template <typename T, bool WIDE>
 static void pr_configure(const op3<T>& A, const transfer3<T>& P, const T* e, const T* x, const T* b, T* xn, T* r,
-                         hipStream_t stream)
+                         hipStream_t stream, int target)
 {
     constexpr int TX = pr_tile<T, WIDE>::X, TY = pr_tile<T, WIDE>::Y;
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     const int tiles_x = (nx + TX - 1) / TX, tiles_xy = tiles_x * ((ny + TY - 1) / TY);
-    int zchunk = (int) (((long long) nz * tiles_xy + PR_TARGET_BLOCKS - 1) / PR_TARGET_BLOCKS);
+    if (target <= 0) target = PR_TARGET_BLOCKS;
+    int zchunk = (int) (((long long) nz * tiles_xy + target - 1) / target);
     zchunk = std::min(nz, std::max(zchunk, 2));
     const int nblocks = tiles_xy * ((nz + zchunk - 1) / zchunk);
     const int blocks = (nblocks + 7) / 8 * 8;
@@ -1763,8 +1815,9 @@ ibm_status launch_prolong_residual(ibm_ctx* ctx, const op3<T>& A, const transfer
             return fail(ctx, IBM_ERR_ARG, "prolongation/residual: operator and transfer grids differ");
     // e, x, b read, xn and r written
     prof_scope prof(ctx, "prolong_residual", (double) sizeof(T) * (nc + 4 * nf));
-    if (nf >= MARCH_WIDE_POINTS) pr_configure<T, true>(A, P, e, x, b, xn, r, ctx->stream);
-    else pr_configure<T, false>(A, P, e, x, b, xn, r, ctx->stream);
+    const int target = tuned_target(TUNE_PR, (int) sizeof(T), A.a[0].n, A.a[1].n, A.a[2].n, per_mask(A));
+    if (march_wide(nf)) pr_configure<T, true>(A, P, e, x, b, xn, r, ctx->stream, target);
+    else pr_configure<T, false>(A, P, e, x, b, xn, r, ctx->stream, target);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -1977,6 +2030,129 @@ ibm_status shifted_grid(const ibm_grid& src, int view, ibm_grid* dst)
     al[view] = sh - std::floor(sh);
     return ibm_grid_make(dims, src.dims, src.refinement, al, dst);
 }
+
+
+// ---------------------------------------------------------------------------
+// Batched operators over nbatch independent copies of one grid, stored one
+// after another (vector of copy b at offset b * n): the stencil, restriction
+// and prolongation of the kernels above, each point by the same device
+// functions (stencil_at, restrict_at, prolong_at), one launch for all copies.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+batch_stencil_kernel(op3<T> A, int nbatch, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y, T gamma,
+                     const T* __restrict__ z, T* __restrict__ out)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n;
+    const size_t n = (size_t) nx * ny * A.a[2].n;
+    const size_t total = n * nbatch;
+    for (size_t g = (size_t) blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t) gridDim.x * blockDim.x) {
+        const size_t b = g / n, t = g - b * n;
+        const size_t o = b * n;
+        const int i = (int) (t % nx), j = (int) ((t / nx) % ny), k = (int) (t / ((size_t) nx * ny));
+        out[g] = stencil_at(A, alpha, x + o, beta, y ? y + o : y, gamma, z ? z + o : z, i, j, k);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+batch_restrict_kernel(transfer3<T> R, int nbatch, const T* __restrict__ fine, T* __restrict__ coarse)
+{
+    const size_t nc = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc;
+    const size_t nf = (size_t) R.a[0].nf * R.a[1].nf * R.a[2].nf;
+    const size_t total = nc * nbatch;
+    for (size_t g = (size_t) blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t) gridDim.x * blockDim.x) {
+        const size_t b = g / nc, t = g - b * nc;
+        coarse[g] = restrict_at(R, fine + b * nf, (unsigned) t);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+batch_prolong_add_kernel(transfer3<T> P, int nbatch, const T* __restrict__ coarse, T* __restrict__ fine)
+{
+    const size_t nc = (size_t) P.a[0].nc * P.a[1].nc * P.a[2].nc;
+    const size_t nf = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf;
+    const size_t total = nf * nbatch;
+    for (size_t g = (size_t) blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t) gridDim.x * blockDim.x) {
+        const size_t b = g / nf, t = g - b * nf;
+        fine[g] += prolong_at(P, coarse + b * nc, (unsigned) t);
+    }
+}
+
+// Per copy: out[b] = sum over its n entries of x (x * y when y is given),
+// in FP64 with a fixed summation order (one workgroup per copy).
+template <typename T>
+__global__ void __launch_bounds__(256)
+batch_dot_kernel(size_t n, const T* __restrict__ x, const T* __restrict__ y, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const size_t o = (size_t) blockIdx.x * n;
+    double acc = 0;
+    for (size_t t = threadIdx.x; t < n; t += blockDim.x) {
+        double v = (double) x[o + t];
+        acc += y ? v * (double) y[o + t] : v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int) threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+static int batch_blocks(size_t total)
+{
+    size_t b = (total + 255) / 256;
+    return (int) (b < 65536 ? (b ? b : 1) : 65536);
+}
+
+template <typename T>
+ibm_status launch_batch_stencil(ibm_ctx* ctx, const op3<T>& A, int nbatch, T alpha, const T* x, T beta, const T* y,
+                                T gamma, const T* z, T* out)
+{
+    size_t total = (size_t) A.a[0].n * A.a[1].n * A.a[2].n * nbatch;
+    prof_scope prof(ctx, sizeof(T) == 8 ? "batch_stencil_f64" : "batch_stencil_f32",
+                    (double) sizeof(T) * total * (2 + (y ? 1 : 0) + (z ? 1 : 0)));
+    batch_stencil_kernel<T><<<batch_blocks(total), 256, 0, ctx->stream>>>(A, nbatch, alpha, x, beta, y, gamma, z, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_batch_restrict(ibm_ctx* ctx, const transfer3<T>& R, int nbatch, const T* fine, T* coarse)
+{
+    size_t total = (size_t) R.a[0].nc * R.a[1].nc * R.a[2].nc * nbatch;
+    batch_restrict_kernel<T><<<batch_blocks(total), 256, 0, ctx->stream>>>(R, nbatch, fine, coarse);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_batch_prolong_add(ibm_ctx* ctx, const transfer3<T>& P, int nbatch, const T* coarse, T* fine)
+{
+    size_t total = (size_t) P.a[0].nf * P.a[1].nf * P.a[2].nf * nbatch;
+    batch_prolong_add_kernel<T><<<batch_blocks(total), 256, 0, ctx->stream>>>(P, nbatch, coarse, fine);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+template <typename T>
+ibm_status launch_batch_dot(ibm_ctx* ctx, size_t n, int nbatch, const T* x, const T* y, double* out)
+{
+    batch_dot_kernel<T><<<nbatch, 256, 0, ctx->stream>>>(n, x, y, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+#define IBM_INSTANTIATE_BATCH(T)                                                                              \
+    template ibm_status launch_batch_stencil<T>(ibm_ctx*, const op3<T>&, int, T, const T*, T, const T*, T, const T*, T*); \
+    template ibm_status launch_batch_restrict<T>(ibm_ctx*, const transfer3<T>&, int, const T*, T*);          \
+    template ibm_status launch_batch_prolong_add<T>(ibm_ctx*, const transfer3<T>&, int, const T*, T*);       \
+    template ibm_status launch_batch_dot<T>(ibm_ctx*, size_t, int, const T*, const T*, double*);
+IBM_INSTANTIATE_BATCH(double)
+IBM_INSTANTIATE_BATCH(float)
 
 // Gershgorin bounds over row classes (see fd_ops.hpp).
 void gershgorin_bounds(const ibm_grid& g, double sigma, double tau, double* lo, double* hi)

@@ -19,6 +19,7 @@
 //     x += V(b - A x), with the residual, its norm and the update in FP64.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 #include "context.hpp"
@@ -354,6 +355,98 @@ static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* 
     return IBM_OK;
 }
 
+// Block-target autotuning of the z-marching kernels.  For every level above
+// the coarse end whose grid the process has not timed yet, each kernel
+// (smoothing, residual + restriction, prolongation + residual) runs on the
+// level's zeroed buffers with z chunkings aimed at 256 ... 2048 blocks, and
+// the fastest target is recorded for the grid.  The tile stays the one the
+// grid size selects, so every point is computed by the same code whatever
+// the target: tuning changes timings, never results.
+static std::atomic<bool> autotune_on{true};
+constexpr int TUNE_TARGETS[] = {256, 512, 1024, 2048};
+constexpr int TUNE_REPS = 4;
+constexpr float TUNE_MARGIN = 0.97f;   // a target must beat the default's time by 3%
+
+static int per_of(const ibm_grid& g)
+{
+    return (g.axis[0].solid ? 0 : 1) | (g.axis[1].solid ? 0 : 2) | (g.axis[2].solid ? 0 : 4);
+}
+
+template <typename T>
+static ibm_status tune_run(ibm_ctx* ctx, ibm_mg* m, size_t l, int kind)
+{
+    const mg_level& L = m->lv[l];
+    level_bufs<T>& B = bufs<T>(m)[l];
+    const op3<T> A = level_op<T>(L);
+    if (kind == TUNE_CHEB) return launch_cheb_fused<T>(ctx, A, L.w, B.b, B.x, false, nullptr);
+    level_bufs<T>& C = bufs<T>(m)[l + 1];
+    if (kind == TUNE_RR) return launch_residual_restrict<T>(ctx, A, level_tr<T>(L), B.x, B.b, C.b);
+    return launch_prolong_residual<T>(ctx, A, level_tr<T>(L), C.x, B.x, B.b, B.s1, B.r);
+}
+
+template <typename T>
+static ibm_status tune_levels(ibm_ctx* ctx, ibm_mg* m)
+{
+    if (!autotune_on.load()) return IBM_OK;
+    const int elem = (int) sizeof(T);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return fail(ctx, IBM_ERR_HIP, "multigrid tuning: event creation failed");
+    }
+    const bool profiling = ctx->profiling;
+    ctx->profiling = false;
+    ibm_status st = IBM_OK;
+    for (size_t l = 0; st == IBM_OK && l < m->coarse_start; ++l) {
+        const ibm_grid& g = m->lv[l].g;
+        const int nx = g.axis[0].points, ny = g.axis[1].points, nz = g.axis[2].points;
+        if (!cheb_fused_fits(nx, ny)) continue;
+        const int per = per_of(g);
+        bool zeroed = false;
+        for (int kind = TUNE_CHEB; st == IBM_OK && kind <= TUNE_PR; ++kind) {
+            if (kind != TUNE_CHEB && l + 1 == m->lv.size()) continue;
+            if (target_tuned(kind, elem, nx, ny, nz, per)) continue;
+            if (!zeroed) {
+                for (size_t k = l; st == IBM_OK && k <= std::min(l + 1, m->lv.size() - 1); ++k) {
+                    const level_bufs<T>& B = bufs<T>(m)[k];
+                    for (T* p : {B.x, B.b, B.r, B.s1, B.s2})
+                        if (hipMemsetAsync(p, 0, m->lv[k].n * sizeof(T), ctx->stream) != hipSuccess)
+                            st = fail(ctx, IBM_ERR_HIP, "multigrid tuning: clearing level %zu failed", k);
+                }
+                zeroed = true;
+            }
+            const int fallback = kind == TUNE_CHEB ? MARCH_DEFAULT_TARGET : TRANSFER_DEFAULT_TARGET;
+            float ms[sizeof(TUNE_TARGETS) / sizeof(int)] = {};
+            for (size_t c = 0; st == IBM_OK && c < sizeof(TUNE_TARGETS) / sizeof(int); ++c) {
+                set_tuned_target(kind, elem, nx, ny, nz, per, TUNE_TARGETS[c]);
+                if ((st = tune_run<T>(ctx, m, l, kind)) != IBM_OK) break;   // warm-up
+                hipEventRecord(e0, ctx->stream);
+                for (int r = 0; st == IBM_OK && r < TUNE_REPS; ++r) st = tune_run<T>(ctx, m, l, kind);
+                hipEventRecord(e1, ctx->stream);
+                if (st != IBM_OK) break;
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[c], e0, e1) != hipSuccess)
+                    st = fail(ctx, IBM_ERR_HIP, "multigrid tuning: timing failed");
+            }
+            // The default stays unless another target beats it by the margin;
+            // then the fastest target wins.
+            int best = fallback;
+            float bar = 0;
+            for (size_t c = 0; c < sizeof(TUNE_TARGETS) / sizeof(int); ++c)
+                if (TUNE_TARGETS[c] == fallback) bar = TUNE_MARGIN * ms[c];
+            for (size_t c = 0; c < sizeof(TUNE_TARGETS) / sizeof(int); ++c)
+                if (ms[c] < bar) {
+                    best = TUNE_TARGETS[c];
+                    bar = ms[c];
+                }
+            set_tuned_target(kind, elem, nx, ny, nz, per, st == IBM_OK ? best : 0);
+        }
+    }
+    ctx->profiling = profiling;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return st;
+}
+
 template <typename T>
 static ibm_status alloc(ibm_ctx* ctx, ibm_mg* m, size_t n, T** p)
 {
@@ -501,6 +594,7 @@ ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* g, double sigma, double t
             st = prepare_coarse_vcycle<float>(ctx, m->coarse32.lds_bytes);
         }
     }
+    if (st == IBM_OK) st = precision == 64 ? tune_levels<double>(ctx, m) : tune_levels<float>(ctx, m);
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->r);
     if (st == IBM_OK) st = alloc(ctx, m, m->lv[0].n, &m->rhs);
     if (st == IBM_OK) st = alloc(ctx, m, REDUCE_PARTS, &m->partial);
@@ -528,6 +622,24 @@ ibm_status ibm_mg_levels(const ibm_mg* m, int* levels, ibm_grid* grids, int max_
     if (!m || !levels) return IBM_ERR_ARG;
     *levels = (int) m->lv.size();
     for (int i = 0; grids && i < max_grids && i < (int) m->lv.size(); ++i) grids[i] = m->lv[i].g;
+    return IBM_OK;
+}
+
+ibm_status ibm_mg_autotune(int enable, int clear)
+{
+    if (clear) clear_tuned_targets();
+    autotune_on.store(enable != 0);
+    return IBM_OK;
+}
+
+ibm_status ibm_mg_shape(const ibm_mg* m, int level, int kind, int* target, int* wide)
+{
+    if (!m || !target || !wide || kind < TUNE_CHEB || kind > TUNE_PR) return IBM_ERR_ARG;
+    if (level < 0 || (size_t) level >= m->lv.size()) return IBM_ERR_ARG;
+    const ibm_grid& g = m->lv[level].g;
+    *target = tuned_target(kind, m->precision == 64 ? 8 : 4, g.axis[0].points, g.axis[1].points, g.axis[2].points,
+                           per_of(g));
+    *wide = march_wide(m->lv[level].n) ? 1 : 0;
     return IBM_OK;
 }
 
@@ -665,6 +777,173 @@ ibm_status ibm_dot(ibm_ctx* ctx, int64_t n, const double* x, const double* y, do
     double* partial = (double*) p;
     IBM_TRY(launch_dot<double, double>(ctx, (size_t) n, x, y, partial, partial + REDUCE_PARTS));
     return read_scalar(ctx, partial + REDUCE_PARTS, out);
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Helmholtz solves on many independent copies of one periodic grid (a batch
+// of small boxes): the V-cycle of solvers::multigrid (FP32, degree-4
+// Chebyshev smoothing over the upper Gershgorin interval, 4-tap transfers,
+// smoother-only coarsest level) inside FP64 defect correction, every
+// operation one launch over all copies; a copy converges when its own
+// relative FP64 residual reaches rtol, and the solve stops when all have.
+// ---------------------------------------------------------------------------
+struct ibm_boxes_mg {
+    int nbatch = 0;
+    double sigma = 0, tau = 0;
+    std::vector<ibm::mg_level> lv;
+    std::vector<ibm::level_bufs<float>> b32;
+    double* r = nullptr;            // FP64 residual, nbatch x n
+    double* norms = nullptr;        // device: |b|^2 then |r|^2 per copy
+    std::vector<double> host;       // host copy of the norms
+    std::vector<void*> owned;
+};
+
+namespace ibm {
+namespace {
+
+template <typename T>
+ibm_status boxes_alloc(ibm_ctx* ctx, ibm_boxes_mg* m, size_t n, T** p)
+{
+    void* v = nullptr;
+    IBM_HIP(ctx, hipMalloc(&v, sizeof(T) * std::max<size_t>(n, 1)));
+    m->owned.push_back(v);
+    *p = (T*) v;
+    return IBM_OK;
+}
+
+// out = p(A) v, or out += p(A) v (algo/chebyshev.h:149-163 with chebw<4>).
+ibm_status boxes_cheb(ibm_ctx* ctx, const ibm_boxes_mg* m, int l, const float* v, float* out, bool add)
+{
+    const mg_level& L = m->lv[l];
+    const level_bufs<float>& B = m->b32[l];
+    const int nb = m->nbatch;
+    IBM_TRY(launch_batch_stencil<float>(ctx, L.A32, nb, (float) -L.w[0], v, 0.0f, nullptr, (float) L.w[1], v, B.s1));
+    IBM_TRY(launch_batch_stencil<float>(ctx, L.A32, nb, -1.0f, B.s1, 0.0f, nullptr, (float) L.w[2], v, B.s2));
+    return launch_batch_stencil<float>(ctx, L.A32, nb, -1.0f, B.s2, add ? 1.0f : 0.0f, add ? out : nullptr,
+                                       (float) L.w[3], v, out);
+}
+
+// x_l = V(b_l) (solvers/multigrid.h:120-128)
+ibm_status boxes_vcycle(ibm_ctx* ctx, const ibm_boxes_mg* m, int l)
+{
+    const mg_level& L = m->lv[l];
+    const level_bufs<float>& B = m->b32[l];
+    const int nb = m->nbatch;
+    IBM_TRY(boxes_cheb(ctx, m, l, B.b, B.x, false));
+    if (l + 1 == (int) m->lv.size()) return IBM_OK;
+    IBM_TRY(launch_batch_stencil<float>(ctx, L.A32, nb, -1.0f, B.x, 0.0f, nullptr, 1.0f, B.b, B.r));
+    IBM_TRY(launch_batch_restrict<float>(ctx, L.T32, nb, B.r, m->b32[l + 1].b));
+    IBM_TRY(boxes_vcycle(ctx, m, l + 1));
+    IBM_TRY(launch_batch_prolong_add<float>(ctx, L.T32, nb, m->b32[l + 1].x, B.x));
+    IBM_TRY(launch_batch_stencil<float>(ctx, L.A32, nb, -1.0f, B.x, 0.0f, nullptr, 1.0f, B.b, B.r));
+    return boxes_cheb(ctx, m, l, B.r, B.x, true);
+}
+
+} // namespace
+} // namespace ibm
+
+extern "C" {
+
+ibm_status ibm_boxes_mg_create(ibm_ctx* ctx, const ibm_grid* grid, int nbatch, double sigma, double tau,
+                               ibm_boxes_mg** out)
+{
+    using namespace ibm;
+    if (!ctx || !out) return IBM_ERR_ARG;
+    *out = nullptr;
+    if (!grid || grid->dims != 3) return fail(ctx, IBM_ERR_ARG, "batched solves take one 3-dimensional grid");
+    if (nbatch < 1) return fail(ctx, IBM_ERR_ARG, "at least one copy");
+    for (int d = 0; d < 3; ++d)
+        if (grid->axis[d].solid) return fail(ctx, IBM_ERR_BAD_GRID, "batched solves take periodic boxes");
+    if (!(sigma != 0.0)) return fail(ctx, IBM_ERR_ARG, "batched solves take Helmholtz operators (sigma != 0)");
+    auto* m = new ibm_boxes_mg;
+    m->nbatch = nbatch;
+    m->sigma = sigma;
+    m->tau = tau;
+    ibm_grid cur = *grid;
+    ibm_status st = IBM_OK;
+    for (;;) {
+        mg_level L{};
+        L.g = cur;
+        L.n = (size_t) ibm_grid_size(&cur);
+        double a, b;
+        smoother_interval(cur, sigma, tau, &a, &b);
+        chebyshev_weights(a, b, L.w);
+        L.A64 = make_op<double>(cur, sigma, tau);
+        L.A32 = make_op<float>(cur, sigma, tau);
+        ibm_grid next;
+        bool more = refinable(cur) && ibm_grid_coarsen(&cur, &next) == IBM_OK;
+        if (more) L.T32 = make_transfer<float>(cur, next);
+        m->lv.push_back(L);
+        if (!more) break;
+        cur = next;
+    }
+    for (auto& L : m->lv) {
+        level_bufs<float> B;
+        size_t n = L.n * (size_t) nbatch;
+        if ((st = boxes_alloc(ctx, m, n, &B.x)) || (st = boxes_alloc(ctx, m, n, &B.b)) ||
+            (st = boxes_alloc(ctx, m, n, &B.r)) || (st = boxes_alloc(ctx, m, n, &B.s1)) ||
+            (st = boxes_alloc(ctx, m, n, &B.s2)))
+            break;
+        m->b32.push_back(B);
+    }
+    if (st == IBM_OK) st = boxes_alloc(ctx, m, m->lv[0].n * (size_t) nbatch, &m->r);
+    if (st == IBM_OK) st = boxes_alloc(ctx, m, 2 * (size_t) nbatch, &m->norms);
+    m->host.resize(2 * (size_t) nbatch);
+    if (st != IBM_OK) {
+        ibm_boxes_mg_destroy(m);
+        return st;
+    }
+    *out = m;
+    return IBM_OK;
+}
+
+ibm_status ibm_boxes_mg_destroy(ibm_boxes_mg* m)
+{
+    if (!m) return IBM_OK;
+    hipDeviceSynchronize();
+    for (void* p : m->owned) hipFree(p);
+    delete m;
+    return IBM_OK;
+}
+
+ibm_status ibm_boxes_mg_solve(ibm_ctx* ctx, ibm_boxes_mg* m, const double* b, double* x, double rtol, int maxit,
+                              int* iters, double* max_relres)
+{
+    using namespace ibm;
+    if (!ctx || !m) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    const int nb = m->nbatch;
+    const mg_level& L0 = m->lv[0];
+    const size_t n = L0.n, total = n * (size_t) nb;
+    IBM_TRY(launch_batch_dot<double>(ctx, n, nb, b, b, m->norms));
+    int it = 0;
+    double worst = INFINITY;
+    for (;;) {
+        // r = b - A x (FP64), then each copy's |r|^2
+        IBM_TRY(launch_batch_stencil<double>(ctx, L0.A64, nb, -1.0, x, 0.0, nullptr, 1.0, b, m->r));
+        IBM_TRY(launch_batch_dot<double>(ctx, n, nb, m->r, m->r, m->norms + nb));
+        IBM_HIP(ctx, hipMemcpyAsync(m->host.data(), m->norms, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        worst = 0;
+        for (int k = 0; k < nb; ++k) {
+            double bn = m->host[k], rn = m->host[nb + k];
+            double rel = bn > 0 ? std::sqrt(rn / bn) : (rn > 0 ? INFINITY : 0.0);
+            if (!(rel <= worst)) worst = rel;      // carries NaN
+        }
+        if (worst <= rtol || it >= maxit) break;
+        IBM_TRY(launch_convert<float, double>(ctx, total, m->r, m->b32[0].b));
+        IBM_TRY(boxes_vcycle(ctx, m, 0));
+        IBM_TRY(launch_axpby<double, float>(ctx, total, 1.0, m->b32[0].x, 1.0, x));
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (max_relres) *max_relres = worst;
+    if (!(worst <= rtol))
+        return fail(ctx, IBM_ERR_NOT_CONVERGED, "batched solve: worst relative residual %g after %d cycles", worst, it);
+    return IBM_OK;
 }
 
 } // extern "C"

@@ -992,6 +992,337 @@ ibm_status static_bin_work(ibm_ctx* ctx, unsigned char* base, int total, int nbi
     return IBM_OK;
 }
 
+// ----------------------------------------------------------------------
+// Tile spreading: every marker is visited once.
+//
+// The markers of nboxes independent boxes (one set of component grids,
+// repeated; the C4 channel is one box, the small periodic boxes of a batch
+// are many) are counting-sorted into bins of 8 x 8 x 8 face-aligned cells
+// ("bricks"), brick-major.  One workgroup per brick adds each of its
+// markers' 64 stencil values per component into a 12 x 12 x 12 tile of
+// 64-bit fixed-point integers in LDS (the brick's points plus the reach of
+// its stencils), and stores the tile once.  A merge pass then forms every
+// grid point from the (at most eight) tiles that cover it.  Integer addition
+// is associative, so the result does not depend on scheduling: no
+// floating-point atomics, no global atomics, bitwise reproducible.
+//
+// Tile geometry along axis j: brick b covers bins q in [8 b, 8 b + 8) (bin
+// coordinates relative to qmin); a marker of bin q has base cell
+// base_c = q + qmin + d_c for component c with d_c in [lo_j, hi_j]
+// (hi_j - lo_j <= 1), and touches points base_c - 1 .. base_c + 2, so the
+// tile starts at point T0 = 8 b + qmin + lo - 1 and a marker's stencil
+// origin lies at tile index (q - 8 b) + (d_c - lo) in [0, 8].
+// ----------------------------------------------------------------------
+constexpr int TB = 8;                       // brick edge in bins
+constexpr int TT = 12;                      // tile edge in points
+constexpr int TILE_PTS = TT * TT * TT;
+constexpr int TILE_THREADS = 256;
+
+struct tile_desc {
+    int nbr[3];         // bricks per axis
+    int bricks;         // bricks per box
+    int nboxes;
+    int keys;           // nboxes * bricks * 512
+    int box_points[3];  // points of one box's component grid
+};
+
+// Box of marker m: offsets holds nboxes + 1 increasing marker indices.
+__device__ inline int box_of(const int* __restrict__ off, int nboxes, int m)
+{
+    int lo = 0, hi = nboxes;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= m) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void tile_count_kernel(marker_sets sets, kgrids g, bin_desc bd, tile_desc td,
+                                  const int* __restrict__ box_off, int* __restrict__ key, int* __restrict__ rank,
+                                  int* __restrict__ code, int* __restrict__ count, bin_stats* __restrict__ stats)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long fbits = 0;
+    const bool valid = m < sets.total;
+    int bin = -1;
+    if (valid) {
+        int k = set_of(sets, m);
+        int l = m - sets.offset[k];
+        int n = sets.offset[k + 1] - sets.offset[k];
+        const double* x = sets.x[k];
+        double p[3] = {x[l], x[n + l], x[2 * n + l]};
+        int q[3], br[3], lb[3];
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = kstencil(g.a[bd.fc[j]][j], p[j]);
+            q[j] = st.base + bd.fs[j];
+            int qb = q[j] - bd.qmin[j];
+            qb = qb < 0 ? 0 : (qb >= bd.nb[j] ? bd.nb[j] - 1 : qb);
+            br[j] = qb / TB;
+            lb[j] = qb - br[j] * TB;
+        }
+        const int box = box_of(box_off, td.nboxes, m);
+        const int brick = ((box * td.nbr[2] + br[2]) * td.nbr[1] + br[1]) * td.nbr[0] + br[0];
+        bin = brick * (TB * TB * TB) + lb[0] + TB * (lb[1] + TB * lb[2]);
+        int c9 = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) {
+                stencil_axis st = kstencil(g.a[c][j], p[j]);
+                c9 |= ((st.base - q[j]) & 1) << (3 * c + j);
+            }
+        key[m] = bin;
+        code[m] = c9;
+        const double* f = sets.f[k];
+        const double f0 = fabs(f[l]), f1 = fabs(f[n + l]), f2 = fabs(f[2 * n + l]);
+        double a = f0 > f1 ? f0 : f1;
+        a = a > f2 ? a : f2;
+        if (!(isfinite(f0) && isfinite(f1) && isfinite(f2))) a = __longlong_as_double(0x7ff8000000000000ll);
+        fbits = (unsigned long long) __double_as_longlong(a);
+    }
+    {
+        // ranks inside a bin, one atomic per group of lanes sharing a bin
+        const int lane = threadIdx.x & 63;
+        const unsigned long long lt = (1ull << lane) - 1;
+        unsigned long long todo = __ballot(valid);
+        int leader = lane, before = 0, size = 0;
+        while (todo) {
+            const int l = __ffsll((long long) todo) - 1;
+            const int lbin = __shfl(bin, l, 64);
+            const unsigned long long grp = __ballot(valid && bin == lbin) & todo;
+            if ((grp >> lane) & 1) {
+                leader = l;
+                before = __popcll(grp & lt);
+            }
+            if (lane == l) size = __popcll(grp);
+            todo &= ~grp;
+        }
+        int base = 0;
+        if (valid && leader == lane) base = atomicAdd(&count[bin], size);
+        base = __shfl(base, leader, 64);
+        if (valid) rank[m] = base + before;
+    }
+    fbits = wave_max(fbits);
+    if ((threadIdx.x & 63) == 0 && fbits) atomicMax(&stats->max_abs_f[blockIdx.x % STAT_SLOTS], fbits);
+}
+
+// The quantum of the fixed-point tiles: q = 2^(e - 62) with e the exponent
+// of the bound 64 * max_count * max|f| * max_c res3_c * 0.3 on any point's
+// sum; a non-finite bound marks the result as poisoned (all NaN).
+struct tile_scale {
+    double inv_q, q;
+    int poisoned, pad;
+};
+
+__global__ void tile_scale_kernel(kgrids g, const bin_stats* __restrict__ stats, tile_scale* __restrict__ out)
+{
+    const int tid = threadIdx.x;      // one wavefront
+    unsigned long long fb = wave_max(stats->max_abs_f[tid]);
+    unsigned long long cm = wave_max(stats->max_count[tid]);
+    if (tid == 0) {
+        double res3 = fmax(g.res3[0], fmax(g.res3[1], g.res3[2]));
+        double bound = 64.0 * (double) cm * __longlong_as_double((long long) fb) * res3 * 0.3;
+        tile_scale s;
+        s.poisoned = !(bound <= 1.7976931348623157e308);
+        int e2 = 0;
+        if (!s.poisoned && bound > 0) frexp(bound, &e2);
+        s.inv_q = ldexp(1.0, 62 - e2);
+        s.q = s.poisoned ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, e2 - 62);
+        s.pad = 0;
+        *out = s;
+    }
+}
+
+// One workgroup per brick: lanes are markers, each adds its 3 x 64 stencil
+// values to the LDS tile; the tile is then stored (flags[brick] = 0 for a
+// brick without markers, whose tile is not written).
+__global__ void __launch_bounds__(TILE_THREADS)
+tile_spread_kernel(kgrids g, bin_desc bd, tile_desc td, int delta, int total, const int* __restrict__ start,
+                   const double* __restrict__ sorted, const tile_scale* __restrict__ scale,
+                   unsigned long long* __restrict__ tiles, int* __restrict__ flags)
+{
+    __shared__ unsigned long long acc[3][TILE_PTS];
+    const int brick = blockIdx.x;
+    const int s0 = start[(size_t) brick * (TB * TB * TB)];
+    const int s1 = start[(size_t) (brick + 1) * (TB * TB * TB)];
+    const int tid = threadIdx.x;
+    if (s1 == s0) {
+        if (tid == 0) flags[brick] = 0;
+        return;                                      // whole block
+    }
+    for (int i = tid; i < 3 * TILE_PTS; i += TILE_THREADS) (&acc[0][0])[i] = 0ull;
+    int r = brick % td.bricks;
+    int b0 = r % td.nbr[0];
+    int b1 = (r / td.nbr[0]) % td.nbr[1];
+    int b2 = r / (td.nbr[0] * td.nbr[1]);
+    const int bq[3] = {b0 * TB, b1 * TB, b2 * TB};
+    const double inv_q = scale->inv_q;
+    __syncthreads();
+    for (int i = s0 + tid; i < s1; i += TILE_THREADS) {
+        double p[3], fv[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            p[j] = sorted[(size_t) j * total + i];
+            fv[j] = sorted[(size_t) (3 + j) * total + i];
+        }
+        // bin coordinates of the marker (relative to its brick), as binned
+        int lq[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = kstencil(g.a[bd.fc[j]][j], p[j]);
+            int qb = st.base + bd.fs[j] - bd.qmin[j];
+            qb = qb < 0 ? 0 : (qb >= bd.nb[j] ? bd.nb[j] - 1 : qb);
+            lq[j] = qb - bq[j];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double w[3][4];
+            int o[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const kaxis& a = g.a[c][j];
+                stencil_axis st = kstencil(a, p[j]);
+                delta4(delta, st.d, w[j]);
+                // base_c - q, unwrapped across a periodic seam
+                stencil_axis fq = kstencil(g.a[bd.fc[j]][j], p[j]);
+                int d = st.base - (fq.base + bd.fs[j]);
+                if (!a.solid) {
+                    if (d > 2) d -= a.cells;
+                    else if (d < -3) d += a.cells;
+                }
+                o[j] = lq[j] + d - bd.lo[j];
+            }
+            // A marker outside a walled axis's range (clamped by the index
+            // policy) can have an origin off the tile: its points lie outside
+            // the grid, which drops them, so it is skipped.
+            if (o[0] < 0 || o[0] > TT - 4 || o[1] < 0 || o[1] > TT - 4 || o[2] < 0 || o[2] > TT - 4) continue;
+            const double fc = fv[c] * g.res3[c];
+            unsigned long long* t = acc[c];
+#pragma unroll
+            for (int sz = 0; sz < 4; ++sz) {
+#pragma unroll
+                for (int sy = 0; sy < 4; ++sy) {
+                    const double wyz = w[2][sz] * (w[1][sy] * fc);
+                    const int row = o[0] + TT * ((o[1] + sy) + TT * (o[2] + sz));
+#pragma unroll
+                    for (int sx = 0; sx < 4; ++sx)
+                        atomicAdd(&t[row + sx], (unsigned long long) __double2ll_rn(w[0][sx] * wyz * inv_q));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long* dst = tiles + (size_t) brick * 3 * TILE_PTS;
+    for (int i = tid; i < 3 * TILE_PTS; i += TILE_THREADS) dst[i] = (&acc[0][0])[i];
+    if (tid == 0) flags[brick] = 1;
+}
+
+// Every grid point of every box and component: the sum of the tile entries
+// that cover it, times the quantum.
+__global__ void __launch_bounds__(256)
+tile_merge_kernel(kgrids g, bin_desc bd, tile_desc td, const unsigned long long* __restrict__ tiles,
+                  const int* __restrict__ flags, const tile_scale* __restrict__ scale, double* __restrict__ fe0,
+                  double* __restrict__ fe1, double* __restrict__ fe2)
+{
+    const size_t n0 = (size_t) td.box_points[0], n1 = (size_t) td.box_points[1];
+    const size_t per_box = n0 + n1 + (size_t) td.box_points[2];
+    const size_t total = per_box * td.nboxes;
+    const double q = scale->q;
+    for (size_t gi = (size_t) blockIdx.x * blockDim.x + threadIdx.x; gi < total;
+         gi += (size_t) gridDim.x * blockDim.x) {
+        const int box = (int) (gi / per_box);
+        size_t r = gi - (size_t) box * per_box;
+        int c = 0;
+        if (r >= n0) { r -= n0; c = 1; if (r >= n1) { r -= n1; c = 2; } }
+        const int* pt = g.points[c];
+        int pp[3];
+        pp[0] = (int) (r % pt[0]);
+        pp[1] = (int) ((r / pt[0]) % pt[1]);
+        pp[2] = (int) (r / ((size_t) pt[0] * pt[1]));
+        // per axis: up to two (brick, tile index) pairs covering the point
+        int nb_c[3], bb[3][2], tt[3][2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const kaxis& a = g.a[c][j];
+            const int t0 = bd.qmin[j] + bd.lo[j] - 1;        // first tile point of brick 0
+            int rel = pp[j] - t0;
+            int cnt = 0;
+            if (!a.solid) {
+                // periodic: cells is a multiple of the brick edge here
+                rel = ((rel % a.cells) + a.cells) % a.cells;
+                int b = rel / TB, t = rel - b * TB;
+                bb[j][cnt] = b; tt[j][cnt] = t; ++cnt;
+                if (t + TB < TT) {
+                    int bp = b - 1 < 0 ? td.nbr[j] - 1 : b - 1;
+                    bb[j][cnt] = bp; tt[j][cnt] = t + TB; ++cnt;
+                }
+            } else {
+                int b = rel >= 0 ? rel / TB : -1;
+                int t = rel - b * TB;
+                if (b >= 0 && b < td.nbr[j]) { bb[j][cnt] = b; tt[j][cnt] = t; ++cnt; }
+                if (b - 1 >= 0 && b - 1 < td.nbr[j] && t + TB < TT) { bb[j][cnt] = b - 1; tt[j][cnt] = t + TB; ++cnt; }
+            }
+            nb_c[j] = cnt;
+        }
+        long long sum = 0;
+        for (int iz = 0; iz < nb_c[2]; ++iz)
+            for (int iy = 0; iy < nb_c[1]; ++iy)
+                for (int ix = 0; ix < nb_c[0]; ++ix) {
+                    const int brick = ((box * td.nbr[2] + bb[2][iz]) * td.nbr[1] + bb[1][iy]) * td.nbr[0] + bb[0][ix];
+                    if (!flags[brick]) continue;
+                    const size_t at = ((size_t) brick * 3 + c) * TILE_PTS + tt[0][ix] + TT * (tt[1][iy] + TT * tt[2][iz]);
+                    sum += (long long) tiles[at];
+                }
+        double* fe = c == 0 ? fe0 : (c == 1 ? fe1 : fe2);
+        fe[(size_t) box * td.box_points[c] + r] = (double) sum * q;
+    }
+}
+
+// Interpolation over boxes: one lane per marker, grid vectors of box b at
+// offset b * box_points[c] (ib/interpolate.h:49-76 per box).
+__global__ void __launch_bounds__(256)
+interp_boxes_kernel(marker_sets sets, kgrids g, int delta, const int* __restrict__ box_off, int nboxes,
+                    const double* __restrict__ u0, const double* __restrict__ u1, const double* __restrict__ u2)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= sets.total) return;
+    int k = set_of(sets, m);
+    int l = m - sets.offset[k];
+    int n = sets.offset[k + 1] - sets.offset[k];
+    const int box = box_of(box_off, nboxes, m);
+    const double* x = sets.x[k];
+    double p[3] = {x[l], x[n + l], x[2 * n + l]};
+    const double* ue[3] = {u0, u1, u2};
+    for (int c = 0; c < 3; ++c) {
+        double w[3][4];
+        size_t ix[4], oy[4], oz[4];
+        const int* pt = g.points[c];
+        for (int j = 0; j < 3; ++j) {
+            stencil_axis st = axis_stencil(g.a[c][j], p[j]);
+            delta4(delta, st.d, w[j]);
+            for (int o = 0; o < 4; ++o) {
+                size_t idx = (size_t) axis_interp_index(g.a[c][j], st.base + o - 1);
+                if (j == 0) ix[o] = idx;
+                else if (j == 1) oy[o] = idx * (size_t) pt[0];
+                else oz[o] = idx * (size_t) pt[0] * (size_t) pt[1];
+            }
+        }
+        const double* u = ue[c] + (size_t) box * pt[0] * pt[1] * pt[2];
+        double acc = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            double sy = 0.0;
+            for (int b = 0; b < 4; ++b) {
+                const double* row = u + oz[a] + oy[b];
+                double sx = w[0][0] * row[ix[0]];
+                sx = fma(w[0][1], row[ix[1]], sx);
+                sx = fma(w[0][2], row[ix[2]], sx);
+                sx = fma(w[0][3], row[ix[3]], sx);
+                sy = fma(w[1][b], sx, sy);
+            }
+            acc = fma(w[2][a], sy, acc);
+        }
+        sets.out[k][(size_t) c * n + l] = acc;
+    }
+}
+
 } // namespace
 
 // Resolves the device address of `scratch` on `device` (made current by the
@@ -1078,6 +1409,156 @@ ibm_status interp_sets(ibm_ctx* ctx, const ibm_grid* grids, int delta, const mar
     return IBM_OK;
 }
 
+
+// Whether the tile spread takes these grids: 3-dimensional, every periodic
+// axis a multiple of the brick edge (so bricks tile it exactly).
+bool tiles_fit(const kgrids& kg)
+{
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j)
+            if (!kg.a[c][j].solid && kg.a[c][j].cells % TB != 0) return false;
+    return true;
+}
+
+// Spread of the markers of nboxes boxes (box b: markers [off[b], off[b+1])
+// of the sets' concatenation, off on the device) into nboxes consecutive
+// copies of each component grid, by bricks and tiles.
+ibm_status spread_tiles(ibm_ctx* ctx, const ibm_grid* grids, int delta, const marker_sets& sets, int nboxes,
+                        const int* box_off, double* const* fe, bool timed)
+{
+    kgrids kg;
+    IBM_TRY(make_kgrids(ctx, grids, &kg));
+    IBM_TRY(check_delta(ctx, delta));
+    if (!fe || !fe[0] || !fe[1] || !fe[2]) return fail(ctx, IBM_ERR_ARG, "null output grid");
+    if (nboxes < 1) return fail(ctx, IBM_ERR_ARG, "at least one box");
+    if (!tiles_fit(kg))
+        return fail(ctx, IBM_ERR_BAD_GRID, "batched spreading needs periodic axes of a multiple of %d cells", TB);
+    bin_desc bd = make_bins(grids);
+    if (bd.nbins <= 0) return fail(ctx, IBM_ERR_SIZE, "too many grid bins");
+    for (int j = 0; j < 3; ++j)
+        if (bd.hi[j] - bd.lo[j] > 1 || bd.hi[j] < bd.lo[j])
+            return fail(ctx, IBM_ERR_BAD_GRID, "component grids are not a staggering of one cell grid");
+    tile_desc td{};
+    long long bricks = 1;
+    for (int j = 0; j < 3; ++j) {
+        td.nbr[j] = (bd.nb[j] + TB - 1) / TB;
+        bricks *= td.nbr[j];
+    }
+    long long keys = bricks * nboxes * (TB * TB * TB);
+    if (keys > INT_MAX - 1) return fail(ctx, IBM_ERR_SIZE, "too many boxes or bins for one batched spread");
+    td.bricks = (int) bricks;
+    td.nboxes = nboxes;
+    td.keys = (int) keys;
+    double grid_points = 0;
+    for (int c = 0; c < 3; ++c) {
+        td.box_points[c] = kg.points[c][0] * kg.points[c][1] * kg.points[c][2];
+        grid_points += (double) td.box_points[c] * nboxes;
+    }
+    bin_work w;
+    IBM_TRY(ctx_bin_work(ctx, sets.total, td.keys, &w));
+    void* p;
+    const size_t nbricks = (size_t) bricks * nboxes;
+    IBM_TRY(scratch(ctx, SLOT_TILES, sizeof(unsigned long long) * 3 * TILE_PTS * nbricks, &p));
+    unsigned long long* tiles = (unsigned long long*) p;
+    IBM_TRY(scratch(ctx, SLOT_TILE_FLAGS, sizeof(int) * nbricks + 256 + sizeof(tile_scale), &p));
+    int* flags = (int*) p;
+    tile_scale* sc = (tile_scale*) ((unsigned char*) p + ((sizeof(int) * nbricks + 255) / 256) * 256);
+    {
+        prof_scope prof(ctx, timed ? "bin" : nullptr);
+        size_t words = (size_t) td.keys + 1;
+        zero_words_kernel<<<blocks_for((long) std::min<size_t>(words, 1u << 22), 256), 256, 0, ctx->stream>>>(
+            (unsigned int*) w.count, words);
+        IBM_LAUNCH_CHECK(ctx);
+        zero_words_kernel<<<1, 256, 0, ctx->stream>>>((unsigned int*) w.stats, sizeof(bin_stats) / 4);
+        IBM_LAUNCH_CHECK(ctx);
+        if (sets.total > 0) {
+            tile_count_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, td, box_off, w.key,
+                                                                                     w.rank, w.code, w.count, w.stats);
+            IBM_LAUNCH_CHECK(ctx);
+        }
+        size_t tbytes = w.scan_bytes;
+        IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tbytes, w.count, w.start, td.keys + 1, ctx->stream));
+        if (sets.total > 0) {
+            bin_scatter_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets.total, w.key, w.rank, w.code,
+                                                                                      w.start, w.tmp);
+            IBM_LAUNCH_CHECK(ctx);
+            bin_canon_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, 1, w.key, w.start, w.tmp,
+                                                                                    w.perm, w.sorted, w.stats);
+            IBM_LAUNCH_CHECK(ctx);
+        }
+        tile_scale_kernel<<<1, 64, 0, ctx->stream>>>(kg, w.stats, sc);
+        IBM_LAUNCH_CHECK(ctx);
+    }
+    {
+        prof_scope prof(ctx, timed ? "spread_tile" : nullptr, 48.0 * sets.total);
+        tile_spread_kernel<<<(int) nbricks, TILE_THREADS, 0, ctx->stream>>>(kg, bd, td, delta, sets.total, w.start,
+                                                                            w.sorted, sc, tiles, flags);
+        IBM_LAUNCH_CHECK(ctx);
+    }
+    {
+        prof_scope prof(ctx, timed ? "spread_merge" : nullptr, 8.0 * grid_points);
+        long long blocks = std::min<long long>(((long long) grid_points + 255) / 256, 65536);
+        tile_merge_kernel<<<(int) blocks, 256, 0, ctx->stream>>>(kg, bd, td, tiles, flags, sc, fe[0], fe[1], fe[2]);
+        IBM_LAUNCH_CHECK(ctx);
+    }
+    return IBM_OK;
+}
+
+// The timestep's spread of several marker sets: the tile spread where the
+// grids allow it (every periodic axis a multiple of the brick edge), else
+// the brick spread.  Both are deterministic; they differ in the last bits
+// (their fixed-point scales differ), so one solver always uses one of them.
+bool tiles_usable(ibm_ctx* ctx, const ibm_grid* grids)
+{
+    kgrids kg;
+    if (make_kgrids(ctx, grids, &kg) != IBM_OK || !tiles_fit(kg)) return false;
+    bin_desc bd = make_bins(grids);
+    if (bd.nbins <= 0) return false;
+    long long keys = TB * TB * TB;
+    for (int j = 0; j < 3; ++j) keys *= (bd.nb[j] + TB - 1) / TB;
+    return keys <= INT_MAX - 1;
+}
+
+ibm_status spread_sets_step(ibm_ctx* ctx, const ibm_grid* grids, int delta, bool tiles, int nsets, const int* n,
+                            const double* const* x, const double* const* f, double* const* fe)
+{
+    marker_sets s;
+    IBM_TRY(make_sets(ctx, nsets, n, x, f, nullptr, &s));
+    if (tiles) return spread_tiles(ctx, grids, delta, s, 1, nullptr, fe, true);   // one box: no offsets read
+    return spread_sets(ctx, grids, delta, s, fe);
+}
+
+// The box offsets (host, nboxes + 1) in the context's scratch.
+ibm_status upload_box_offsets(ibm_ctx* ctx, int nboxes, const int* offsets, int total, const int** dev)
+{
+    if (!offsets) return fail(ctx, IBM_ERR_ARG, "null box offsets");
+    if (offsets[0] != 0 || offsets[nboxes] != total)
+        return fail(ctx, IBM_ERR_SIZE, "box offsets must run from 0 to the marker count %d", total);
+    for (int b = 0; b < nboxes; ++b)
+        if (offsets[b + 1] < offsets[b]) return fail(ctx, IBM_ERR_SIZE, "box offsets must not decrease");
+    void* p;
+    IBM_TRY(scratch(ctx, SLOT_BOX_OFF, sizeof(int) * (size_t) (nboxes + 1), &p));
+    IBM_HIP(ctx, hipMemcpyAsync(p, offsets, sizeof(int) * (size_t) (nboxes + 1), hipMemcpyHostToDevice, ctx->stream));
+    // the host array may be reused once the call returns
+    IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *dev = (const int*) p;
+    return IBM_OK;
+}
+
+ibm_status interp_boxes(ibm_ctx* ctx, const ibm_grid* grids, int delta, const marker_sets& sets, int nboxes,
+                        const int* box_off, const double* const* ue)
+{
+    kgrids kg;
+    IBM_TRY(make_kgrids(ctx, grids, &kg));
+    IBM_TRY(check_delta(ctx, delta));
+    if (!ue || !ue[0] || !ue[1] || !ue[2]) return fail(ctx, IBM_ERR_ARG, "null input grid");
+    if (sets.total == 0) return IBM_OK;
+    prof_scope prof(ctx, "interp");
+    interp_boxes_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, kg, delta, box_off, nboxes, ue[0],
+                                                                              ue[1], ue[2]);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
 } // namespace ibm
 
 extern "C" {
@@ -1127,6 +1608,34 @@ ibm_status ibm_interpolate(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta,
                            const double* x, const double* const* ue, double* ul)
 {
     return ibm_interpolate_sets(ctx, grids, delta, 1, &n, &x, ue, &ul);
+}
+
+ibm_status ibm_spread_boxes(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nboxes, const int* offsets,
+                            const double* x, const double* f, double* const* fe)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (nboxes < 1) return ibm::fail(ctx, IBM_ERR_ARG, "at least one box");
+    if (!offsets) return ibm::fail(ctx, IBM_ERR_ARG, "null box offsets");
+    int n = offsets[nboxes];
+    ibm::marker_sets s;
+    IBM_TRY(ibm::make_sets(ctx, 1, &n, &x, &f, nullptr, &s));
+    const int* dev_off = nullptr;
+    IBM_TRY(ibm::upload_box_offsets(ctx, nboxes, offsets, s.total, &dev_off));
+    return ibm::spread_tiles(ctx, grids, (int) delta, s, nboxes, dev_off, fe, true);
+}
+
+ibm_status ibm_interpolate_boxes(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nboxes, const int* offsets,
+                                 const double* x, const double* const* ue, double* ul)
+{
+    if (!ctx) return IBM_ERR_ARG;
+    if (nboxes < 1) return ibm::fail(ctx, IBM_ERR_ARG, "at least one box");
+    if (!offsets) return ibm::fail(ctx, IBM_ERR_ARG, "null box offsets");
+    int n = offsets[nboxes];
+    ibm::marker_sets s;
+    IBM_TRY(ibm::make_sets(ctx, 1, &n, &x, nullptr, &ul, &s));
+    const int* dev_off = nullptr;
+    IBM_TRY(ibm::upload_box_offsets(ctx, nboxes, offsets, s.total, &dev_off));
+    return ibm::interp_boxes(ctx, grids, (int) delta, s, nboxes, dev_off, ue);
 }
 
 } // extern "C"

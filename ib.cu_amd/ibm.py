@@ -106,6 +106,10 @@ _SIGS = {
                              ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "ibm_spread_sets_noalloc": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                                      ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ibm_spread_boxes": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I), _VP, _VP,
+                              ctypes.POINTER(_VP)]),
+    "ibm_interpolate_boxes": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I), _VP,
+                                   ctypes.POINTER(_VP), _VP]),
     "ibm_interpolate": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _VP, ctypes.POINTER(_VP), _VP]),
     "ibm_interpolate_sets": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, ctypes.POINTER(_I),
                                   ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
@@ -119,6 +123,8 @@ _SIGS = {
     "ibm_mg_destroy": (_I, [_VP]),
     "ibm_mg_levels": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(ibm_grid), _I]),
     "ibm_mg_vcycle": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_mg_autotune": (_I, [_I, _I]),
+    "ibm_mg_shape": (_I, [_VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ibm_mg_solve": (_I, [_VP, _VP, _VP, _VP, _D, _I, ctypes.POINTER(_I), _PD]),
     "ibm_chebpcg_solve": (_I, [_VP, ctypes.POINTER(ibm_grid), _D, _D, _VP, _VP, _D, _I, _VP,
                                ctypes.POINTER(_I), _PD]),
@@ -132,11 +138,16 @@ _SIGS = {
     "ibm_rbf_destroy": (_I, [_VP]),
     "ibm_rbf_eval": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_I), ctypes.POINTER(_VP),
                           ctypes.POINTER(_VP)]),
+    "ibm_rbf_build": (_I, [_VP, _I, _VP, _I, _VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_VP)]),
+    "ibm_sphere_weights": (_I, [_VP, _I, _VP, _VP]),
     "ibm_surface_geometry": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "ibm_axpby": (_I, [_VP, ctypes.c_int64, _D, _VP, _D, _VP]),
     "ibm_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _I, _D, _VP, _D, _VP]),
     "ibm_mg_solve_fixed": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "ibm_ins_boundary_apply": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _I, _D, _VP, _D, _VP]),
+    "ibm_boxes_mg_create": (_I, [_VP, ctypes.POINTER(ibm_grid), _I, _D, _D, ctypes.POINTER(_VP)]),
+    "ibm_boxes_mg_destroy": (_I, [_VP]),
+    "ibm_boxes_mg_solve": (_I, [_VP, _VP, _VP, _VP, _D, _I, ctypes.POINTER(_I), _PD]),
     "ibm_mg_solve_atol": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(_I), _PD]),
     "ibm_solver_create": (_I, [_VP, ctypes.POINTER(ibm_dimension), _I, _I, _VP, _I, _VP, ctypes.POINTER(_VP)]),
     "ibm_solver_destroy": (_I, [_VP]),
@@ -373,6 +384,35 @@ def spread_sets_noalloc(ctx, grids, delta, xs, fs, out):
     return out
 
 
+def spread_boxes(ctx, grids, delta, offsets, x, f, out=None):
+    """ibm_spread_boxes: markers of box b are x[offsets[b]:offsets[b+1]]
+    (SoA over all markers); returns three tensors of nboxes * size(grid_c)."""
+    import torch
+    nb = len(offsets) - 1
+    n = int(offsets[-1])
+    _check_markers(x, n)
+    _check_markers(f, n)
+    if out is None:
+        out = [torch.empty(nb * grids[c].size, dtype=torch.float64, device=x.device) for c in range(3)]
+    off = (ctypes.c_int * (nb + 1))(*[int(o) for o in offsets])
+    _check(lib().ibm_spread_boxes(ctx.handle, grids, DELTA.get(delta, delta), nb, off, _ptr(x), _ptr(f),
+                                  _ptrs(out)), ctx)
+    return out
+
+
+def interpolate_boxes(ctx, grids, delta, offsets, x, ue, out=None):
+    import torch
+    nb = len(offsets) - 1
+    n = int(offsets[-1])
+    _check_markers(x, n)
+    if out is None:
+        out = torch.empty(3 * n, dtype=torch.float64, device=x.device)
+    off = (ctypes.c_int * (nb + 1))(*[int(o) for o in offsets])
+    _check(lib().ibm_interpolate_boxes(ctx.handle, grids, DELTA.get(delta, delta), nb, off, _ptr(x), _ptrs(ue),
+                                       _ptr(out)), ctx)
+    return out
+
+
 def interpolate(ctx, grids, delta, x, ue, out=None):
     import torch
     n = x.numel() // 3
@@ -454,6 +494,12 @@ def axpby(ctx, a, x, b, y):
     return y
 
 
+def mg_autotune(enable=True, clear=False):
+    """Switches the multigrid block-target autotuning (ibm_mg_autotune);
+    clear=True forgets the targets recorded so far."""
+    _check(lib().ibm_mg_autotune(1 if enable else 0, 1 if clear else 0))
+
+
 class Multigrid:
     """solvers::multigrid(grid, tol, op, smoother) over A = sigma*I2 + tau*L."""
 
@@ -468,6 +514,20 @@ class Multigrid:
         grids = (ibm_grid * 32)()
         _check(lib().ibm_mg_levels(self.handle, ctypes.byref(n), grids, 32))
         return [grids[i] for i in range(n.value)]
+
+    def shapes(self):
+        """Per level, the launch shape (block target, wide tile) of the
+        smoothing, residual + restriction and prolongation + residual kernels
+        (target 0: none recorded, the built-in target is used)."""
+        out = []
+        for level in range(len(self.levels())):
+            row = []
+            for kind in range(3):
+                t, w = ctypes.c_int(), ctypes.c_int()
+                _check(lib().ibm_mg_shape(self.handle, level, kind, ctypes.byref(t), ctypes.byref(w)))
+                row.append((t.value, w.value))
+            out.append(row)
+        return out
 
     def vcycle(self, b):
         import torch
@@ -504,6 +564,37 @@ class Multigrid:
     def close(self):
         if self.handle:
             lib().ibm_mg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BoxesMultigrid:
+    """ibm_boxes_mg: Helmholtz solves on nbatch copies of one periodic grid."""
+
+    def __init__(self, ctx, grid, nbatch, op):
+        sigma, tau = op
+        self.ctx, self.grid, self.nbatch = ctx, grid, nbatch
+        h = _VP()
+        _check(lib().ibm_boxes_mg_create(ctx.handle, ctypes.byref(grid), nbatch, sigma, tau, ctypes.byref(h)), ctx)
+        self.handle = h
+
+    def solve(self, b, x=None, rtol=1e-10, maxit=60):
+        import torch
+        if x is None:
+            x = torch.zeros_like(b)
+        it, rel = _I(0), _D(0)
+        _check(lib().ibm_boxes_mg_solve(self.ctx.handle, self.handle, _ptr(b), _ptr(x), rtol, maxit,
+                                        ctypes.byref(it), ctypes.byref(rel)), self.ctx)
+        return x, it.value, rel.value
+
+    def close(self):
+        if self.handle:
+            lib().ibm_boxes_mg_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -625,6 +716,31 @@ def rbf_eval(ctx, ops, xs, ys=None):
     handles = (_VP * len(ops))(*[op.handle.value for op in ops])
     _check(lib().ibm_rbf_eval(ctx.handle, len(ops), handles, (_I * len(ns))(*ns), _ptrs(xs), _ptrs(ys)), ctx)
     return ys
+
+
+RBF_KINDS = {"value": 0, "t": 1, "p": 2, "tt": 3, "tp": 4, "pp": 5}
+
+
+def rbf_build(ctx, data, sample, degree, kinds=("value", "t", "p")):
+    """ibm_rbf_build: operators (ns, nd + np) (sample-major, the reference's
+    (nd + np) x ns column-major layout) built on the device from (theta, phi)
+    parameter tensors data (2, nd) and sample (2, ns)."""
+    import torch
+    nd, ns = data.shape[1], sample.shape[1]
+    npoly = (degree + 1) ** 2
+    outs = [torch.empty((ns, nd + npoly), dtype=torch.float64, device=data.device) for _ in kinds]
+    kk = (_I * len(kinds))(*[RBF_KINDS.get(k, k) for k in kinds])
+    _check(lib().ibm_rbf_build(ctx.handle, nd, _ptr(data.contiguous()), ns, _ptr(sample.contiguous()), degree,
+                               len(kinds), kk, _ptrs(outs)), ctx)
+    return outs
+
+
+def sphere_weights(ctx, params):
+    import torch
+    n = params.shape[1]
+    w = torch.empty(n, dtype=torch.float64, device=params.device)
+    _check(lib().ibm_sphere_weights(ctx.handle, n, _ptr(params.contiguous()), _ptr(w)), ctx)
+    return w
 
 
 def surface_geometry(ctx, ns, nc, y, t1, t2, w):

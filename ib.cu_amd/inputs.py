@@ -215,3 +215,67 @@ def random_markers(n, L, seed, margin=0.0):
         u = rng.uniform(n)
         out.append(margin + u * (L[j] - 2 * margin))
     return np.ascontiguousarray(np.concatenate(out))
+
+
+def config_c5(seed=5, nboxes=64, L=32 * UM, refinement=32, ns_rbc=546, ns_plt=386):
+    """64 independent periodic boxes of 32 um with 32^3 cells; box b holds
+    U{1..20} cells (seeded), each an RBC (546 sample sites) or a platelet
+    (386) with probability 1/2, at the first slots of a jittered 3 x 3 x 3
+    lattice (10.7 um spacing, so cells do not overlap), randomly rotated;
+    forces iid N(0,1).  Returns box-local SoA positions of all markers in
+    box order, the box offsets and the forces."""
+    rng = SplitMix64(seed)
+    rbc_ref = rbc_reference(ns_rbc)
+    plt_ref = platelet_reference(ns_plt)
+    spacing = L / 3
+    xs, offsets, kinds = [], [0], []
+    for b in range(nboxes):
+        ncells = 1 + int(rng.uniform(1)[0] * 20)
+        pts = []
+        for k in range(ncells):
+            u = rng.uniform(6)
+            iz, rem = divmod(k, 9)
+            iy, ix = divmod(rem, 3)
+            centre = np.array([(ix + 0.5 + 0.1 * (u[0] - 0.5)) * spacing,
+                               (iy + 0.5 + 0.1 * (u[1] - 0.5)) * spacing,
+                               (iz + 0.5 + 0.1 * (u[2] - 0.5)) * spacing])
+            ref = rbc_ref if u[3] < 0.5 else plt_ref
+            kinds.append(0 if u[3] < 0.5 else 1)
+            p = rotate(ref, 2 * math.pi * u[4], (0.0, 1.0, 0.0))
+            p = rotate(p, math.pi * u[5], (1.0, 0.0, 0.0))
+            pts.append(p + centre[:, None])
+        box = np.concatenate(pts, axis=1)
+        xs.append(box)
+        offsets.append(offsets[-1] + box.shape[1])
+    x = np.ascontiguousarray(np.concatenate(xs, axis=1).reshape(-1))
+    n = offsets[-1]
+    f = rng.normal(3 * n)
+    return {"L": L, "refinement": refinement, "nboxes": nboxes, "x": x, "f": f, "offsets": offsets, "n": n,
+            "kinds": kinds}
+
+
+# The eight RBC poses of test/ib-whole.cc:163-170: rotation (angle, axis)
+# then translation (um).
+C2_POSES = [
+    (5.068183442568522, (1.030437666936895, -0.021108450766973392, -0.04630059618399827),
+     (4.015214448901787, 6.100138060189724, 1.9880461041978537)),
+    (5.007815431576859, (0.9792381457522887, 0.06271914842308295, -0.05642529490935586),
+     (11.877848594136138, 5.83714041009, 3.97975684227)),
+    (5.074088738134238, (1.0142231005007727, -2.9623760263254767e-05, -0.039362970458103955),
+     (4.146345896973395, 5.881458848974533, 5.962170272743408)),
+    (5.079676113839819, (0.9890875402932139, -0.031109409809641483, -0.05032173371464803),
+     (12.013925005641164, 6.158047729278891, 7.93432970522188)),
+    (5.1362464926490805, (1.0348848942058133, 0.01720012363571293, -0.11947361656641693),
+     (3.908805848485636, 6.091134893946915, 9.855992001042118)),
+    (5.054989211831597, (0.9675754189340313, -0.048388248918717285, 0.03738865256187718),
+     (12.083267206719906, 5.80425714837, 12.1799588627)),
+    (5.108732820408364, (0.9569821581817692, -0.056252540193673733, -0.07872719696376375),
+     (4.110024751201524, 5.91689772487407, 13.976372295071807)),
+    (4.931079097988467, (0.9593196951752679, -0.020300578072308676, 0.07572396922527558),
+     (12.02165847715475, 6.012459328852655, 15.96571613373596)),
+]
+
+
+def config_c2_cells(points):
+    """The eight posed copies (3, 8, n) of reference points (3, n)."""
+    return np.stack([rotate(points, a, ax) + np.array(t)[:, None] * UM for a, ax, t in C2_POSES], axis=1)

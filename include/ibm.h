@@ -187,6 +187,24 @@ ibm_status ibm_spread_sets_noalloc(ibm_ctx* ctx, const ibm_grid* grids, ibm_delt
                                    int nsets, const int* n, const double* const* x,
                                    const double* const* f, double* const* fe);
 
+/* Many independent boxes in one launch per operator (no counterpart as
+ * one call in the reference, which spreads per instance with
+ * ib::novel::spread, ib/novel.h:197-227): nboxes boxes share one set of
+ * component grids (`grids`, the geometry of one box); the markers of box b
+ * are x[offsets[b] .. offsets[b+1]) of one SoA array of offsets[nboxes]
+ * markers (offsets on the host); fe[c] holds nboxes consecutive box grids
+ * of component c.  Every periodic axis must have a multiple of 8 cells.
+ * The spread bins every marker into 8 x 8 x 8-cell bricks of its box,
+ * accumulates each brick's stencils once in a 64-bit fixed-point LDS tile
+ * and merges the tiles: deterministic, no floating-point atomics.  The call
+ * synchronises the context stream to upload the offsets. */
+ibm_status ibm_spread_boxes(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nboxes, const int* offsets,
+                            const double* x, const double* f, double* const* fe);
+/* The matching interpolation: ul (SoA, offsets[nboxes] markers) from
+ * ue[c], nboxes consecutive box grids (ib/interpolate.h:92-115 per box). */
+ibm_status ibm_interpolate_boxes(ibm_ctx* ctx, const ibm_grid* grids, ibm_delta delta, int nboxes,
+                                 const int* offsets, const double* x, const double* const* ue, double* ul);
+
 /* Interpolates grid velocities to markers:
  *     U_c(m) = sum_g u_c(g) * prod_j phi((g_j - X_j(m)) / h)
  * Replaces ib::interpolate<domain, delta>::operator()(n, x, ue, ul)
@@ -256,6 +274,19 @@ ibm_status ibm_mg_create(ibm_ctx* ctx, const ibm_grid* grid, double sigma, doubl
 ibm_status ibm_mg_destroy(ibm_mg* mg);
 /* Number of levels and (optionally) their grids, finest first. */
 ibm_status ibm_mg_levels(const ibm_mg* mg, int* levels, ibm_grid* grids, int max_grids);
+/* Block-target autotuning of the z-marching smoothing, restriction and
+ * prolongation kernels (no counterpart in the reference, whose kernels run
+ * with fixed launch shapes).  While enabled (the default), ibm_mg_create
+ * times z chunkings aimed at 256, 512, 1024 and 2048 blocks on every level
+ * whose grid the process has not seen and keeps the fastest (if it beats the
+ * built-in target by 3%) for the rest of the process; targets change
+ * timings only, never values.  clear != 0 forgets the recorded targets.
+ * ibm_mg_shape reports a level's launch shape for kind 0 (smoothing),
+ * 1 (residual + restriction) or 2 (prolongation + residual): *target = the
+ * recorded block target (0: none recorded, the built-in one is used),
+ * *wide = 1 for the wide tile (grids of 2^19 points or more), 0 narrow. */
+ibm_status ibm_mg_autotune(int enable, int clear);
+ibm_status ibm_mg_shape(const ibm_mg* mg, int level, int kind, int* target, int* wide);
 /* x = V(b): one V-cycle from a zero guess (solvers/multigrid.h:120-128). */
 ibm_status ibm_mg_vcycle(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x);
 /* Solves A x = b: x += V(b - A x) until ||b - A x|| <= rtol ||b|| (FP64
@@ -263,6 +294,22 @@ ibm_status ibm_mg_vcycle(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x);
  * stopping rule).  Returns IBM_ERR_NOT_CONVERGED after maxit cycles. */
 ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, double rtol, int maxit,
                         int* iters, double* relres);
+
+/* Helmholtz solves A x = b, A = sigma I + tau L (sigma != 0), on nbatch
+ * independent copies of one periodic grid (a batch of small boxes, or the
+ * boxes of one MAC component): solvers::multigrid's V-cycle (as
+ * ibm_mg_create) in FP32 inside FP64 defect correction, every operation one
+ * launch over all copies.  b and x hold nbatch consecutive grid vectors; x is
+ * the starting guess.  Stops when every copy's ||b - A x|| <= rtol ||b||
+ * (host reads of the per-copy norms), IBM_ERR_NOT_CONVERGED after maxit
+ * cycles.  No counterpart as one call in the reference (one solver per
+ * instance there, solvers/multigrid.h:204-212). */
+typedef struct ibm_boxes_mg ibm_boxes_mg;
+ibm_status ibm_boxes_mg_create(ibm_ctx* ctx, const ibm_grid* grid, int nbatch, double sigma, double tau,
+                               ibm_boxes_mg** out);
+ibm_status ibm_boxes_mg_destroy(ibm_boxes_mg* mg);
+ibm_status ibm_boxes_mg_solve(ibm_ctx* ctx, ibm_boxes_mg* mg, const double* b, double* x, double rtol, int maxit,
+                              int* iters, double* max_relres);
 
 /* Chebyshev-preconditioned CG, solvers::chebpcg (solvers/chebpcg.h:9-23,
  * solvers/pcg.h:11-44): the Helmholtz solver of ins::diffusion.  Stops when
@@ -327,6 +374,29 @@ ibm_status ibm_rbf_destroy(ibm_rbf* op);
  * precision. */
 ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N, const double* const* X,
                         double* const* Y);
+
+/* Operator construction on the device (bases/operators.h:145-190,
+ * bases/fill.h:15-150, bases/phs.h:13-46, solvers/lu.h:14-48): the r^7
+ * polyharmonic spline of the chord distance between the parameter-space
+ * sites mapped to the unit sphere, augmented with the (degree + 1)^2
+ * monomials of the embedding that span the spherical harmonics of degree
+ * <= degree; LU of the (nd + np)^2 interpolation matrix (rocSOLVER getrf)
+ * and, for each requested kind, E[k] = solve(lu, [Phi_s; P_s]) as a
+ * (nd + np) x ns column-major device matrix (lde = nd + np), ready for
+ * ibm_rbf_create.  data and sample are device (theta, phi) arrays, n x 2
+ * column-major.  Kinds: the evaluator and the first / second derivatives in
+ * the sample parameters (bases/rbf.h:29-48). */
+enum {
+    IBM_RBF_VALUE = 0, IBM_RBF_D_T = 1, IBM_RBF_D_P = 2,
+    IBM_RBF_D_TT = 3, IBM_RBF_D_TP = 4, IBM_RBF_D_PP = 5
+};
+ibm_status ibm_rbf_build(ibm_ctx* ctx, int nd, const double* data, int ns, const double* sample, int degree,
+                         int nkinds, const int* kinds, double* const* E);
+/* Parameter-space quadrature weights of n sites on a sphere-like closed
+ * surface (bases/closed_surface.h:38-68, bases/shapes/sphere.h:101-115):
+ * the r^1 interpolation system with a constant augmentation solved for the
+ * right-hand side (0, ..., 0, 1), divided by cos(phi) / (4 pi). */
+ibm_status ibm_sphere_weights(ibm_ctx* ctx, int n, const double* params, double* w);
 
 /* Surface geometry of nc cells with ns sample sites, in FP64, from the
  * evaluated positions y and tangents t1, t2 (each ns x 3nc as above) and

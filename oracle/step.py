@@ -19,6 +19,7 @@ of the reference, so this follows the library's documented scheme:
   X += dt interpolate(u, X).
 """
 import numpy as np
+import scipy.fft as sfft
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
@@ -30,7 +31,13 @@ class Step:
     """spec: [(length, solid, lower, upper)] x 3 as for orc.grid_axes."""
 
     def __init__(self, orc, fd, rbf, spec, refinement, dt, nu, rho, delta, types, pseudo_pressure=False,
-                 forces=None):
+                 forces=None, solver="direct"):
+        """solver "direct": sparse LU of every operator; "iterative": the
+        Helmholtz solves by the CSR multigrid (solvers/multigrid.h) to
+        1e-14 relative and the pressure Poisson by its exact spectral
+        solution (Fourier modes on periodic axes, DCT-II modes on Neumann
+        walls: the eigenvectors of the cell-centred second difference), for
+        grids too large for sparse LU."""
         self.orc, self.fd, self.rbf = orc, fd, rbf
         self.forces_mod = forces
         self.spec, self.ref = spec, refinement
@@ -50,15 +57,76 @@ class Step:
         self.Bw = [fd.ins_boundary(ax, i, 2) for i, ax in enumerate(self.uaxes)]
         self.I = [fd.identity(ax, 2) for ax in self.uaxes]
         self.L = [fd.laplacian(ax) for ax in self.uaxes]
-        self.helm = [spla.splu(sp.csc_matrix(fd.operator(ax, 1.0, -lam))) for ax in self.uaxes]
-        Lp = sp.lil_matrix(fd.laplacian(self.paxes))
-        Lp[0, :] = 0
-        Lp[0, 0] = 1.0
-        self.pois = spla.splu(sp.csc_matrix(Lp))
+        self.solver = solver
+        if solver == "direct":
+            self.helm = [spla.splu(sp.csc_matrix(fd.operator(ax, 1.0, -lam))) for ax in self.uaxes]
+            Lp = sp.lil_matrix(fd.laplacian(self.paxes))
+            Lp[0, :] = 0
+            Lp[0, 0] = 1.0
+            self.pois = spla.splu(sp.csc_matrix(Lp))
+        else:
+            self.helm_mg = [fd.Multigrid(lambda r, al=al: self._axes_ref(spec, r, al), refinement, 1.0, -lam,
+                                         order="x-fastest") for al in MAC]
+            self.helm_A = [fd.operator(ax, 1.0, -lam) for ax in self.uaxes]
 
     def _axes(self, spec, align):
         a = self.orc.grid_axes(spec, self.ref, align)
         return [a[j] for j in range(3)]
+
+    def _axes_ref(self, spec, refinement, align):
+        a = self.orc.grid_axes(spec, refinement, list(align))
+        return [a[j] for j in range(3)]
+
+    def _helmholtz(self, i, rhs):
+        if self.solver == "direct":
+            return self.helm[i].solve(rhs)
+        mg, A = self.helm_mg[i], self.helm_A[i]
+        x = np.zeros_like(rhs)
+        r = rhs.copy()
+        for _ in range(100):
+            if np.linalg.norm(r) <= 1e-14 * np.linalg.norm(rhs):
+                break
+            e = mg.vcycle(r)
+            x = x + e
+            r = rhs - A @ x
+        return x
+
+    def _poisson(self, div):
+        """Mean-free phi with L_p phi = div (div mean-free)."""
+        if self.solver == "direct":
+            d = div.copy()
+            d[0] = 0.0
+            phi = self.pois.solve(d)
+            return phi - phi.mean()
+        ax = self.paxes
+        n = [a.points for a in ax]
+        f = div.reshape(n[2], n[1], n[0])                # z, y, x
+        lam = np.zeros((n[2], n[1], n[0]))
+        coef = f
+        for j, a in enumerate(ax):
+            axis = 2 - j
+            k = np.arange(n[j])
+            r2 = a.resolution ** 2
+            if a.solid:
+                coef = sfft.dct(coef, type=2, norm="ortho", axis=axis)
+                ev = -4 * r2 * np.sin(np.pi * k / (2 * n[j])) ** 2
+            else:
+                coef = sfft.fft(coef, axis=axis)
+                ev = -4 * r2 * np.sin(np.pi * k / n[j]) ** 2
+            shape = [1, 1, 1]
+            shape[axis] = n[j]
+            lam = lam + ev.reshape(shape)
+        lam[0, 0, 0] = 1.0
+        coef = coef / lam
+        coef[0, 0, 0] = 0.0
+        for j, a in enumerate(ax):
+            axis = 2 - j
+            if a.solid:
+                coef = sfft.idct(coef, type=2, norm="ortho", axis=axis)
+            else:
+                coef = sfft.ifft(coef, axis=axis)
+        phi = np.real(coef).reshape(-1)
+        return phi - phi.mean()
 
     def geometry(self, xs):
         out = []
@@ -145,12 +213,10 @@ class Step:
             rhs = rhs + dt * nu * lu
             if Gp is not None:
                 rhs = rhs + (dt * nu / 2) * dt * (self.Bw[i] @ Gp[i])
-            us.append(u[i] + self.helm[i].solve(rhs))
+            us.append(u[i] + self._helmholtz(i, rhs))
         div = fd.divergence(self.axes_of, MAC, us) / dt
         div -= div.mean()
-        div[0] = 0.0
-        phi = self.pois.solve(div)
-        phi -= phi.mean()
+        phi = self._poisson(div)
         G = fd.gradient(self.paxes_of, CENTRE, phi)
         un = [us[i] - dt * G[i] for i in range(3)]
         xn = []

@@ -1,0 +1,123 @@
+"""The C2 workload on the GPU: a 64^3 MAC grid over a 16 um box, periodic in
+x and z with no-slip walls in y and the top wall moving at 1e-3 m/s (at
+pi/4 in x-z, as test/ib-whole.cc:129-152 sets it), and the eight RBCs of
+test/ib-whole.cc:163-170 (2210 data / 8832 sample sites each, Skalak
+tension + bending with the reference's moduli).
+
+Bars: spread of the 70,656 sample-site forces and interpolation at the
+17,680 data sites within 1e-12 relative of the oracle (ib/novel.h,
+ib/interpolate.h restated); one FP64 step (solves to 1e-14) within 1e-10 of
+the oracle step (oracle/step.py with CSR multigrid Helmholtz solves to
+1e-14 and the exact spectral pressure solve, the grid being too large for
+sparse LU), and the divergence after the projection at rounding level.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, _load, load_inputs, load_surfaces, load_orc_rbf
+
+pytestmark = pytest.mark.gpu
+
+UM = 0.1
+L = 16 * UM
+REF = 64
+NU, RHO, DT = 10.0, 1e-3, 1e-4
+WALL = 0.1                 # 1e-3 m/s
+RBC_MEMBRANE = ("skalak", 2.5e-3, 2.5e-1, 2e-6, False)
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    I = load_inputs()
+    S = load_surfaces()
+    ct = S.CellType(I.rbc_shape, 2210, 8832, 5, I.bauer_params, membrane=True)
+    ref = ct.data_reference()
+    X = I.config_c2_cells(ref)                                # (3, 8, 2210)
+    return {"ct": ct, "X": np.ascontiguousarray(X.reshape(-1)), "reference": ref}
+
+
+def _spec():
+    return [(L, 0, (0.0, 0.0), (0.0, 0.0)), (L, 1, (1.0, 0.0), (1.0, 0.0)), (L, 0, (0.0, 0.0), (0.0, 0.0))]
+
+
+def _dom(ibm):
+    return [ibm.periodic(L), ibm.dirichlet(L), ibm.periodic(L)]
+
+
+def test_c2_spread_and_interpolation_match_oracle(ibm, orc, ctx, c2):
+    O = load_orc_rbf()
+    ct = c2["ct"]
+    y = O.evaluate(ct.E, ct.nd, c2["X"].reshape(3 * 8, ct.nd), dtype=np.float64).reshape(-1)   # 70,656 markers
+    rng = load_inputs().SplitMix64(2)
+    f = rng.normal(y.size)
+    grids = ibm.mac_grids(_dom(ibm), REF)
+    axes = orc.mac_axes(_spec(), REF)
+    fe = ibm.spread(ctx, grids, "cosine", _t(y), _t(f))
+    ue = [rng.normal(grids[c].size) for c in range(3)]
+    ul = ibm.interpolate(ctx, grids, "cosine", _t(c2["X"]), [_t(u) for u in ue])
+    ctx.synchronize()
+    ref = orc.spread(axes, "cosine", y, f)
+    errs = [_rel(fe[c].cpu().numpy(), ref[c]) for c in range(3)]
+    refl = orc.interpolate(axes, "cosine", c2["X"], ue)
+    errs.append(_rel(ul.cpu().numpy(), refl))
+    print("C2 spread (u, v, w) and interpolation relative errors", errs)
+    assert max(errs) < 1e-12
+
+
+def test_c2_step_matches_oracle(ibm, orc, ctx, c2):
+    import torch
+    fd = _load("orc_fd", os.path.join(ROOT, "oracle", "fd.py"))
+    step_mod = _load("orc_step", os.path.join(ROOT, "oracle", "step.py"))
+    fo = _load("orc_forces", os.path.join(ROOT, "oracle", "forces.py"))
+    O = load_orc_rbf()
+    ct = c2["ct"]
+    mats = [_t(m) for m in (ct.E, ct.Et, ct.Ep, *ct.seconds)]
+    op = ibm.RbfOperator(ctx, "f64", mats, ct.nd)
+    dop = ibm.RbfOperator(ctx, "f64", [_t(m) for m in ct.data_ops], ct.nd)
+    cell = {"ops": op, "data_ops": dop, "cells": 8, "weights": _t(ct.w), "membrane": ibm.membrane(*RBC_MEMBRANE),
+            "reference": _t(c2["reference"].reshape(-1))}
+    # Tolerance 1e-14: the pressure solve stops at tolerance x the
+    # divergence scale of u* (1.5e-11 relative at 1e-12 here), and the
+    # Poisson operator amplifies that residual by its condition number
+    # (~ (64 / pi)^2) in p; the oracle's pressure is the exact spectral one.
+    s = ibm.Solver(ctx, _dom(ibm), REF, DT, NU, density=RHO, tolerance=1e-14, precision=64, max_iterations=80,
+                   cell_types=[cell])
+    g, _ = s.grids()
+    ub = [torch.zeros(g[i].size, dtype=torch.float64, device="cuda") for i in range(3)]
+    ibm.boundary_apply(ctx, g[0], 1, 1, ub[0], value=math.sin(math.pi / 4) * WALL, order=2)
+    ibm.boundary_apply(ctx, g[2], 1, 1, ub[2], value=math.cos(math.pi / 4) * WALL, order=2)
+    st = s.zero_state([_t(c2["X"])])
+    s.step(st, boundary=ub)
+    info = s.info()
+    print("C2 step", info)
+    # The pressure solve also stops at the absolute floor tolerance x the
+    # divergence scale of u* (2 max(1/h) ||u*|| / dt), so its residual
+    # relative to ||div u*|| / dt may exceed the tolerance.
+    assert max(info["residual"][:3]) <= 1e-13 and info["residual"][3] <= 1e-12
+    T = {"E": ct.E, "Et": ct.Et, "Ep": ct.Ep, "seconds": ct.seconds, "data_ops": ct.data_ops, "nd": ct.nd,
+         "ns": ct.ns, "nc": 8, "w": ct.w, "k": 0.0, "reference": c2["reference"].reshape(-1),
+         "membrane": dict(zip(("tension", "shear", "bulk", "bending", "preferred"), RBC_MEMBRANE))}
+    ost = step_mod.Step(orc, fd, O, _spec(), REF, DT, NU, RHO, "cosine", [T], forces=fo, solver="iterative")
+    u0 = [np.zeros(g[i].size) for i in range(3)]
+    p0 = np.zeros(st.p.numel())
+    un, p, xs, _ = ost.step(u0, p0, [c2["X"].copy()], boundary=[b.cpu().numpy() for b in ub])
+    errs = [_rel(st.u[i].cpu().numpy(), un[i]) for i in range(3)]
+    errs.append(_rel(st.p.cpu().numpy(), p))
+    errs.append(_rel(st.xs[0].cpu().numpy(), xs[0]))
+    print("C2 step relative errors u, v, w, p, x", errs)
+    assert max(errs) < 1e-10
+    umax = max(np.abs(a).max() for a in un)
+    assert info["divergence"] <= 1e-9 * umax / (L / REF)
+    s.close()
