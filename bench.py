@@ -462,7 +462,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (f32 multigrid cycles in f64 defect correction)",
-        "data": "synthetic (seeded RBC/platelet poses, Bauer-spiral sites, tether forces, sheared channel)",
+        "data": "synthetic (seeded RBC/platelet poses 1% inflated, Bauer-spiral sites, Skalak / neo-Hookean tension "
+                "and bending forces, sheared channel)",
         "config": {"workload": "C4 IB timestep: 200 RBCs (2210/8832 sites) + 400 platelets (605/1552), "
                                "128x256x128 MAC grid, y walls with moving top wall, cosine delta, hipGraph replay",
                    "sample_sites": n_sample, "data_sites": n_data, "grid": [128, 256, 128], "dt_us": 0.1,

