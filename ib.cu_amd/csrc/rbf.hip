@@ -25,7 +25,11 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 #include "context.hpp"
 
@@ -135,10 +139,13 @@ __device__ inline void mfma_step(const u4& xa, const u4& eb, const u4& xa_lo, co
 // Results: lane l holds four rows of column l % 16: rows 4 (l / 16) + r,
 // r = 0..3, for the FP32-accumulating instructions, but rows (l / 16) + 4 r
 // for v_mfma_f64_16x16x4 (both measured on gfx950).
-template <int MODE>
-__global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
+template <int MODE, int WN>
+__global__ __launch_bounds__(128 * WN, 2) void rbf_gemm_kernel(gemm_groups G)
 {
     constexpr int PL = planes(MODE);
+    constexpr int THREADS = 128 * WN;                // waves: 2 along the samples x WN along the columns
+    constexpr int JN = 8 / WN;                       // 16-column blocks per wave
+    constexpr int LOADS = 4 * NTHREADS / THREADS;    // 16-byte staging loads per thread, tile and plane
     using acc_t = typename std::conditional<MODE == MODE_F64, d4, f4>::type;
     __shared__ __attribute__((aligned(16))) unsigned char lds[PL][2][TILE * STRIDE];
 
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
 
-    // Global -> register staging: 4 x 16 bytes per thread per tile and plane.
+    // Global -> register staging: LOADS x 16 bytes per thread per tile and plane.
     const unsigned char* srcE[PL];
     const unsigned char* srcX[PL];
 #pragma unroll
@@ -165,26 +172,26 @@ __global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
         srcE[p] = g.E[p] + (long) tm * TILE * row_bytes;
         srcX[p] = g.X[p] + (long) tn * TILE * row_bytes;
     }
-    u4 pe[PL][4], px[PL][4];
-    long goff[4];
-    int loff[4];
+    u4 pe[PL][LOADS], px[PL][LOADS];
+    long goff[LOADS];
+    int loff[LOADS];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int idx = tid + NTHREADS * i;
+    for (int i = 0; i < LOADS; ++i) {
+        int idx = tid + THREADS * i;
         int r = idx >> 3, q = idx & 7;
         goff[i] = (long) r * row_bytes + q * 16;
         loff[i] = r * STRIDE + q * 16;
     }
 #define RBF_LOAD(c)                                                                   \
     _Pragma("unroll") for (int p = 0; p < PL; ++p)                                    \
-        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                               \
+        _Pragma("unroll") for (int i = 0; i < LOADS; ++i) {                           \
             pe[p][i] = *(const u4*) (srcE[p] + goff[i] + (long) (c) * ROWB);       \
             px[p][i] = *(const u4*) (srcX[p] + goff[i] + (long) (c) * ROWB);       \
         }
 
-    acc_t acc[4][4];
+    acc_t acc[JN][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[j][i] = acc_t{};
 
@@ -195,7 +202,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
 #pragma unroll
         for (int p = 0; p < PL; ++p)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < LOADS; ++i) {
                 *(u4*) (&lds[p][0][loff[i]]) = pe[p][i];
                 *(u4*) (&lds[p][1][loff[i]]) = px[p][i];
             }
@@ -205,17 +212,19 @@ __global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
         }
 #pragma unroll
         for (int s = 0; s < ROWB / 64; ++s) {
-            u4 ef[PL][4], xf[PL][4];
+            u4 ef[PL][4], xf[PL][JN];
+            const int off = s * 64 + fgrp * 16;
 #pragma unroll
-            for (int p = 0; p < PL; ++p)
+            for (int p = 0; p < PL; ++p) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int off = s * 64 + fgrp * 16;
+                for (int i = 0; i < 4; ++i)
                     ef[p][i] = *(const u4*) (&lds[p][0][(wm * 64 + i * 16 + frow) * STRIDE + off]);
-                    xf[p][i] = *(const u4*) (&lds[p][1][(wn * 64 + i * 16 + frow) * STRIDE + off]);
-                }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < JN; ++j)
+                    xf[p][j] = *(const u4*) (&lds[p][1][(wn * 16 * JN + j * 16 + frow) * STRIDE + off]);
+            }
+#pragma unroll
+            for (int j = 0; j < JN; ++j)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     mfma_step<MODE>(xf[0][j], ef[0][i], xf[PL - 1][j], ef[PL - 1][i], acc[j][i]);
@@ -224,10 +233,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void rbf_gemm_kernel(gemm_groups G)
 #undef RBF_LOAD
 
     const int m_base = tm * TILE + wm * 64 + frow;
-    const int n_base = tn * TILE + wn * 64 + (MODE == MODE_F64 ? fgrp : fgrp * 4);
+    const int n_base = tn * TILE + wn * 16 * JN + (MODE == MODE_F64 ? fgrp : fgrp * 4);
     constexpr int RSTEP = MODE == MODE_F64 ? 4 : 1;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             int m = m_base + i * 16;
@@ -330,6 +339,65 @@ namespace ibm {
 
 int rbf_nops(const ibm_rbf* op) { return op ? op->nops : 0; }
 
+// FP64 GEMM variants, by waves along the columns: 2 (64 x 64 wave tiles,
+// about 250 registers, two waves per SIMD) or 4 (64 x 32, at most 128
+// registers, four waves per SIMD to hide the barrier and load latency).
+// Both give every output the same MFMA sequence over the same chunks, so
+// their results are bitwise equal.  Unless forced, the first eager launch of
+// a shape (tiles, summed contraction length) times both and keeps the faster
+// for the process; launches under graph capture use the recorded choice or
+// the 4-wave default.
+std::atomic<int> forced_waves_n{0};
+std::mutex variant_lock;
+std::map<std::pair<int, long>, int> variant_table;
+constexpr int DEFAULT_WAVES_N = 4;
+
+void launch_f64(int wn, const gemm_groups& G, int blocks, hipStream_t stream)
+{
+    if (wn == 4) rbf_gemm_kernel<MODE_F64, 4><<<blocks, 512, 0, stream>>>(G);
+    else rbf_gemm_kernel<MODE_F64, 2><<<blocks, NTHREADS, 0, stream>>>(G);
+}
+
+int f64_variant(ibm_ctx* ctx, hipStream_t stream, const gemm_groups& G, int blocks)
+{
+    (void) ctx;
+    const int forced = forced_waves_n.load();
+    if (forced) return forced;
+    long ksum = 0;
+    for (int i = 0; i < G.count; ++i) ksum += G.g[i].Kp;
+    const std::pair<int, long> key{G.total_tiles, ksum};
+    {
+        std::lock_guard<std::mutex> guard(variant_lock);
+        auto it = variant_table.find(key);
+        if (it != variant_table.end()) return it->second;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return DEFAULT_WAVES_N;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return DEFAULT_WAVES_N;
+    }
+    float ms[2] = {0, 0};
+    bool ok = true;
+    const int cand[2] = {2, 4};
+    for (int c = 0; c < 2 && ok; ++c) {
+        launch_f64(cand[c], G, blocks, stream);   // warm-up; every launch writes the same outputs
+        ok = hipEventRecord(e0, stream) == hipSuccess;
+        for (int r = 0; r < 2 && ok; ++r) launch_f64(cand[c], G, blocks, stream);
+        ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, stream) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms[c], e0, e1) == hipSuccess;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    const int pick = !ok ? DEFAULT_WAVES_N : (ms[1] <= ms[0] ? 4 : 2);
+    if (ok) {
+        std::lock_guard<std::mutex> guard(variant_lock);
+        variant_table[key] = pick;
+    }
+    return pick;
+}
+
 ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed)
@@ -390,10 +458,15 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
     double flops = 0;
     for (int b = 0; b < count; ++b)
         flops += 2.0 * ops[b]->M * (double) N[b] * ops[b]->K * std::max(0, std::min(j1, ops[b]->nops) - j0);
+    const int wn = mode == MODE_F64 ? f64_variant(ctx, stream, G, blocks) : 2;
     prof_scope prof(ctx, "rbf_gemm", flops);
-    if (mode == MODE_F64) rbf_gemm_kernel<MODE_F64><<<blocks, NTHREADS, 0, stream>>>(G);
-    else if (mode == MODE_F32) rbf_gemm_kernel<MODE_F32><<<blocks, NTHREADS, 0, stream>>>(G);
-    else rbf_gemm_kernel<MODE_BF16X3><<<blocks, NTHREADS, 0, stream>>>(G);
+    if (mode == MODE_F64) {
+        launch_f64(wn, G, blocks, stream);
+    } else if (mode == MODE_F32) {
+        rbf_gemm_kernel<MODE_F32, 2><<<blocks, NTHREADS, 0, stream>>>(G);
+    } else {
+        rbf_gemm_kernel<MODE_BF16X3, 2><<<blocks, NTHREADS, 0, stream>>>(G);
+    }
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
@@ -436,6 +509,13 @@ ibm_status ibm_rbf_create(ibm_ctx* ctx, int precision, int M, int K, int nops, c
         return fail(ctx, IBM_ERR_HIP, "operator packing failed");
     }
     *out = op;
+    return IBM_OK;
+}
+
+ibm_status ibm_rbf_gemm_variant(int waves_n)
+{
+    if (waves_n != 0 && waves_n != 2 && waves_n != 4) return IBM_ERR_ARG;
+    ibm::forced_waves_n.store(waves_n);
     return IBM_OK;
 }
 

@@ -136,6 +136,7 @@ _SIGS = {
     "ibm_advection": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "ibm_rbf_create": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_VP), _I, ctypes.POINTER(_VP)]),
     "ibm_rbf_destroy": (_I, [_VP]),
+    "ibm_rbf_gemm_variant": (_I, [_I]),
     "ibm_rbf_eval": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_I), ctypes.POINTER(_VP),
                           ctypes.POINTER(_VP)]),
     "ibm_rbf_build": (_I, [_VP, _I, _VP, _I, _VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_VP)]),
@@ -699,6 +700,12 @@ class RbfOperator:
             self.close()
         except Exception:
             pass
+
+
+def rbf_gemm_variant(waves_n=0):
+    """FP64 evaluation kernel: 2 or 4 waves along the columns of a tile, or
+    0 to time both per shape (ibm_rbf_gemm_variant)."""
+    _check(lib().ibm_rbf_gemm_variant(int(waves_n)))
 
 
 def rbf_eval(ctx, ops, xs, ys=None):

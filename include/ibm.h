@@ -375,6 +375,14 @@ ibm_status ibm_rbf_destroy(ibm_rbf* op);
 ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N, const double* const* X,
                         double* const* Y);
 
+/* Kernel variant of the FP64 evaluation (no counterpart in the reference,
+ * whose products go to cuBLAS): waves_n = 2 runs four 64 x 64-output waves
+ * per 128 x 128 tile, 4 runs eight 64 x 32-output waves (half the
+ * registers, twice the waves per SIMD), 0 (the default) times both on the
+ * first eager launch of each shape and keeps the faster for the process.
+ * Both variants give bitwise the same results.  Process-wide. */
+ibm_status ibm_rbf_gemm_variant(int waves_n);
+
 /* Operator construction on the device (bases/operators.h:145-190,
  * bases/fill.h:15-150, bases/phs.h:13-46, solvers/lu.h:14-48): the r^7
  * polyharmonic spline of the chord distance between the parameter-space

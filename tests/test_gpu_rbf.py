@@ -121,3 +121,42 @@ def test_surface_geometry_matches_oracle_and_sphere(ibm, ctx, rbc_ops):
     assert np.allclose(area.cpu().numpy(), 4 * math.pi * radii ** 2, rtol=1e-9)
     assert np.allclose(volume.cpu().numpy(), 4 * math.pi / 3 * radii ** 3, rtol=1e-9)
     op.close()
+
+
+@pytest.mark.parametrize("shape", [(8832, 2210, 600), (1552, 605, 1200), (300, 77, 5)],
+                         ids=["rbc-c4", "platelet-c4", "small"])
+def test_f64_kernel_variants_agree_bitwise(ibm, ctx, shape):
+    """ibm_rbf_gemm_variant: the 4-wave (64 x 64) and 8-wave (64 x 32)
+    FP64 kernels and the timed choice give bitwise the same products, at the
+    C4 shapes (200 RBCs, 400 platelets: N = 3 x cells) and a padded one."""
+    import torch
+    M, K, N = shape
+    rng = np.random.default_rng(M + K)
+    mats = [rng.standard_normal((M, K)) for _ in range(2)]
+    x = rng.standard_normal((N, K))
+    op = ibm.RbfOperator(ctx, "f64", [_t(m) for m in mats], K)
+    xd = _t(x)
+    out, ms = {}, {}
+    try:
+        for wn in (2, 4, 0):
+            ibm.rbf_gemm_variant(wn)
+            (Y,) = ibm.rbf_eval(ctx, [op], [xd])
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(5):
+                ibm.rbf_eval(ctx, [op], [xd], ys=[Y])
+            e1.record()
+            torch.cuda.synchronize()
+            out[wn], ms[wn] = Y.cpu().numpy(), e0.elapsed_time(e1) / 5
+    finally:
+        ibm.rbf_gemm_variant(0)
+    flops = 2.0 * M * N * K * 2
+    print(f"FP64 {M}x{K}x{N} x2: 4 waves {ms[2]:.3f} ms ({flops / ms[2] / 1e9:.1f} TFLOP/s), "
+          f"8 waves {ms[4]:.3f} ms ({flops / ms[4] / 1e9:.1f} TFLOP/s), timed choice {ms[0]:.3f} ms")
+    assert np.array_equal(out[2], out[4]) and np.array_equal(out[2], out[0])
+    cols = np.arange(0, N, max(1, N // 8))
+    for j in range(2):
+        assert _scaled_err(out[4][j][cols], mats[j], K, x[cols]) < BARS["f64"]
+    with pytest.raises(ibm.IbmError):
+        ibm.rbf_gemm_variant(3)
+    op.close()
