@@ -78,6 +78,9 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed);
 
+// Sizes ctx's packing scratch for such a batch ahead of time.
+ibm_status rbf_reserve(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N);
+
 // Membrane force passes (forces.hip) on `stream`.
 ibm_status membrane_table(ibm_ctx* ctx, hipStream_t stream, int ns, const double* geo, const double* w, double* table);
 ibm_status mean_curvature(ibm_ctx* ctx, hipStream_t stream, int nd, int nc, const double* geo, const double* h0,
@@ -87,6 +90,19 @@ ibm_status membrane_forces(ibm_ctx* ctx, hipStream_t stream, const ibm_membrane*
                            const double* table, const double* y, const double* yref, double beta, double* f);
 // Number of operators an RBF object holds.
 int rbf_nops(const ibm_rbf* op);
+
+// Fixed-count multigrid solves of several independent systems, solve k on
+// ctxs[k] (ibm_mg_solve_fixed each), issued cycle by cycle in turn; with
+// fork_ev (and count - 1 join events) the streams join into ctxs[0]'s and
+// fork again after every cycle.
+ibm_status mg_solve_fixed_multi(int count, ibm_ctx* const* ctxs, ibm_mg* const* ms, const double* const* b,
+                                double* const* x, const int* iters, bool warm, double* const* relres,
+                                hipEvent_t fork_ev = nullptr, const hipEvent_t* join_ev = nullptr);
+
+// The direct channel solver (fft_solve.hip): whether a grid qualifies, and
+// the solve on ctx's stream.
+bool fft_usable(const ibm_grid& g);
+ibm_status fft_solve(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, const double* b, double* x);
 
 // Whether the tile spread takes these MAC grids, and the timestep's spread
 // of nsets marker sets by the tile (tiles = true) or the brick kernel.

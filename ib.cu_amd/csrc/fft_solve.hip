@@ -1,0 +1,434 @@
+// Direct solution of A x = b, A = sigma I~ + tau L, on grids that are
+// periodic and uniform in x and z (power-of-two point counts) and walled in
+// y -- the channel of the timestep's benchmark.
+//
+// Row (i, j, k) of the operator (fd/laplacian.h:53-68, fd/identity.h:25-51)
+// is sigma Iy_j x + tau (Iy_j Lx x + Ly x + Iy_j Lz x): the identity factors
+// of periodic axes are 1 and their 3-point Laplacians are circulant, so the
+// discrete Fourier transform in x and z diagonalises them,
+//   Lx -> -4 rx^2 sin^2(pi kx / nx),   Lz -> -4 rz^2 sin^2(pi kz / nz),
+// and every Fourier mode (kx, kz) leaves one tridiagonal system along y,
+//   Iy_j (sigma + tau mu) X_j + tau ry^2 (lo_j X_{j-1} + d_j X_j + hi_j X_{j+1})
+//   = B_j,   mu = lambda_x(kx) + lambda_z(kz),
+// with the wall rows' ghost coefficients in d_0, d_{n-1} (fd/discretization.h:
+// 94-108).  The solve is: real-to-half-complex FFT along x, complex FFT
+// along z, one Thomas elimination per mode (in LDS), and the inverse
+// transforms -- the exact solution of the same discrete system the multigrid
+// iterates on, in five memory passes.  The singular case (sigma = 0 with
+// Neumann walls: the pressure Poisson) solves the mean-free problem, as the
+// multigrid does: the (0, 0) mode's right-hand side loses its mean, its last
+// unknown is pinned and the solution's mean is removed.
+//
+// The reference solves these systems with multigrid or PCG
+// (solvers/multigrid.h, solvers/pcg.h); this is a different algorithm for the
+// same discrete equations, used where the grid allows it.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <mutex>
+#include <vector>
+#include "context.hpp"
+#include "fd_kernels.hpp"
+#include "fd_ops.hpp"
+
+namespace ibm {
+namespace {
+
+constexpr int FFT_THREADS = 256;
+constexpr int ZT = 8;              // kx columns per z-transform / tridiagonal block
+constexpr int MAX_FFT_POINTS = 512;
+constexpr int MAX_WALL_POINTS = 800;
+constexpr size_t MAX_DYNAMIC_LDS = 160 * 1024;
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+__device__ inline int bitrev(int k, int logn) { return (int) (__brev((unsigned) k) >> (32 - logn)); }
+
+// In-place radix-2 transforms of `count` arrays of n points held in LDS in
+// bit-reversed order (a[t * n + bitrev(k)] = input k of array t).  tw[m] =
+// exp(-2 pi i m / n), m < n / 2; `inverse` conjugates the twiddles (no
+// scaling).  Every thread of the block takes part in every stage.
+__device__ void fft_stages(double2* a, int count, int n, int logn, const double2* __restrict__ tw, bool inverse)
+{
+    const int half_n = n >> 1;
+    const int total = count * half_n;
+    for (int s = 1; s <= logn; ++s) {
+        const int half = 1 << (s - 1);
+        const int stride = n >> s;                 // twiddle index step
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            const int t = b / half_n, bb = b - t * half_n;
+            const int grp = bb >> (s - 1), pos = bb & (half - 1);
+            const int i = t * n + grp * 2 * half + pos, j = i + half;
+            double2 w = tw[pos * stride];
+            if (inverse) w.y = -w.y;
+            const double2 u = a[i], v = cmul(a[j], w);
+            a[i] = make_double2(u.x + v.x, u.y + v.y);
+            a[j] = make_double2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+}
+
+// Forward transform along x of `rows` rows of nx reals into nh = nx / 2 + 1
+// coefficients each (row r of x -> row r of s, x fastest in both).
+__global__ void __launch_bounds__(FFT_THREADS) xfft_forward_kernel(int nx, int logn, int rows,
+                                                                   const double* __restrict__ x,
+                                                                   double2* __restrict__ s,
+                                                                   const double2* __restrict__ tw)
+{
+    extern __shared__ double2 sh[];
+    const int rpb = FFT_THREADS / (nx / 2), nh = nx / 2 + 1;
+    const int r0 = blockIdx.x * rpb;
+    for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
+        const int t = e / nx, k = e - t * nx, row = r0 + t;
+        sh[t * nx + bitrev(k, logn)] = make_double2(row < rows ? x[(size_t) row * nx + k] : 0.0, 0.0);
+    }
+    __syncthreads();
+    fft_stages(sh, rpb, nx, logn, tw, false);
+    for (int e = threadIdx.x; e < rpb * nh; e += blockDim.x) {
+        const int t = e / nh, k = e - t * nh, row = r0 + t;
+        if (row < rows) s[(size_t) row * nh + k] = sh[t * nx + k];
+    }
+}
+
+// Inverse of xfft_forward_kernel (the missing coefficients are the
+// conjugates of the stored ones), scaled by `scale`.
+__global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int logn, int rows,
+                                                                   const double2* __restrict__ s,
+                                                                   double* __restrict__ x, double scale,
+                                                                   const double2* __restrict__ tw)
+{
+    extern __shared__ double2 sh[];
+    const int rpb = FFT_THREADS / (nx / 2), nh = nx / 2 + 1;
+    const int r0 = blockIdx.x * rpb;
+    for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
+        const int t = e / nx, k = e - t * nx, row = r0 + t;
+        double2 v = make_double2(0.0, 0.0);
+        if (row < rows) {
+            if (k < nh) {
+                v = s[(size_t) row * nh + k];
+            } else {
+                v = s[(size_t) row * nh + (nx - k)];
+                v.y = -v.y;
+            }
+        }
+        sh[t * nx + bitrev(k, logn)] = v;
+    }
+    __syncthreads();
+    fft_stages(sh, rpb, nx, logn, tw, true);
+    for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
+        const int t = e / nx, k = e - t * nx, row = r0 + t;
+        if (row < rows) x[(size_t) row * nx + k] = sh[t * nx + k].x * scale;
+    }
+}
+
+// Transform along z, in place, of s laid out [k][j][kx] (kx fastest, nh of
+// them): block (kx tile, j) transforms ZT columns.
+__global__ void __launch_bounds__(FFT_THREADS) zfft_kernel(int nh, int ny, int nz, int logn, double2* __restrict__ s,
+                                                           const double2* __restrict__ tw, int inverse)
+{
+    extern __shared__ double2 sh[];
+    const int kx0 = blockIdx.x * ZT, j = blockIdx.y;
+    const size_t plane = (size_t) nh * ny;
+    for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
+        const int t = e % ZT, k = e / ZT, kx = kx0 + t;
+        sh[t * nz + bitrev(k, logn)] =
+            kx < nh ? s[(size_t) k * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_stages(sh, ZT, nz, logn, tw, inverse != 0);
+    for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
+        const int t = e % ZT, k = e / ZT, kx = kx0 + t;
+        if (kx < nh) s[(size_t) k * plane + (size_t) j * nh + kx] = sh[t * nz + k];
+    }
+}
+
+struct wall_axis {
+    int n;
+    double r2, lap_lo, lap_hi, id_lo, id_hi;
+};
+
+// One Thomas elimination along y per Fourier mode: block (kx tile, kz) loads
+// ZT modes' columns into LDS, one lane per mode eliminates and substitutes
+// back, the block stores the solutions.
+__global__ void __launch_bounds__(FFT_THREADS) tridiagonal_kernel(int nh, int nz, wall_axis y, double sigma, double tau,
+                                                                  int singular, const double* __restrict__ lam_x,
+                                                                  const double* __restrict__ lam_z,
+                                                                  double2* __restrict__ s)
+{
+    extern __shared__ double2 sh[];
+    const int ny = y.n;
+    double2* sd = sh;                                      // ny x ZT values
+    double* sc = (double*) (sh + (size_t) ny * ZT);        // ny x ZT eliminated super-diagonals
+    const int kx0 = blockIdx.x * ZT, kz = blockIdx.y;
+    const size_t plane = (size_t) nh * ny;
+    for (int e = threadIdx.x; e < ZT * ny; e += blockDim.x) {
+        const int t = e % ZT, j = e / ZT, kx = kx0 + t;
+        sd[j * ZT + t] = kx < nh ? s[(size_t) kz * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < ZT && kx0 + t < nh) {
+        const int kx = kx0 + t;
+        const double mu = lam_x[kx] + lam_z[kz];
+        const bool pinned = singular && kx == 0 && kz == 0;
+        const double ty = tau * y.r2;
+        if (pinned) {
+            // mean-free right-hand side (the compatibility condition)
+            double2 m = make_double2(0.0, 0.0);
+            for (int j = 0; j < ny; ++j) {
+                m.x += sd[j * ZT + t].x;
+                m.y += sd[j * ZT + t].y;
+            }
+            m.x /= ny;
+            m.y /= ny;
+            for (int j = 0; j < ny; ++j) {
+                sd[j * ZT + t].x -= m.x;
+                sd[j * ZT + t].y -= m.y;
+            }
+        }
+        double cprev = 0.0;
+        double2 dprev = make_double2(0.0, 0.0);
+        for (int j = 0; j < ny; ++j) {
+            const double lo = j > 0 ? ty : 0.0, hi = j < ny - 1 ? ty : 0.0;
+            double d = -2.0, id = 1.0;
+            if (j == 0) {
+                d += y.lap_lo;
+                id += y.id_lo;
+            }
+            if (j == ny - 1) {
+                d += y.lap_hi;
+                id += y.id_hi;
+            }
+            const double diag = id * (sigma + tau * mu) + ty * d;
+            double cp = 0.0;
+            double2 dp = make_double2(0.0, 0.0);
+            if (!(pinned && j == ny - 1)) {
+                const double inv = 1.0 / (diag - lo * cprev);
+                cp = hi * inv;
+                const double2 r = sd[j * ZT + t];
+                dp = make_double2((r.x - lo * dprev.x) * inv, (r.y - lo * dprev.y) * inv);
+            }
+            sc[j * ZT + t] = cp;
+            sd[j * ZT + t] = dp;
+            cprev = cp;
+            dprev = dp;
+        }
+        double2 xn = sd[(ny - 1) * ZT + t];
+        for (int j = ny - 2; j >= 0; --j) {
+            const double c = sc[j * ZT + t];
+            const double2 dj = sd[j * ZT + t];
+            xn = make_double2(dj.x - c * xn.x, dj.y - c * xn.y);
+            sd[j * ZT + t] = xn;
+        }
+        if (pinned) {
+            double2 m = make_double2(0.0, 0.0);
+            for (int j = 0; j < ny; ++j) {
+                m.x += sd[j * ZT + t].x;
+                m.y += sd[j * ZT + t].y;
+            }
+            m.x /= ny;
+            m.y /= ny;
+            for (int j = 0; j < ny; ++j) {
+                sd[j * ZT + t].x -= m.x;
+                sd[j * ZT + t].y -= m.y;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ZT * ny; e += blockDim.x) {
+        const int tt = e % ZT, j = e / ZT, kx = kx0 + tt;
+        if (kx < nh) s[(size_t) kz * plane + (size_t) j * nh + kx] = sd[j * ZT + tt];
+    }
+}
+
+int log2_exact(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+std::vector<double2> twiddles(int n)
+{
+    std::vector<double2> w(n / 2);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < n / 2; ++m) {
+        const long double a = -2.0L * pi * m / n;
+        w[m] = make_double2((double) cosl(a), (double) sinl(a));
+    }
+    return w;
+}
+
+std::vector<double> eigenvalues(int n, double r2, int count)
+{
+    std::vector<double> l(count);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < count; ++k) {
+        const long double sn = sinl(pi * k / n);
+        l[k] = (double) (-4.0L * (long double) r2 * sn * sn);
+    }
+    return l;
+}
+
+} // namespace
+} // namespace ibm
+
+struct ibm_fft {
+    int device;
+    int nx, ny, nz, nh, lx, lz;
+    ibm::wall_axis y;
+    bool neumann;                 // both walls Neumann: sigma = 0 is the singular (mean-free) case
+    double2* spec = nullptr;      // nh x ny x nz coefficients
+    double2* twx = nullptr;
+    double2* twz = nullptr;
+    double* lam_x = nullptr;
+    double* lam_z = nullptr;
+    std::vector<void*> owned;
+};
+
+namespace ibm {
+
+// Whether the FFT solver takes grid g (and why not).
+static const char* fft_unsupported(const ibm_grid& g)
+{
+    if (g.axis[0].solid || g.axis[2].solid) return "x and z must be periodic";
+    if (!g.axis[1].solid) return "y must be walled";
+    const int nx = g.axis[0].points, nz = g.axis[2].points, ny = g.axis[1].points;
+    if (log2_exact(nx) < 2 || log2_exact(nz) < 2 || nx > MAX_FFT_POINTS || nz > MAX_FFT_POINTS)
+        return "x and z point counts must be powers of two from 4 to 512";
+    if (ny < 2 || ny > MAX_WALL_POINTS) return "y must hold 2 to 800 points";
+    return nullptr;
+}
+
+bool fft_usable(const ibm_grid& g) { return fft_unsupported(g) == nullptr; }
+
+static ibm_status fft_alloc(ibm_ctx* ctx, ibm_fft* p, size_t bytes, void** out)
+{
+    IBM_HIP(ctx, hipMalloc(out, bytes));
+    p->owned.push_back(*out);
+    return IBM_OK;
+}
+
+static ibm_status fft_lds_attributes(ibm_ctx* ctx)
+{
+    static std::mutex lock;
+    static bool done = false;
+    std::lock_guard<std::mutex> guard(lock);
+    if (done) return IBM_OK;
+    IBM_HIP(ctx, hipFuncSetAttribute((const void*) tridiagonal_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int) MAX_DYNAMIC_LDS));
+    IBM_HIP(ctx, hipFuncSetAttribute((const void*) zfft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int) MAX_DYNAMIC_LDS));
+    done = true;
+    return IBM_OK;
+}
+
+ibm_status fft_solve(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, const double* b, double* x)
+{
+    if (!ctx || !p) return IBM_ERR_ARG;
+    if (!b || !x) return fail(ctx, IBM_ERR_ARG, "null vector");
+    if (p->device != ctx->device) return fail(ctx, IBM_ERR_ARG, "FFT solver lives on another device");
+    if (!std::isfinite(sigma) || !std::isfinite(tau)) return fail(ctx, IBM_ERR_ARG, "non-finite operator coefficients");
+    const int rows = p->ny * p->nz;
+    const double points = (double) p->nx * p->ny * p->nz, coeffs = (double) p->nh * p->ny * p->nz;
+    // b read and x written once; the coefficients written and read four times
+    prof_scope prof(ctx, "fft_solve", 8.0 * 2.0 * points + 16.0 * 8.0 * coeffs);
+    const int rpb = FFT_THREADS / (p->nx / 2);
+    const size_t xlds = sizeof(double2) * rpb * p->nx;
+    xfft_forward_kernel<<<(rows + rpb - 1) / rpb, FFT_THREADS, xlds, ctx->stream>>>(p->nx, p->lx, rows, b, p->spec,
+                                                                                      p->twx);
+    IBM_LAUNCH_CHECK(ctx);
+    const dim3 zgrid((p->nh + ZT - 1) / ZT, p->ny);
+    const size_t zlds = sizeof(double2) * ZT * p->nz;
+    zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 0);
+    IBM_LAUNCH_CHECK(ctx);
+    const dim3 tgrid((p->nh + ZT - 1) / ZT, p->nz);
+    const size_t tlds = (sizeof(double2) + sizeof(double)) * ZT * p->ny;
+    const int singular = sigma == 0.0 && p->neumann;
+    tridiagonal_kernel<<<tgrid, FFT_THREADS, tlds, ctx->stream>>>(p->nh, p->nz, p->y, sigma, tau, singular, p->lam_x,
+                                                                   p->lam_z, p->spec);
+    IBM_LAUNCH_CHECK(ctx);
+    zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 1);
+    IBM_LAUNCH_CHECK(ctx);
+    xfft_inverse_kernel<<<(rows + rpb - 1) / rpb, FFT_THREADS, xlds, ctx->stream>>>(
+        p->nx, p->lx, rows, p->spec, x, 1.0 / ((double) p->nx * p->nz), p->twx);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+} // namespace ibm
+
+using namespace ibm;
+
+extern "C" {
+
+ibm_status ibm_fft_create(ibm_ctx* ctx, const ibm_grid* g, ibm_fft** out)
+{
+    if (!ctx || !g || !out) return IBM_ERR_ARG;
+    *out = nullptr;
+    if (const char* why = fft_unsupported(*g)) return fail(ctx, IBM_ERR_BAD_GRID, "FFT solver: %s", why);
+    IBM_TRY(fft_lds_attributes(ctx));
+    auto* p = new ibm_fft;
+    p->device = ctx->device;
+    p->nx = g->axis[0].points;
+    p->ny = g->axis[1].points;
+    p->nz = g->axis[2].points;
+    p->nh = p->nx / 2 + 1;
+    p->lx = log2_exact(p->nx);
+    p->lz = log2_exact(p->nz);
+    const op_axis<double> ya = make_op_axis<double>(g->axis[1]);
+    p->y = wall_axis{ya.n, ya.r2, ya.lap_lo, ya.lap_hi, ya.id_lo, ya.id_hi};
+    p->neumann = g->axis[1].interior[0] == 1.0 && g->axis[1].interior[1] == 1.0;
+    const std::vector<double2> twx = twiddles(p->nx), twz = twiddles(p->nz);
+    const std::vector<double> lx = eigenvalues(p->nx, g->axis[0].resolution * g->axis[0].resolution, p->nh);
+    const std::vector<double> lz = eigenvalues(p->nz, g->axis[2].resolution * g->axis[2].resolution, p->nz);
+    void* m = nullptr;
+    ibm_status st = IBM_OK;
+    if ((st = fft_alloc(ctx, p, sizeof(double2) * p->nh * p->ny * (size_t) p->nz, &m)) == IBM_OK) {
+        p->spec = (double2*) m;
+        st = fft_alloc(ctx, p, sizeof(double2) * twx.size(), &m);
+    }
+    if (st == IBM_OK) {
+        p->twx = (double2*) m;
+        st = fft_alloc(ctx, p, sizeof(double2) * twz.size(), &m);
+    }
+    if (st == IBM_OK) {
+        p->twz = (double2*) m;
+        st = fft_alloc(ctx, p, sizeof(double) * lx.size(), &m);
+    }
+    if (st == IBM_OK) {
+        p->lam_x = (double*) m;
+        st = fft_alloc(ctx, p, sizeof(double) * lz.size(), &m);
+    }
+    if (st == IBM_OK) {
+        p->lam_z = (double*) m;
+        if (hipMemcpy(p->twx, twx.data(), sizeof(double2) * twx.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->twz, twz.data(), sizeof(double2) * twz.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->lam_x, lx.data(), sizeof(double) * lx.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->lam_z, lz.data(), sizeof(double) * lz.size(), hipMemcpyHostToDevice) != hipSuccess)
+            st = fail(ctx, IBM_ERR_HIP, "FFT solver: table upload failed");
+    }
+    if (st != IBM_OK) {
+        for (void* q : p->owned) hipFree(q);
+        delete p;
+        return st;
+    }
+    *out = p;
+    return IBM_OK;
+}
+
+ibm_status ibm_fft_destroy(ibm_fft* p)
+{
+    if (!p) return IBM_OK;
+    hipDeviceSynchronize();
+    for (void* q : p->owned) hipFree(q);
+    delete p;
+    return IBM_OK;
+}
+
+ibm_status ibm_fft_solve(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, const double* b, double* x)
+{
+    return ibm::fft_solve(ctx, p, sigma, tau, b, x);
+}
+
+} // extern "C"

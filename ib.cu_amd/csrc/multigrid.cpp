@@ -355,6 +355,80 @@ static ibm_status solve_fixed(ibm_ctx* ctx, ibm_mg* m, const double* b, double* 
     return IBM_OK;
 }
 
+// Several independent fixed-count solves (each on its own context, so on
+// its own stream), issued cycle by cycle in turn: cycle k of every solve
+// before cycle k + 1 of any.  Each solve runs exactly the kernels of
+// solve_fixed; only the issue order changes.  With events, the streams also
+// join into ctxs[0]'s and fork again after every cycle, so that a graph
+// captured from them consists of one short parallel section per cycle: a
+// graph executor that submits each parallel branch whole before the next
+// then still keeps every stream supplied.
+template <typename T>
+static ibm_status solve_fixed_multi(int count, ibm_ctx* const* ctxs, ibm_mg* const* ms, const double* const* bs,
+                                    double* const* xs, const int* iters, bool warm, double* const* relres_dev,
+                                    hipEvent_t fork_ev, const hipEvent_t* join_ev)
+{
+    auto rejoin = [&]() -> ibm_status {
+        if (!fork_ev || count < 2) return IBM_OK;
+        for (int k = 1; k < count; ++k) {
+            IBM_HIP(ctxs[0], hipEventRecord(join_ev[k - 1], ctxs[k]->stream));
+            IBM_HIP(ctxs[0], hipStreamWaitEvent(ctxs[0]->stream, join_ev[k - 1], 0));
+        }
+        IBM_HIP(ctxs[0], hipEventRecord(fork_ev, ctxs[0]->stream));
+        for (int k = 1; k < count; ++k) IBM_HIP(ctxs[0], hipStreamWaitEvent(ctxs[k]->stream, fork_ev, 0));
+        return IBM_OK;
+    };
+    std::vector<const double*> b(bs, bs + count);
+    int most = 0;
+    for (int k = 0; k < count; ++k) {
+        ibm_mg* m = ms[k];
+        const size_t n = m->lv[0].n;
+        if (m->singular) {
+            IBM_TRY(launch_convert<double, double>(ctxs[k], n, b[k], m->rhs));
+            IBM_TRY(launch_remove_mean<double>(ctxs[k], n, m->rhs, m->partial, m->scalars + 1));
+            b[k] = m->rhs;
+        }
+        IBM_TRY(start(ctxs[k], m, b[k], xs[k], warm));
+        if (fused<T>(m) && iters[k] > 0) IBM_TRY(launch_convert<T, double>(ctxs[k], n, m->r, bufs<T>(m)[0].b));
+        most = std::max(most, iters[k]);
+    }
+    for (int it = 0; it < most; ++it) {
+        for (int k = 0; k < count; ++k)
+            if (it < iters[k])
+                IBM_TRY(correct<T>(ctxs[k], ms[k], b[k], xs[k], relres_dev[k] && it == iters[k] - 1));
+        if (it + 1 < most) IBM_TRY(rejoin());
+    }
+    for (int k = 0; k < count; ++k) {
+        ibm_mg* m = ms[k];
+        const size_t n = m->lv[0].n;
+        if (m->singular) IBM_TRY(launch_remove_mean<double>(ctxs[k], n, xs[k], m->partial, m->scalars + 1));
+        if (relres_dev[k]) {
+            IBM_TRY(launch_dot<double, double>(ctxs[k], n, b[k], b[k], m->partial, m->scalars + 2));
+            IBM_TRY(launch_dot<double, double>(ctxs[k], n, m->r, m->r, m->partial, m->scalars + 3));
+            IBM_TRY(launch_sqrt_ratio(ctxs[k], m->scalars + 3, m->scalars + 2, relres_dev[k]));
+        }
+    }
+    return IBM_OK;
+}
+
+ibm_status mg_solve_fixed_multi(int count, ibm_ctx* const* ctxs, ibm_mg* const* ms, const double* const* b,
+                                double* const* x, const int* iters, bool warm, double* const* relres,
+                                hipEvent_t fork_ev, const hipEvent_t* join_ev)
+{
+    if (fork_ev && (!join_ev || count < 2)) return IBM_ERR_ARG;
+    for (int k = 1; fork_ev && k < count; ++k)
+        if (ctxs[k] == ctxs[0] || !join_ev[k - 1]) return fail(ctxs[0], IBM_ERR_ARG, "fixed multi-solve: bad fork");
+    if (count < 1 || !ctxs || !ms || !b || !x || !iters || !relres) return IBM_ERR_ARG;
+    for (int k = 0; k < count; ++k) {
+        if (!ctxs[k] || !ms[k] || !b[k] || !x[k] || b[k] == x[k] || iters[k] < 0 ||
+            ms[k]->precision != ms[0]->precision)
+            return fail(ctxs[0], IBM_ERR_ARG, "fixed multi-solve: bad entry %d", k);
+    }
+    if (ms[0]->precision == 64)
+        return solve_fixed_multi<double>(count, ctxs, ms, b, x, iters, warm, relres, fork_ev, join_ev);
+    return solve_fixed_multi<float>(count, ctxs, ms, b, x, iters, warm, relres, fork_ev, join_ev);
+}
+
 // Block-target autotuning of the z-marching kernels.  For every level above
 // the coarse end whose grid the process has not timed yet, each kernel
 // (smoothing, residual + restriction, prolongation + residual) runs on the

@@ -398,6 +398,20 @@ int f64_variant(ibm_ctx* ctx, hipStream_t stream, const gemm_groups& G, int bloc
     return pick;
 }
 
+// Sizes ctx's packing scratch for a batch of rbf_eval_range, so that a
+// later evaluation on ctx (inside a graph capture, say) allocates nothing.
+ibm_status rbf_reserve(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N)
+{
+    size_t xbytes = 0;
+    for (int b = 0; b < count; ++b) {
+        if (!ops[b]) return fail(ctx, IBM_ERR_ARG, "bad batch entry %d", b);
+        xbytes += (size_t) round_up(std::max(N[b], 1), TILE) * ops[b]->Kp * elem_bytes(ops[b]->mode) *
+                  planes(ops[b]->mode);
+    }
+    void* p;
+    return scratch(ctx, SLOT_RBF_X, xbytes, &p);
+}
+
 ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed)

@@ -119,6 +119,12 @@ struct ibm_solver {
     int hist[16]{};                    // tracker exponents of the last steps
     int hist_len = 0, hist_next = 0;
     ibm_mg* pois = nullptr;
+    // Direct FFT x tridiagonal solves (channel grids, unless multigrid is
+    // forced): one solver per velocity grid and one for the pressure.
+    bool direct = false;
+    ibm_fft* fft_u[3]{};
+    ibm_fft* fft_p = nullptr;
+    double* dpartial[3]{};             // per-component reduction partials of the residual checks
     ins_boundary_op Bw[3]{};           // ins::boundary(g[i], i), order 2 (pseudo-pressure walls)
     struct type_t {
         ibm_rbf* ops;
@@ -149,6 +155,7 @@ struct ibm_solver {
     // (outputs of the step that nothing later in it reads) on the first
     // auxiliary stream, beside the tether forces, spreading and advection.
     hipEvent_t geo_ev = nullptr;
+    hipEvent_t bend_fork_ev = nullptr, bend_ev = nullptr;   // bending chain fork / join
     double history_dt = 0;
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
@@ -206,6 +213,8 @@ void release(ibm_solver* s)
     }
     if (s->fork_ev) (void) hipEventDestroy(s->fork_ev);
     if (s->geo_ev) (void) hipEventDestroy(s->geo_ev);
+    if (s->bend_fork_ev) (void) hipEventDestroy(s->bend_fork_ev);
+    if (s->bend_ev) (void) hipEventDestroy(s->bend_ev);
     if (s->par.fscale > 0) {
         for (int e = 0; e < 5; ++e)
             for (int i = 0; i < 3; ++i)
@@ -215,6 +224,9 @@ void release(ibm_solver* s)
             if (s->helm[i]) ibm_mg_destroy(s->helm[i]);
     }
     if (s->pois) ibm_mg_destroy(s->pois);
+    for (int i = 0; i < 3; ++i)
+        if (s->fft_u[i]) ibm_fft_destroy(s->fft_u[i]);
+    if (s->fft_p) ibm_fft_destroy(s->fft_p);
     hipDeviceSynchronize();
     for (void* p : s->owned) hipFree(p);
     delete s;
@@ -256,12 +268,21 @@ ibm_status track(ibm_ctx* ctx, ibm_solver* s)
     s->last_exponent = e;
     if (e < 0) return fail(ctx, IBM_ERR_DT_TOO_SMALL, "time step bound %.3e below kmin %.3e", kmax, P.timestep);
     s->dt = P.timestep * (double) (1 << e);
+    if (s->direct) return IBM_OK;
     if (!s->helm_e[e][0]) {
         const double lambda = s->dt * P.diffusivity / 2;
         for (int i = 0; i < 3; ++i) IBM_TRY(ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm_e[e][i]));
     }
     for (int i = 0; i < 3; ++i) s->helm[i] = s->helm_e[e][i];
     return IBM_OK;
+}
+
+// ||r|| / ||b|| into *out (device), from the residual r of a direct solve.
+ibm_status direct_relres(ibm_ctx* c, size_t n, const double* b, const double* r, double* partial, double* out)
+{
+    IBM_TRY(launch_dot<double, double>(c, n, b, b, partial, out + 8));
+    IBM_TRY(launch_dot<double, double>(c, n, r, r, partial, out + 12));
+    return launch_sqrt_ratio(c, out + 12, out + 8, out);
 }
 
 // One timestep; `fixed` selects fixed V-cycle counts and no host reads.
@@ -275,6 +296,55 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     // that every launch is timed).  Everything forked is joined back into
     // the caller's stream before the projection.
     const bool concurrent = fixed && !ctx->profiling && s->aux[0] && s->aux[1];
+
+    // Bending's chain: the data-site geometry of every bending type (one
+    // launch), the mean curvature at the data sites, its derivatives at the
+    // sample sites (one launch).  It needs only the data sites, so
+    // fixed-count steps fork it onto the first auxiliary stream before the
+    // sample-site geometry and join it before the force pass (the auxiliary
+    // context packs the operands into its own scratch).
+    auto bending_chain = [&](ibm_ctx* bc) -> ibm_status {
+        ibm_rbf* dops[MAX_TYPES];
+        ibm_rbf* hops[MAX_TYPES];
+        int dN[MAX_TYPES], hN[MAX_TYPES];
+        const double* dX[MAX_TYPES];
+        const double* hX[MAX_TYPES];
+        double* dY[MAX_TYPES];
+        double* hY[MAX_TYPES];
+        int nb = 0;
+        for (int t = 0; t < nt; ++t) {
+            auto& T = s->types[t];
+            if (!T.nc || !T.bend) continue;
+            dops[nb] = T.dops;
+            dN[nb] = 3 * T.nc;
+            dX[nb] = st->x[t];
+            dY[nb] = T.Yd;
+            hops[nb] = T.ops;
+            hN[nb] = T.nc;
+            hX[nb] = T.h;
+            hY[nb] = T.hd;
+            ++nb;
+        }
+        if (!nb) return IBM_OK;
+        unsigned char* dpacked = nullptr;
+        IBM_TRY(step_status(ctx, bc, rbf_eval_range(bc, bc->stream, nb, dops, dN, dX, dY, 0, 5, true, &dpacked)));
+        for (int t = 0; t < nt; ++t) {
+            auto& T = s->types[t];
+            if (!T.nc || !T.bend) continue;
+            IBM_TRY(step_status(ctx, bc, mean_curvature(bc, bc->stream, T.nd, T.nc, T.Yd,
+                                                         T.mem.preferred ? T.h0 : nullptr, T.h)));
+        }
+        unsigned char* hpacked = nullptr;
+        return step_status(ctx, bc, rbf_eval_range(bc, bc->stream, nb, hops, hN, hX, hY, 1, 6, true, &hpacked));
+    };
+    bool any_bending = false;
+    for (int t = 0; t < nt; ++t) any_bending = any_bending || (s->types[t].bend && s->types[t].nc);
+    const bool bend_forked = concurrent && any_bending;
+    if (bend_forked) {
+        IBM_HIP(ctx, hipEventRecord(s->bend_fork_ev, ctx->stream));
+        IBM_HIP(ctx, hipStreamWaitEvent(s->aux[0]->stream, s->bend_fork_ev, 0));
+        IBM_TRY(bending_chain(s->aux[0]));
+    }
 
     // 1. geometry: positions, tangents, normals and surface measures of
     // every cell type (the positions in one launch for all types, the
@@ -337,37 +407,11 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         // derivatives at the sample sites (one launch), then per type the
         // tension + bending (+ tether) pass
         if (any_membrane) {
-            ibm_rbf* dops[MAX_TYPES];
-            ibm_rbf* hops[MAX_TYPES];
-            int dN[MAX_TYPES], hN[MAX_TYPES];
-            const double* dX[MAX_TYPES];
-            const double* hX[MAX_TYPES];
-            double* dY[MAX_TYPES];
-            double* hY[MAX_TYPES];
-            int nb = 0;
-            for (int t = 0; t < nt; ++t) {
-                auto& T = s->types[t];
-                if (!T.nc || !T.bend) continue;
-                dops[nb] = T.dops;
-                dN[nb] = 3 * T.nc;
-                dX[nb] = st->x[t];
-                dY[nb] = T.Yd;
-                hops[nb] = T.ops;
-                hN[nb] = T.nc;
-                hX[nb] = T.h;
-                hY[nb] = T.hd;
-                ++nb;
-            }
-            if (nb) {
-                unsigned char* dpacked = nullptr;
-                IBM_TRY(rbf_eval_range(ctx, ctx->stream, nb, dops, dN, dX, dY, 0, 5, true, &dpacked));
-                for (int t = 0; t < nt; ++t) {
-                    auto& T = s->types[t];
-                    if (!T.nc || !T.bend) continue;
-                    IBM_TRY(mean_curvature(ctx, ctx->stream, T.nd, T.nc, T.Yd, T.mem.preferred ? T.h0 : nullptr, T.h));
-                }
-                unsigned char* hpacked = nullptr;
-                IBM_TRY(rbf_eval_range(ctx, ctx->stream, nb, hops, hN, hX, hY, 1, 6, true, &hpacked));
+            if (bend_forked) {
+                IBM_HIP(ctx, hipEventRecord(s->bend_ev, s->aux[0]->stream));
+                IBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->bend_ev, 0));
+            } else {
+                IBM_TRY(bending_chain(ctx));
             }
             for (int t = 0; t < nt; ++t) {
                 auto& T = s->types[t];
@@ -432,12 +476,22 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             IBM_TRY(step_status(ctx, c, launch_ins_boundary(c, s->Bw[i], 0.5 * dt * nu * dt, s->F[i], 1.0,
                                                              s->rhs[i])));
         }
+        if (s->direct) {
+            // du = A^-1 rhs exactly; H[i] (consumed) receives rhs - A du for
+            // the FP64 residual check.
+            const double lambda = dt * nu / 2;
+            IBM_TRY(step_status(ctx, c, fft_solve(c, s->fft_u[i], 1.0, -lambda, s->rhs[i], s->du[i])));
+            const op3<double> A = make_op<double>(s->g[i], 1.0, -lambda);
+            IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, A, -1.0, s->du[i], 0.0, nullptr, 1.0, s->rhs[i],
+                                                                s->H[i])));
+            IBM_TRY(step_status(ctx, c, direct_relres(c, n, s->rhs[i], s->H[i], s->dpartial[i], s->info + i)));
+            IBM_TRY(step_status(ctx, c, launch_axpby<double, double>(c, n, 1.0, s->du[i], 1.0, st->u[i])));
+            s->last_iters[i] = 1;
+            continue;
+        }
         extrapolate_kernel<<<blocks(n), 256, 0, c->stream>>>(n, s->warm_hist, s->du[i], s->du_prev[i]);
         IBM_LAUNCH_CHECK(c);
-        if (fixed) {
-            IBM_TRY(step_status(ctx, c, ibm_mg_solve_fixed(c, s->helm[i], s->rhs[i], s->du[i], s->max_iters[i] + 1, 1,
-                                                            s->info + i)));
-        } else {
+        if (!fixed) {
             int it = 0;
             double rr = 0;
             IBM_TRY(ibm_mg_solve_atol(ctx, s->helm[i], s->rhs[i], s->du[i], P.tolerance, 0.0, P.max_iterations, 1,
@@ -445,8 +499,29 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             s->last_iters[i] = it;
             IBM_HIP(ctx, hipMemcpyAsync(s->info + i, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            IBM_TRY(launch_axpby<double, double>(ctx, n, 1.0, s->du[i], 1.0, st->u[i]));
         }
-        IBM_TRY(step_status(ctx, c, launch_axpby<double, double>(c, n, 1.0, s->du[i], 1.0, st->u[i])));
+    }
+    if (fixed && !s->direct) {
+        // The three solves cycle by cycle in turn, each on its component's
+        // stream (ibm_mg_solve_fixed each).
+        ibm_mg* ms[3] = {s->helm[0], s->helm[1], s->helm[2]};
+        const double* bs[3] = {s->rhs[0], s->rhs[1], s->rhs[2]};
+        double* xs[3] = {s->du[0], s->du[1], s->du[2]};
+        double* rr[3] = {s->info + 0, s->info + 1, s->info + 2};
+        const int its[3] = {s->max_iters[0] + 1, s->max_iters[1] + 1, s->max_iters[2] + 1};
+        for (int i = 1; i < 3; ++i) cc[i]->error.clear();
+        const ibm_status mst = mg_solve_fixed_multi(3, cc, ms, bs, xs, its, true, rr, concurrent ? s->fork_ev : nullptr,
+                                                    concurrent ? s->join_ev : nullptr);
+        if (mst != IBM_OK) {
+            // the message is on the context of the solve that failed
+            for (int i = 1; i < 3; ++i)
+                if (cc[i] != ctx && !cc[i]->error.empty()) ctx->error = cc[i]->error;
+            return mst;
+        }
+        for (int i = 0; i < 3; ++i)
+            IBM_TRY(step_status(ctx, cc[i], launch_axpby<double, double>(cc[i], s->nu[i], 1.0, s->du[i], 1.0,
+                                                                          st->u[i])));
     }
     if (concurrent) {
         for (int k = 0; k < 2; ++k) {
@@ -460,6 +535,16 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
         scale_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, 1.0 / dt, s->div);
         IBM_LAUNCH_CHECK(ctx);
+        if (s->direct) {
+            // mean-free p with L p = div - mean(div); p_prev receives the
+            // residual for the FP64 check
+            IBM_TRY(fft_solve(ctx, s->fft_p, 0.0, 1.0, s->div, st->p));
+            const op3<double> Lp = make_op<double>(s->gp, 0.0, 1.0);
+            IBM_TRY(launch_remove_mean<double>(ctx, s->np, s->div, s->partial, s->info + 7));
+            IBM_TRY(launch_stencil<double>(ctx, Lp, -1.0, st->p, 0.0, nullptr, 1.0, s->div, s->p_prev));
+            IBM_TRY(direct_relres(ctx, s->np, s->div, s->p_prev, s->partial, s->info + 3));
+            s->last_iters[3] = 1;
+        } else {
         extrapolate_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, s->warm_hist, st->p, s->p_prev);
         IBM_LAUNCH_CHECK(ctx);
         if (fixed) {
@@ -485,6 +570,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             s->last_iters[3] = it;
             IBM_HIP(ctx, hipMemcpyAsync(s->info + 3, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             IBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
         }
         double* out[3] = {st->u[0], st->u[1], st->u[2]};
         IBM_TRY(ibm_gradient(ctx, &s->gp, -dt, st->p, 1.0, out));
@@ -521,7 +607,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         }
         ++s->adaptive_steps;
     } else {
-        for (int i = 0; i < 4; ++i) s->last_iters[i] = s->max_iters[i] + 1;
+        for (int i = 0; i < 4; ++i) s->last_iters[i] = s->direct ? 1 : s->max_iters[i] + 1;
     }
     return IBM_OK;
 }
@@ -544,7 +630,8 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         !(P.fscale >= 0))
         return fail(ctx, IBM_ERR_ARG, "bad solver parameters");
     if (P.precision != 64 && P.precision != 32) return fail(ctx, IBM_ERR_ARG, "precision must be 64 or 32");
-    if (P.flags & ~IBM_SOLVER_PSEUDO_PRESSURE) return fail(ctx, IBM_ERR_ARG, "unknown solver flags 0x%x", P.flags);
+    if (P.flags & ~(IBM_SOLVER_PSEUDO_PRESSURE | IBM_SOLVER_MULTIGRID))
+        return fail(ctx, IBM_ERR_ARG, "unknown solver flags 0x%x", P.flags);
     auto* s = new ibm_solver;
     s->par = P;
     ibm_status st = ibm_mac_grids(domain, ndims, refinement, s->g);
@@ -572,12 +659,22 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     s->tile_spread = ndims == 3 && tiles_usable(ctx, s->g);
     s->np = (size_t) ibm_grid_size(&s->gp);
     s->dt = P.timestep;
-    if (!(P.fscale > 0)) {
-        const double lambda = P.timestep * P.diffusivity / 2;
-        for (int i = 0; i < 3 && st == IBM_OK; ++i)
-            st = ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm[i]);
+    // Channel grids (periodic x and z of power-of-two sizes, walls in y) are
+    // solved directly; others, or IBM_SOLVER_MULTIGRID, by multigrid.
+    s->direct = ndims == 3 && !(P.flags & IBM_SOLVER_MULTIGRID) && fft_usable(s->gp);
+    for (int i = 0; i < 3; ++i) s->direct = s->direct && fft_usable(s->g[i]);
+    if (s->direct) {
+        for (int i = 0; i < 3 && st == IBM_OK; ++i) st = ibm_fft_create(ctx, &s->g[i], &s->fft_u[i]);
+        if (st == IBM_OK) st = ibm_fft_create(ctx, &s->gp, &s->fft_p);
+        for (int i = 0; i < 3 && st == IBM_OK; ++i) st = alloc(ctx, s, REDUCE_PARTS, &s->dpartial[i]);
+    } else {
+        if (!(P.fscale > 0)) {
+            const double lambda = P.timestep * P.diffusivity / 2;
+            for (int i = 0; i < 3 && st == IBM_OK; ++i)
+                st = ibm_mg_create(ctx, &s->g[i], 1.0, -lambda, P.precision, &s->helm[i]);
+        }
+        if (st == IBM_OK) st = ibm_mg_create(ctx, &s->gp, 0.0, 1.0, P.precision, &s->pois);
     }
-    if (st == IBM_OK) st = ibm_mg_create(ctx, &s->gp, 0.0, 1.0, P.precision, &s->pois);
     for (int i = 0; i < 3 && st == IBM_OK; ++i) {
         if ((st = alloc(ctx, s, s->nu[i], &s->F[i]))) break;
         if ((st = alloc(ctx, s, s->nu[i], &s->H[i]))) break;
@@ -605,7 +702,9 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         s->aux[k]->stream = s->aux_stream[k];
     }
     if (st == IBM_OK && (hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess ||
-                         hipEventCreateWithFlags(&s->geo_ev, hipEventDisableTiming) != hipSuccess))
+                         hipEventCreateWithFlags(&s->geo_ev, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&s->bend_fork_ev, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&s->bend_ev, hipEventDisableTiming) != hipSuccess))
         st = fail(ctx, IBM_ERR_HIP, "solver: fork event");
     if (st == IBM_OK) {
         double* h = nullptr;
@@ -617,8 +716,8 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         }
     }
     if (st == IBM_OK) st = alloc(ctx, s, s->np, &s->div);
-    if (st == IBM_OK) st = alloc(ctx, s, 8, &s->info);
-    if (st == IBM_OK) st = launch_fill<double>(ctx, 8, 0.0, s->info);
+    if (st == IBM_OK) st = alloc(ctx, s, 16, &s->info);
+    if (st == IBM_OK) st = launch_fill<double>(ctx, 16, 0.0, s->info);
     if (st == IBM_OK) st = alloc(ctx, s, REDUCE_PARTS, &s->partial);
     for (int t = 0; t < ntypes && st == IBM_OK; ++t) {
         const ibm_cell_type& c = types[t];
@@ -700,6 +799,28 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
             }
         }
         s->types.push_back(T);
+    }
+    // The bending chain of fixed-count steps packs its operands on the first
+    // auxiliary context: size that scratch now, so that a captured step
+    // allocates nothing.
+    if (st == IBM_OK && s->aux[0]) {
+        ibm_rbf* dops[MAX_TYPES];
+        ibm_rbf* hops[MAX_TYPES];
+        int dN[MAX_TYPES], hN[MAX_TYPES];
+        int nb = 0;
+        for (auto& T : s->types)
+            if (T.bend && T.nc) {
+                dops[nb] = T.dops;
+                hops[nb] = T.ops;
+                dN[nb] = 3 * T.nc;
+                hN[nb] = T.nc;
+                ++nb;
+            }
+        if (nb) {
+            st = rbf_reserve(s->aux[0], nb, dops, dN);
+            if (st == IBM_OK) st = rbf_reserve(s->aux[0], nb, hops, hN);
+            if (st != IBM_OK) st = step_status(ctx, s->aux[0], st);
+        }
     }
     if (st == IBM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(ctx, IBM_ERR_HIP, "solver setup");
     if (st != IBM_OK) {

@@ -123,6 +123,9 @@ _SIGS = {
     "ibm_mg_destroy": (_I, [_VP]),
     "ibm_mg_levels": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(ibm_grid), _I]),
     "ibm_mg_vcycle": (_I, [_VP, _VP, _VP, _VP]),
+    "ibm_fft_create": (_I, [_VP, ctypes.POINTER(ibm_grid), ctypes.POINTER(_VP)]),
+    "ibm_fft_destroy": (_I, [_VP]),
+    "ibm_fft_solve": (_I, [_VP, _VP, _D, _D, _VP, _VP]),
     "ibm_mg_autotune": (_I, [_I, _I]),
     "ibm_mg_shape": (_I, [_VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ibm_mg_solve": (_I, [_VP, _VP, _VP, _VP, _D, _I, ctypes.POINTER(_I), _PD]),
@@ -495,6 +498,34 @@ def axpby(ctx, a, x, b, y):
     return y
 
 
+class FftSolver:
+    """Direct solver of sigma I~ + tau L on channel grids (ibm_fft_*)."""
+
+    def __init__(self, ctx, grid):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().ibm_fft_create(ctx.handle, ctypes.byref(grid), ctypes.byref(h)), ctx)
+        self.handle = h
+
+    def solve(self, b, op, x=None):
+        import torch
+        if x is None:
+            x = torch.empty_like(b)
+        _check(lib().ibm_fft_solve(self.ctx.handle, self.handle, op[0], op[1], _ptr(b), _ptr(x)), self.ctx)
+        return x
+
+    def close(self):
+        if self.handle:
+            lib().ibm_fft_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def mg_autotune(enable=True, clear=False):
     """Switches the multigrid block-target autotuning (ibm_mg_autotune);
     clear=True forgets the targets recorded so far."""
@@ -783,6 +814,7 @@ def boundary_apply(ctx, grid, view, side, y, value=0.0, values=None, alpha=1.0, 
 # ----------------------------------------------------------------------
 MAX_CELL_TYPES = 8
 PSEUDO_PRESSURE = 1       # IBM_SOLVER_PSEUDO_PRESSURE
+MULTIGRID = 2             # IBM_SOLVER_MULTIGRID
 
 
 class ibm_solver_params(ctypes.Structure):
@@ -960,11 +992,13 @@ class Solver:
 
     def __init__(self, ctx, domain, refinement, timestep, diffusivity, density=1.0, tolerance=1e-10,
                  delta="cosine", precision=64, max_iterations=100, cell_types=(), fscale=0.0,
-                 pseudo_pressure=False):
+                 pseudo_pressure=False, multigrid=False):
         """With fscale > 0, `timestep` is the tracker's smallest step k_min;
-        pseudo_pressure selects the u* = g + dt grad p wall condition."""
+        pseudo_pressure selects the u* = g + dt grad p wall condition;
+        multigrid forces V-cycle solves on channel grids, which are
+        otherwise solved directly (FFT x tridiagonal)."""
         self.ctx = ctx
-        flags = PSEUDO_PRESSURE if pseudo_pressure else 0
+        flags = (PSEUDO_PRESSURE if pseudo_pressure else 0) | (MULTIGRID if multigrid else 0)
         par = ibm_solver_params(timestep, diffusivity, density, tolerance, DELTA.get(delta, delta), precision,
                                 max_iterations, flags, fscale)
         n = len(cell_types)

@@ -295,6 +295,20 @@ ibm_status ibm_mg_vcycle(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x);
 ibm_status ibm_mg_solve(ibm_ctx* ctx, ibm_mg* mg, const double* b, double* x, double rtol, int maxit,
                         int* iters, double* relres);
 
+/* Direct solves of A x = b, A = sigma I~ + tau L, on grids periodic in x
+ * and z with power-of-two point counts (4 to 512) and walled in y (2 to
+ * 800 points): FFTs in x and z diagonalise the periodic parts, leaving one
+ * tridiagonal system along y per Fourier mode (Thomas elimination).  The
+ * exact solution of the system solvers::multigrid iterates on
+ * (solvers/multigrid.h:176-212); no counterpart as an algorithm in the
+ * reference.  sigma = 0 with Neumann walls is the singular Poisson case:
+ * the mean-free solution of the mean-free problem, as ibm_mg_solve.  b and
+ * x may alias.  ibm_fft_create returns IBM_ERR_BAD_GRID for other grids. */
+typedef struct ibm_fft ibm_fft;
+ibm_status ibm_fft_create(ibm_ctx* ctx, const ibm_grid* grid, ibm_fft** fft);
+ibm_status ibm_fft_destroy(ibm_fft* fft);
+ibm_status ibm_fft_solve(ibm_ctx* ctx, ibm_fft* fft, double sigma, double tau, const double* b, double* x);
+
 /* Helmholtz solves A x = b, A = sigma I + tau L (sigma != 0), on nbatch
  * independent copies of one periodic grid (a batch of small boxes, or the
  * boxes of one MAC component): solvers::multigrid's V-cycle (as
@@ -529,6 +543,10 @@ typedef struct {
  * projection the tangential wall slip is then dt |grad(p_prev - p)| instead
  * of dt |grad p|.  Without the flag u* takes the wall data g. */
 #define IBM_SOLVER_PSEUDO_PRESSURE 1
+/* Solve the Helmholtz and pressure systems by multigrid even where the grids
+ * allow the direct FFT x tridiagonal solves (ibm_fft_*: periodic x and z of
+ * power-of-two sizes, walls in y), which are used there otherwise. */
+#define IBM_SOLVER_MULTIGRID 2
 
 typedef struct {
     ibm_rbf* ops;            /* operators data -> sample: position, d/dtheta, d/dphi, and with a

@@ -7,7 +7,9 @@ within 1e-10 relative (max norm) of the oracle (solves to rtol 1e-12 on both
 sides; sparse direct solves in the oracle; measured <= 7.6e-12); the mixed
 form (FP32 V-cycles inside FP64 defect correction) within 1e-12 of the FP64
 form (measured 6.4e-14); the sheared channel within 1e-10 of the oracle; a
-replayed HIP graph bitwise equal to the same fixed-count step run eagerly."""
+replayed HIP graph bitwise equal to the same fixed-count step run eagerly.
+Channel domains run both solve paths: the direct FFT x tridiagonal solves
+(the default there) and forced multigrid."""
 import os
 
 import numpy as np
@@ -38,12 +40,12 @@ def types():
     return C.cell_types()
 
 
-def _solver(ibm, ctx, types, walls, precision=64, rbf="f64", tol=1e-12, pseudo_pressure=False):
+def _solver(ibm, ctx, types, walls, precision=64, rbf="f64", tol=1e-12, pseudo_pressure=False, multigrid=False):
     ops = [ibm.RbfOperator(ctx, rbf, [_t(T["E"]), _t(T["Et"]), _t(T["Ep"])], T["nd"]) for T in types]
     ct = [{"ops": op, "cells": T["nc"], "weights": _t(T["w"]), "tether": _t(T["ref"].reshape(-1)),
            "stiffness": T["k"]} for op, T in zip(ops, types)]
     s = ibm.Solver(ctx, C.domain(ibm, walls), REF, C.DT, C.NU, density=C.RHO, tolerance=tol, precision=precision,
-                   max_iterations=60, cell_types=ct, pseudo_pressure=pseudo_pressure)
+                   max_iterations=60, cell_types=ct, pseudo_pressure=pseudo_pressure, multigrid=multigrid)
     st = s.zero_state([_t(T["X"]) for T in types])
     return s, st, ops
 
@@ -53,9 +55,12 @@ def _oracle(orc, types, walls, pseudo_pressure=False):
                          pseudo_pressure=pseudo_pressure)
 
 
-@pytest.mark.parametrize("walls,pseudo", [(False, False), (True, False), (True, True)])
-def test_three_steps_match_oracle(ibm, orc, ctx, types, walls, pseudo):
-    s, st, ops = _solver(ibm, ctx, types, walls, pseudo_pressure=pseudo)
+# Channel domains (walls) are solved directly (FFT x tridiagonal) unless
+# multigrid is forced; the periodic box always uses multigrid.
+@pytest.mark.parametrize("walls,pseudo,mg", [(False, False, False), (True, False, False), (True, True, False),
+                                             (True, False, True), (True, True, True)])
+def test_three_steps_match_oracle(ibm, orc, ctx, types, walls, pseudo, mg):
+    s, st, ops = _solver(ibm, ctx, types, walls, pseudo_pressure=pseudo, multigrid=mg)
     ost = _oracle(orc, types, walls, pseudo_pressure=pseudo)
     u = [np.zeros(st.u[i].numel()) for i in range(3)]
     p = np.zeros(st.p.numel())
@@ -86,8 +91,8 @@ def test_three_steps_match_oracle(ibm, orc, ctx, types, walls, pseudo):
 
 
 def test_mixed_precision_matches_fp64(ibm, ctx, types):
-    a, sa, _ = _solver(ibm, ctx, types, True, precision=64, tol=1e-11)
-    b, sb, _ = _solver(ibm, ctx, types, True, precision=32, tol=1e-11)
+    a, sa, _ = _solver(ibm, ctx, types, True, precision=64, tol=1e-11, multigrid=True)
+    b, sb, _ = _solver(ibm, ctx, types, True, precision=32, tol=1e-11, multigrid=True)
     for _ in range(3):
         a.step(sa)
         b.step(sb)
@@ -100,15 +105,16 @@ def test_mixed_precision_matches_fp64(ibm, ctx, types):
     b.close()
 
 
-def test_graph_replay_matches_fixed_steps(ibm, ctx, types):
+@pytest.mark.parametrize("mg", [True, False])
+def test_graph_replay_matches_fixed_steps(ibm, ctx, types, mg):
     """A replayed HIP graph equals the same fixed-count step run eagerly by a
     second, identically calibrated solver (each solver keeps its own
     warm-start state), bitwise, with unrelated eager work between replays."""
     import torch
     side = torch.cuda.Stream()                   # capture needs a created stream
     gctx = ibm.Context(0, stream=side)
-    s, st, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10)
-    e, eager, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10)
+    s, st, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10, multigrid=mg)
+    e, eager, _ = _solver(ibm, gctx, types, True, precision=32, tol=1e-10, multigrid=mg)
     torch.cuda.synchronize()
     s.step(st)                                   # calibrates the V-cycle counts
     e.step(eager)
