@@ -78,9 +78,6 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
                           const double* const* X, double* const* Y, int j0, int j1, bool do_pack,
                           unsigned char** packed);
 
-// Sizes ctx's packing scratch for such a batch ahead of time.
-ibm_status rbf_reserve(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N);
-
 // Membrane force passes (forces.hip) on `stream`.
 ibm_status membrane_table(ibm_ctx* ctx, hipStream_t stream, int ns, const double* geo, const double* w, double* table);
 ibm_status mean_curvature(ibm_ctx* ctx, hipStream_t stream, int nd, int nc, const double* geo, const double* h0,

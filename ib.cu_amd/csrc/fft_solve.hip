@@ -12,7 +12,7 @@
 //   = B_j,   mu = lambda_x(kx) + lambda_z(kz),
 // with the wall rows' ghost coefficients in d_0, d_{n-1} (fd/discretization.h:
 // 94-108).  The solve is: real-to-half-complex FFT along x, complex FFT
-// along z, one Thomas elimination per mode (in LDS), and the inverse
+// along z, one Thomas elimination per mode, and the inverse
 // transforms -- the exact solution of the same discrete system the multigrid
 // iterates on, in five memory passes.  The singular case (sigma = 0 with
 // Neumann walls: the pressure Poisson) solves the mean-free problem, as the
@@ -34,7 +34,8 @@ namespace ibm {
 namespace {
 
 constexpr int FFT_THREADS = 256;
-constexpr int ZT = 8;              // kx columns per z-transform / tridiagonal block
+constexpr int ZT = 8;              // kx columns per z-transform block
+constexpr int SWEEP = 32;          // rows per step of the tridiagonal sweeps (loads in flight per lane)
 constexpr int MAX_FFT_POINTS = 512;
 constexpr int MAX_WALL_POINTS = 800;
 constexpr size_t MAX_DYNAMIC_LDS = 160 * 1024;
@@ -45,9 +46,10 @@ __device__ inline int bitrev(int k, int logn) { return (int) (__brev((unsigned) 
 
 // In-place radix-2 transforms of `count` arrays of n points held in LDS in
 // bit-reversed order (a[t * n + bitrev(k)] = input k of array t).  tw[m] =
-// exp(-2 pi i m / n), m < n / 2; `inverse` conjugates the twiddles (no
-// scaling).  Every thread of the block takes part in every stage.
-__device__ void fft_stages(double2* a, int count, int n, int logn, const double2* __restrict__ tw, bool inverse)
+// exp(-2 pi i m / n), m < n / 2, also in LDS (a stage's twiddle reads would
+// otherwise be dependent global loads); `inverse` conjugates the twiddles
+// (no scaling).  Every thread of the block takes part in every stage.
+__device__ void fft_stages(double2* a, int count, int n, int logn, const double2* tw, bool inverse)
 {
     const int half_n = n >> 1;
     const int total = count * half_n;
@@ -77,13 +79,15 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_forward_kernel(int nx, int l
 {
     extern __shared__ double2 sh[];
     const int rpb = FFT_THREADS / (nx / 2), nh = nx / 2 + 1;
+    double2* stw = sh + rpb * nx;
     const int r0 = blockIdx.x * rpb;
+    for (int e = threadIdx.x; e < nx / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
         const int t = e / nx, k = e - t * nx, row = r0 + t;
         sh[t * nx + bitrev(k, logn)] = make_double2(row < rows ? x[(size_t) row * nx + k] : 0.0, 0.0);
     }
     __syncthreads();
-    fft_stages(sh, rpb, nx, logn, tw, false);
+    fft_stages(sh, rpb, nx, logn, stw, false);
     for (int e = threadIdx.x; e < rpb * nh; e += blockDim.x) {
         const int t = e / nh, k = e - t * nh, row = r0 + t;
         if (row < rows) s[(size_t) row * nh + k] = sh[t * nx + k];
@@ -99,7 +103,9 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
 {
     extern __shared__ double2 sh[];
     const int rpb = FFT_THREADS / (nx / 2), nh = nx / 2 + 1;
+    double2* stw = sh + rpb * nx;
     const int r0 = blockIdx.x * rpb;
+    for (int e = threadIdx.x; e < nx / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
         const int t = e / nx, k = e - t * nx, row = r0 + t;
         double2 v = make_double2(0.0, 0.0);
@@ -114,7 +120,7 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
         sh[t * nx + bitrev(k, logn)] = v;
     }
     __syncthreads();
-    fft_stages(sh, rpb, nx, logn, tw, true);
+    fft_stages(sh, rpb, nx, logn, stw, true);
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
         const int t = e / nx, k = e - t * nx, row = r0 + t;
         if (row < rows) x[(size_t) row * nx + k] = sh[t * nx + k].x * scale;
@@ -127,15 +133,17 @@ __global__ void __launch_bounds__(FFT_THREADS) zfft_kernel(int nh, int ny, int n
                                                            const double2* __restrict__ tw, int inverse)
 {
     extern __shared__ double2 sh[];
+    double2* stw = sh + ZT * nz;
     const int kx0 = blockIdx.x * ZT, j = blockIdx.y;
     const size_t plane = (size_t) nh * ny;
+    for (int e = threadIdx.x; e < nz / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
         const int t = e % ZT, k = e / ZT, kx = kx0 + t;
         sh[t * nz + bitrev(k, logn)] =
             kx < nh ? s[(size_t) k * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_stages(sh, ZT, nz, logn, tw, inverse != 0);
+    fft_stages(sh, ZT, nz, logn, stw, inverse != 0);
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
         const int t = e % ZT, k = e / ZT, kx = kx0 + t;
         if (kx < nh) s[(size_t) k * plane + (size_t) j * nh + kx] = sh[t * nz + k];
@@ -147,97 +155,133 @@ struct wall_axis {
     double r2, lap_lo, lap_hi, id_lo, id_hi;
 };
 
-// One Thomas elimination along y per Fourier mode: block (kx tile, kz) loads
-// ZT modes' columns into LDS, one lane per mode eliminates and substitutes
-// back, the block stores the solutions.
-__global__ void __launch_bounds__(FFT_THREADS) tridiagonal_kernel(int nh, int nz, wall_axis y, double sigma, double tau,
-                                                                  int singular, const double* __restrict__ lam_x,
-                                                                  const double* __restrict__ lam_z,
-                                                                  double2* __restrict__ s)
+// Row j's coefficients of mode (kx, kz): sub- and super-diagonal lo, hi and
+// the diagonal.
+__device__ inline void tri_row(const wall_axis& y, double sigma, double tau, double mu, int j, double& lo, double& hi,
+                               double& diag)
 {
-    extern __shared__ double2 sh[];
+    const double ty = tau * y.r2;
+    lo = j > 0 ? ty : 0.0;
+    hi = j < y.n - 1 ? ty : 0.0;
+    double d = -2.0, id = 1.0;
+    if (j == 0) {
+        d += y.lap_lo;
+        id += y.id_lo;
+    }
+    if (j == y.n - 1) {
+        d += y.lap_hi;
+        id += y.id_hi;
+    }
+    diag = id * (sigma + tau * mu) + ty * d;
+}
+
+// The elimination of every mode for one (sigma, tau), computed once:
+// cp = the eliminated super-diagonal, inv = the reciprocal pivot (0 for the
+// pinned last unknown of the singular mode), laid out like the spectrum
+// without its complex pairs.  One lane per mode.
+__global__ void __launch_bounds__(FFT_THREADS) tridiagonal_factor_kernel(int nh, int nz, wall_axis y, double sigma,
+                                                                         double tau, int singular,
+                                                                         const double* __restrict__ lam_x,
+                                                                         const double* __restrict__ lam_z,
+                                                                         double* __restrict__ cp_out,
+                                                                         double* __restrict__ inv_out)
+{
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x, kz = blockIdx.y;
+    if (kx >= nh) return;
     const int ny = y.n;
-    double2* sd = sh;                                      // ny x ZT values
-    double* sc = (double*) (sh + (size_t) ny * ZT);        // ny x ZT eliminated super-diagonals
-    const int kx0 = blockIdx.x * ZT, kz = blockIdx.y;
+    const double mu = lam_x[kx] + lam_z[kz];
+    const bool pinned = singular && kx == 0 && kz == 0;
     const size_t plane = (size_t) nh * ny;
-    for (int e = threadIdx.x; e < ZT * ny; e += blockDim.x) {
-        const int t = e % ZT, j = e / ZT, kx = kx0 + t;
-        sd[j * ZT + t] = kx < nh ? s[(size_t) kz * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < ZT && kx0 + t < nh) {
-        const int kx = kx0 + t;
-        const double mu = lam_x[kx] + lam_z[kz];
-        const bool pinned = singular && kx == 0 && kz == 0;
-        const double ty = tau * y.r2;
-        if (pinned) {
-            // mean-free right-hand side (the compatibility condition)
-            double2 m = make_double2(0.0, 0.0);
-            for (int j = 0; j < ny; ++j) {
-                m.x += sd[j * ZT + t].x;
-                m.y += sd[j * ZT + t].y;
-            }
-            m.x /= ny;
-            m.y /= ny;
-            for (int j = 0; j < ny; ++j) {
-                sd[j * ZT + t].x -= m.x;
-                sd[j * ZT + t].y -= m.y;
-            }
+    double cprev = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        double lo, hi, diag;
+        tri_row(y, sigma, tau, mu, j, lo, hi, diag);
+        double inv = 0.0, cp = 0.0;
+        if (!(pinned && j == ny - 1)) {
+            inv = 1.0 / (diag - lo * cprev);
+            cp = hi * inv;
         }
-        double cprev = 0.0;
-        double2 dprev = make_double2(0.0, 0.0);
+        const size_t at = (size_t) kz * plane + (size_t) j * nh + kx;
+        cp_out[at] = cp;
+        inv_out[at] = inv;
+        cprev = cp;
+    }
+}
+
+// One Thomas elimination along y per Fourier mode with the factors of
+// tridiagonal_factor_kernel: one lane per mode (consecutive lanes take
+// consecutive kx, so every row's accesses are coalesced), the spectrum
+// overwritten in place by d' = (r - lo d'_prev) / pivot and then by the
+// back substitution x = d' - c' x_next, SWEEP rows per step so that their
+// loads are issued before the dependent arithmetic.  The singular mode works
+// on its mean-free right-hand side and returns its mean-free solution.
+__global__ void __launch_bounds__(64) tridiagonal_kernel(int nh, int ny, double ty, int singular,
+                                                         const double* __restrict__ cp,
+                                                         const double* __restrict__ inv, double2* __restrict__ s)
+{
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x, kz = blockIdx.y;
+    if (kx >= nh) return;
+    const size_t base = (size_t) kz * nh * ny + kx;
+    auto at = [&](int j) { return base + (size_t) j * nh; };
+    const bool pinned = singular && kx == 0 && kz == 0;
+    double2 mean = make_double2(0.0, 0.0);
+    if (pinned) {
         for (int j = 0; j < ny; ++j) {
-            const double lo = j > 0 ? ty : 0.0, hi = j < ny - 1 ? ty : 0.0;
-            double d = -2.0, id = 1.0;
-            if (j == 0) {
-                d += y.lap_lo;
-                id += y.id_lo;
-            }
-            if (j == ny - 1) {
-                d += y.lap_hi;
-                id += y.id_hi;
-            }
-            const double diag = id * (sigma + tau * mu) + ty * d;
-            double cp = 0.0;
-            double2 dp = make_double2(0.0, 0.0);
-            if (!(pinned && j == ny - 1)) {
-                const double inv = 1.0 / (diag - lo * cprev);
-                cp = hi * inv;
-                const double2 r = sd[j * ZT + t];
-                dp = make_double2((r.x - lo * dprev.x) * inv, (r.y - lo * dprev.y) * inv);
-            }
-            sc[j * ZT + t] = cp;
-            sd[j * ZT + t] = dp;
-            cprev = cp;
-            dprev = dp;
+            mean.x += s[at(j)].x;
+            mean.y += s[at(j)].y;
         }
-        double2 xn = sd[(ny - 1) * ZT + t];
-        for (int j = ny - 2; j >= 0; --j) {
-            const double c = sc[j * ZT + t];
-            const double2 dj = sd[j * ZT + t];
-            xn = make_double2(dj.x - c * xn.x, dj.y - c * xn.y);
-            sd[j * ZT + t] = xn;
-        }
-        if (pinned) {
-            double2 m = make_double2(0.0, 0.0);
-            for (int j = 0; j < ny; ++j) {
-                m.x += sd[j * ZT + t].x;
-                m.y += sd[j * ZT + t].y;
-            }
-            m.x /= ny;
-            m.y /= ny;
-            for (int j = 0; j < ny; ++j) {
-                sd[j * ZT + t].x -= m.x;
-                sd[j * ZT + t].y -= m.y;
-            }
-        }
+        mean.x /= ny;
+        mean.y /= ny;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ZT * ny; e += blockDim.x) {
-        const int tt = e % ZT, j = e / ZT, kx = kx0 + tt;
-        if (kx < nh) s[(size_t) kz * plane + (size_t) j * nh + kx] = sd[j * ZT + tt];
+    double2 dprev = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < ny; j0 += SWEEP) {
+        double2 r[SWEEP];
+        double iv[SWEEP];
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u) {
+            const int j = j0 + u < ny ? j0 + u : ny - 1;
+            r[u] = s[at(j)];
+            iv[u] = inv[at(j)];
+        }
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u) {
+            const double lo = j0 + u > 0 ? ty : 0.0;
+            dprev = make_double2((r[u].x - mean.x - lo * dprev.x) * iv[u], (r[u].y - mean.y - lo * dprev.y) * iv[u]);
+            r[u] = dprev;
+        }
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u)
+            if (j0 + u < ny) s[at(j0 + u)] = r[u];
+    }
+    double2 xn = s[at(ny - 1)];
+    double2 xsum = xn;
+    for (int j1 = ny - 2; j1 >= 0; j1 -= SWEEP) {
+        double2 d[SWEEP];
+        double c[SWEEP];
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u) {
+            const int j = j1 - u >= 0 ? j1 - u : 0;
+            d[u] = s[at(j)];
+            c[u] = cp[at(j)];
+        }
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u) {
+            if (j1 - u >= 0) {
+                xn = make_double2(d[u].x - c[u] * xn.x, d[u].y - c[u] * xn.y);
+                d[u] = xn;
+                xsum.x += xn.x;
+                xsum.y += xn.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SWEEP; ++u)
+            if (j1 - u >= 0) s[at(j1 - u)] = d[u];
+    }
+    if (pinned) {
+        for (int j = 0; j < ny; ++j) {
+            s[at(j)].x -= xsum.x / ny;
+            s[at(j)].y -= xsum.y / ny;
+        }
     }
 }
 
@@ -283,10 +327,23 @@ struct ibm_fft {
     double2* twz = nullptr;
     double* lam_x = nullptr;
     double* lam_z = nullptr;
+    // Elimination factors per (sigma, tau) met outside graph capture (at
+    // most FACTOR_SETS; later pairs, and pairs first met under capture, are
+    // eliminated on the fly).  A set is never overwritten, so a captured
+    // solve keeps reading valid factors.
+    struct factor_set {
+        double sigma, tau;
+        double *cp, *inv;
+    };
+    std::vector<factor_set> factors;
+    double *scratch_cp = nullptr, *scratch_inv = nullptr;   // factors recomputed per solve when not cached
+    std::mutex lock;
     std::vector<void*> owned;
 };
 
 namespace ibm {
+
+constexpr size_t FACTOR_SETS = 8;
 
 // Whether the FFT solver takes grid g (and why not).
 static const char* fft_unsupported(const ibm_grid& g)
@@ -315,11 +372,45 @@ static ibm_status fft_lds_attributes(ibm_ctx* ctx)
     static bool done = false;
     std::lock_guard<std::mutex> guard(lock);
     if (done) return IBM_OK;
-    IBM_HIP(ctx, hipFuncSetAttribute((const void*) tridiagonal_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int) MAX_DYNAMIC_LDS));
     IBM_HIP(ctx, hipFuncSetAttribute((const void*) zfft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int) MAX_DYNAMIC_LDS));
     done = true;
+    return IBM_OK;
+}
+
+// The elimination factors of (sigma, tau) into *cp, *inv: a cached set; or
+// a new set computed now on ctx's stream; or, under graph capture (no
+// allocation) or with every set taken, the plan's scratch set recomputed by
+// this solve.
+static ibm_status fft_factors(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, int singular, const double** cp,
+                              const double** inv)
+{
+    std::lock_guard<std::mutex> guard(p->lock);
+    for (const auto& f : p->factors)
+        if (f.sigma == sigma && f.tau == tau) {
+            *cp = f.cp;
+            *inv = f.inv;
+            return IBM_OK;
+        }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    IBM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    double *c = p->scratch_cp, *v = p->scratch_inv;
+    const bool keep = cs == hipStreamCaptureStatusNone && p->factors.size() < FACTOR_SETS;
+    if (keep) {
+        const size_t n = (size_t) p->nh * p->ny * p->nz;
+        void *m0 = nullptr, *m1 = nullptr;
+        IBM_TRY(fft_alloc(ctx, p, sizeof(double) * n, &m0));
+        IBM_TRY(fft_alloc(ctx, p, sizeof(double) * n, &m1));
+        c = (double*) m0;
+        v = (double*) m1;
+    }
+    const dim3 grid((p->nh + FFT_THREADS - 1) / FFT_THREADS, p->nz);
+    tridiagonal_factor_kernel<<<grid, FFT_THREADS, 0, ctx->stream>>>(p->nh, p->nz, p->y, sigma, tau, singular,
+                                                                     p->lam_x, p->lam_z, c, v);
+    IBM_LAUNCH_CHECK(ctx);
+    if (keep) p->factors.push_back(ibm_fft::factor_set{sigma, tau, c, v});
+    *cp = c;
+    *inv = v;
     return IBM_OK;
 }
 
@@ -334,19 +425,19 @@ ibm_status fft_solve(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, const d
     // b read and x written once; the coefficients written and read four times
     prof_scope prof(ctx, "fft_solve", 8.0 * 2.0 * points + 16.0 * 8.0 * coeffs);
     const int rpb = FFT_THREADS / (p->nx / 2);
-    const size_t xlds = sizeof(double2) * rpb * p->nx;
+    const size_t xlds = sizeof(double2) * (rpb * p->nx + p->nx / 2);
     xfft_forward_kernel<<<(rows + rpb - 1) / rpb, FFT_THREADS, xlds, ctx->stream>>>(p->nx, p->lx, rows, b, p->spec,
                                                                                       p->twx);
     IBM_LAUNCH_CHECK(ctx);
     const dim3 zgrid((p->nh + ZT - 1) / ZT, p->ny);
-    const size_t zlds = sizeof(double2) * ZT * p->nz;
+    const size_t zlds = sizeof(double2) * (ZT * p->nz + p->nz / 2);
     zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 0);
     IBM_LAUNCH_CHECK(ctx);
-    const dim3 tgrid((p->nh + ZT - 1) / ZT, p->nz);
-    const size_t tlds = (sizeof(double2) + sizeof(double)) * ZT * p->ny;
     const int singular = sigma == 0.0 && p->neumann;
-    tridiagonal_kernel<<<tgrid, FFT_THREADS, tlds, ctx->stream>>>(p->nh, p->nz, p->y, sigma, tau, singular, p->lam_x,
-                                                                   p->lam_z, p->spec);
+    const double *cp = nullptr, *inv = nullptr;
+    IBM_TRY(fft_factors(ctx, p, sigma, tau, singular, &cp, &inv));
+    const dim3 tgrid((p->nh + 63) / 64, p->nz);
+    tridiagonal_kernel<<<tgrid, 64, 0, ctx->stream>>>(p->nh, p->ny, tau * p->y.r2, singular, cp, inv, p->spec);
     IBM_LAUNCH_CHECK(ctx);
     zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 1);
     IBM_LAUNCH_CHECK(ctx);
@@ -402,6 +493,14 @@ ibm_status ibm_fft_create(ibm_ctx* ctx, const ibm_grid* g, ibm_fft** out)
     }
     if (st == IBM_OK) {
         p->lam_z = (double*) m;
+        st = fft_alloc(ctx, p, sizeof(double) * p->nh * p->ny * (size_t) p->nz, &m);
+    }
+    if (st == IBM_OK) {
+        p->scratch_cp = (double*) m;
+        st = fft_alloc(ctx, p, sizeof(double) * p->nh * p->ny * (size_t) p->nz, &m);
+    }
+    if (st == IBM_OK) {
+        p->scratch_inv = (double*) m;
         if (hipMemcpy(p->twx, twx.data(), sizeof(double2) * twx.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->twz, twz.data(), sizeof(double2) * twz.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->lam_x, lx.data(), sizeof(double) * lx.size(), hipMemcpyHostToDevice) != hipSuccess ||

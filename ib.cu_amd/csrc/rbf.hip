@@ -43,7 +43,6 @@ typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 128;        // rows of a block tile (samples and columns)
 constexpr int ROWB = 128;        // bytes of one LDS row per contraction chunk
-constexpr int STRIDE = 144;      // LDS row stride in bytes
 constexpr int NTHREADS = 256;
 constexpr int KPAD = 64;         // contraction padded to a multiple of 64 elements
 constexpr int MAX_GROUPS = 24;
@@ -62,7 +61,7 @@ struct gemm_group {
 
 struct gemm_groups {
     gemm_group g[MAX_GROUPS];
-    int count, total_tiles, xcd_chunk;
+    int count, total_tiles;
 };
 
 __device__ inline uint16_t bf16_bits(float f)
@@ -139,26 +138,38 @@ __device__ inline void mfma_step(const u4& xa, const u4& eb, const u4& xa_lo, co
 // Results: lane l holds four rows of column l % 16: rows 4 (l / 16) + r,
 // r = 0..3, for the FP32-accumulating instructions, but rows (l / 16) + 4 r
 // for v_mfma_f64_16x16x4 (both measured on gfx950).
-template <int MODE, int WN>
+template <int MODE, int WN, int RB = ROWB, int NB = 1>
 __global__ __launch_bounds__(128 * WN, 2) void rbf_gemm_kernel(gemm_groups G)
 {
     constexpr int PL = planes(MODE);
     constexpr int THREADS = 128 * WN;                // waves: 2 along the samples x WN along the columns
     constexpr int JN = 8 / WN;                       // 16-column blocks per wave
-    constexpr int LOADS = 4 * NTHREADS / THREADS;    // 16-byte staging loads per thread, tile and plane
+    constexpr int RS = RB + 16;                      // LDS row stride: 16-row fragment reads hit disjoint banks
+    constexpr int LOADS = TILE * RB / 16 / THREADS;  // 16-byte staging loads per thread, tile and plane
     using acc_t = typename std::conditional<MODE == MODE_F64, d4, f4>::type;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[PL][2][TILE * STRIDE];
+    __shared__ __attribute__((aligned(16))) unsigned char ldsb[NB][PL][2][TILE * RS];
 
-    int b = blockIdx.x;
-    if (G.xcd_chunk) b = (b & 7) * G.xcd_chunk + (b >> 3);
-    if (b >= G.total_tiles) return;                  // whole block exits together
-    int gi = 0;
-    while (gi + 1 < G.count && b >= G.g[gi + 1].first_tile) ++gi;
+    // Blocks reach the XCDs round-robin (block i on XCD i % 8).  XCD x takes
+    // the x-th eighth of every group's tiles, in order, so that the XCDs get
+    // equal work however the groups' contraction lengths differ, and the
+    // tiles sharing an operator row block share an L2.
+    const int xcd = blockIdx.x & 7;
+    int k = blockIdx.x >> 3, gi = -1, t = 0;
+    for (int q = 0; q < G.count; ++q) {
+        const int tq = (q + 1 < G.count ? G.g[q + 1].first_tile : G.total_tiles) - G.g[q].first_tile;
+        const int lo = xcd * tq / 8, hi = (xcd + 1) * tq / 8;
+        if (k < hi - lo) {
+            gi = q;
+            t = lo + k;
+            break;
+        }
+        k -= hi - lo;
+    }
+    if (gi < 0) return;                              // whole block exits together
     const gemm_group g = G.g[gi];
-    int t = b - g.first_tile;
     int tm = t / g.tiles_n, tn = t - tm * g.tiles_n;
     const long row_bytes = (long) g.Kp * elem_bytes(MODE);
-    const int nchunks = (int) (row_bytes / ROWB);
+    const int nchunks = (int) (row_bytes / RB);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -178,15 +189,15 @@ __global__ __launch_bounds__(128 * WN, 2) void rbf_gemm_kernel(gemm_groups G)
 #pragma unroll
     for (int i = 0; i < LOADS; ++i) {
         int idx = tid + THREADS * i;
-        int r = idx >> 3, q = idx & 7;
+        int r = idx / (RB / 16), q = idx % (RB / 16);
         goff[i] = (long) r * row_bytes + q * 16;
-        loff[i] = r * STRIDE + q * 16;
+        loff[i] = r * RS + q * 16;
     }
 #define RBF_LOAD(c)                                                                   \
     _Pragma("unroll") for (int p = 0; p < PL; ++p)                                    \
         _Pragma("unroll") for (int i = 0; i < LOADS; ++i) {                           \
-            pe[p][i] = *(const u4*) (srcE[p] + goff[i] + (long) (c) * ROWB);       \
-            px[p][i] = *(const u4*) (srcX[p] + goff[i] + (long) (c) * ROWB);       \
+            pe[p][i] = *(const u4*) (srcE[p] + goff[i] + (long) (c) * RB);         \
+            px[p][i] = *(const u4*) (srcX[p] + goff[i] + (long) (c) * RB);         \
         }
 
     acc_t acc[JN][4];
@@ -196,32 +207,53 @@ __global__ __launch_bounds__(128 * WN, 2) void rbf_gemm_kernel(gemm_groups G)
         for (int i = 0; i < 4; ++i) acc[j][i] = acc_t{};
 
     const int frow = lane & 15, fgrp = lane >> 4;
-    RBF_LOAD(0);
-    for (int c = 0; c < nchunks; ++c) {
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < PL; ++p)
-#pragma unroll
-            for (int i = 0; i < LOADS; ++i) {
-                *(u4*) (&lds[p][0][loff[i]]) = pe[p][i];
-                *(u4*) (&lds[p][1][loff[i]]) = px[p][i];
-            }
-        __syncthreads();
-        if (c + 1 < nchunks) {
-            RBF_LOAD(c + 1);
+#define RBF_STORE(buf)                                                                \
+    _Pragma("unroll") for (int p = 0; p < PL; ++p)                                    \
+        _Pragma("unroll") for (int i = 0; i < LOADS; ++i) {                           \
+            *(u4*) (&ldsb[buf][p][0][loff[i]]) = pe[p][i];                            \
+            *(u4*) (&ldsb[buf][p][1][loff[i]]) = px[p][i];                            \
         }
+    RBF_LOAD(0);
+    if constexpr (NB == 2) {
+        // Two LDS buffers, one barrier per chunk: chunk c + 1 is stored into
+        // the buffer chunk c - 1 used (every wave left it at the last
+        // barrier) while chunk c is multiplied from the other.
+        RBF_STORE(0);
+        if (nchunks > 1) {
+            RBF_LOAD(1);
+        }
+        __syncthreads();
+    }
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = NB == 2 ? (c & 1) : 0;
+        if constexpr (NB == 2) {
+            if (c + 1 < nchunks) {
+                RBF_STORE(buf ^ 1);
+            }
+            if (c + 2 < nchunks) {
+                RBF_LOAD(c + 2);
+            }
+        } else {
+            __syncthreads();
+            RBF_STORE(0);
+            __syncthreads();
+            if (c + 1 < nchunks) {
+                RBF_LOAD(c + 1);
+            }
+        }
+        const unsigned char (*lds)[2][TILE * RS] = ldsb[buf];
 #pragma unroll
-        for (int s = 0; s < ROWB / 64; ++s) {
+        for (int s = 0; s < RB / 64; ++s) {
             u4 ef[PL][4], xf[PL][JN];
             const int off = s * 64 + fgrp * 16;
 #pragma unroll
             for (int p = 0; p < PL; ++p) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    ef[p][i] = *(const u4*) (&lds[p][0][(wm * 64 + i * 16 + frow) * STRIDE + off]);
+                    ef[p][i] = *(const u4*) (&lds[p][0][(wm * 64 + i * 16 + frow) * RS + off]);
 #pragma unroll
                 for (int j = 0; j < JN; ++j)
-                    xf[p][j] = *(const u4*) (&lds[p][1][(wn * 16 * JN + j * 16 + frow) * STRIDE + off]);
+                    xf[p][j] = *(const u4*) (&lds[p][1][(wn * 16 * JN + j * 16 + frow) * RS + off]);
             }
 #pragma unroll
             for (int j = 0; j < JN; ++j)
@@ -229,8 +261,10 @@ __global__ __launch_bounds__(128 * WN, 2) void rbf_gemm_kernel(gemm_groups G)
                 for (int i = 0; i < 4; ++i)
                     mfma_step<MODE>(xf[0][j], ef[0][i], xf[PL - 1][j], ef[PL - 1][i], acc[j][i]);
         }
+        if constexpr (NB == 2) __syncthreads();
     }
 #undef RBF_LOAD
+#undef RBF_STORE
 
     const int m_base = tm * TILE + wm * 64 + frow;
     const int n_base = tn * TILE + wn * 16 * JN + (MODE == MODE_F64 ? fgrp : fgrp * 4);
@@ -341,7 +375,9 @@ int rbf_nops(const ibm_rbf* op) { return op ? op->nops : 0; }
 
 // FP64 GEMM variants, by waves along the columns: 2 (64 x 64 wave tiles,
 // about 250 registers, two waves per SIMD) or 4 (64 x 32, at most 128
-// registers, four waves per SIMD to hide the barrier and load latency).
+// registers, four waves per SIMD, and two LDS buffers so that a chunk takes
+// one barrier: 57 against 51 TFLOP/s at the C4 RBC shape; single-buffered it
+// reached 55, with 256-byte chunks 52).
 // Both give every output the same MFMA sequence over the same chunks, so
 // their results are bitwise equal.  Unless forced, the first eager launch of
 // a shape (tiles, summed contraction length) times both and keeps the faster
@@ -354,7 +390,7 @@ constexpr int DEFAULT_WAVES_N = 4;
 
 void launch_f64(int wn, const gemm_groups& G, int blocks, hipStream_t stream)
 {
-    if (wn == 4) rbf_gemm_kernel<MODE_F64, 4><<<blocks, 512, 0, stream>>>(G);
+    if (wn == 4) rbf_gemm_kernel<MODE_F64, 4, ROWB, 2><<<blocks, 512, 0, stream>>>(G);
     else rbf_gemm_kernel<MODE_F64, 2><<<blocks, NTHREADS, 0, stream>>>(G);
 }
 
@@ -396,20 +432,6 @@ int f64_variant(ibm_ctx* ctx, hipStream_t stream, const gemm_groups& G, int bloc
         variant_table[key] = pick;
     }
     return pick;
-}
-
-// Sizes ctx's packing scratch for a batch of rbf_eval_range, so that a
-// later evaluation on ctx (inside a graph capture, say) allocates nothing.
-ibm_status rbf_reserve(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int* N)
-{
-    size_t xbytes = 0;
-    for (int b = 0; b < count; ++b) {
-        if (!ops[b]) return fail(ctx, IBM_ERR_ARG, "bad batch entry %d", b);
-        xbytes += (size_t) round_up(std::max(N[b], 1), TILE) * ops[b]->Kp * elem_bytes(ops[b]->mode) *
-                  planes(ops[b]->mode);
-    }
-    void* p;
-    return scratch(ctx, SLOT_RBF_X, xbytes, &p);
 }
 
 ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* const* ops, const int* N,
@@ -467,8 +489,17 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
     IBM_LAUNCH_CHECK(ctx);
     if (!tiles) return IBM_OK;
     G.total_tiles = tiles;
-    int blocks = round_up(tiles, 8);
-    G.xcd_chunk = blocks / 8;
+    // blocks: 8 x the most tiles any XCD takes (its share of every group)
+    int per_xcd = 0;
+    for (int x = 0; x < 8; ++x) {
+        int n = 0;
+        for (int q = 0; q < G.count; ++q) {
+            const int tq = (q + 1 < G.count ? G.g[q + 1].first_tile : tiles) - G.g[q].first_tile;
+            n += (x + 1) * tq / 8 - x * tq / 8;
+        }
+        per_xcd = std::max(per_xcd, n);
+    }
+    const int blocks = 8 * per_xcd;
     double flops = 0;
     for (int b = 0; b < count; ++b)
         flops += 2.0 * ops[b]->M * (double) N[b] * ops[b]->K * std::max(0, std::min(j1, ops[b]->nops) - j0);

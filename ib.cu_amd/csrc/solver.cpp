@@ -155,7 +155,6 @@ struct ibm_solver {
     // (outputs of the step that nothing later in it reads) on the first
     // auxiliary stream, beside the tether forces, spreading and advection.
     hipEvent_t geo_ev = nullptr;
-    hipEvent_t bend_fork_ev = nullptr, bend_ev = nullptr;   // bending chain fork / join
     double history_dt = 0;
     double* info = nullptr;            // device: residual[4], divergence, scratch scalars
     double* partial = nullptr;         // reduction partials (REDUCE_PARTS)
@@ -213,8 +212,6 @@ void release(ibm_solver* s)
     }
     if (s->fork_ev) (void) hipEventDestroy(s->fork_ev);
     if (s->geo_ev) (void) hipEventDestroy(s->geo_ev);
-    if (s->bend_fork_ev) (void) hipEventDestroy(s->bend_fork_ev);
-    if (s->bend_ev) (void) hipEventDestroy(s->bend_ev);
     if (s->par.fscale > 0) {
         for (int e = 0; e < 5; ++e)
             for (int i = 0; i < 3; ++i)
@@ -299,10 +296,9 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
 
     // Bending's chain: the data-site geometry of every bending type (one
     // launch), the mean curvature at the data sites, its derivatives at the
-    // sample sites (one launch).  It needs only the data sites, so
-    // fixed-count steps fork it onto the first auxiliary stream before the
-    // sample-site geometry and join it before the force pass (the auxiliary
-    // context packs the operands into its own scratch).
+    // sample sites (one launch).  It runs after the sample-site geometry on
+    // the same stream: beside it on a second stream both matrix-core
+    // launches slowed down more than they overlapped (5.2 against 4.5 ms).
     auto bending_chain = [&](ibm_ctx* bc) -> ibm_status {
         ibm_rbf* dops[MAX_TYPES];
         ibm_rbf* hops[MAX_TYPES];
@@ -337,14 +333,6 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         unsigned char* hpacked = nullptr;
         return step_status(ctx, bc, rbf_eval_range(bc, bc->stream, nb, hops, hN, hX, hY, 1, 6, true, &hpacked));
     };
-    bool any_bending = false;
-    for (int t = 0; t < nt; ++t) any_bending = any_bending || (s->types[t].bend && s->types[t].nc);
-    const bool bend_forked = concurrent && any_bending;
-    if (bend_forked) {
-        IBM_HIP(ctx, hipEventRecord(s->bend_fork_ev, ctx->stream));
-        IBM_HIP(ctx, hipStreamWaitEvent(s->aux[0]->stream, s->bend_fork_ev, 0));
-        IBM_TRY(bending_chain(s->aux[0]));
-    }
 
     // 1. geometry: positions, tangents, normals and surface measures of
     // every cell type (the positions in one launch for all types, the
@@ -407,12 +395,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         // derivatives at the sample sites (one launch), then per type the
         // tension + bending (+ tether) pass
         if (any_membrane) {
-            if (bend_forked) {
-                IBM_HIP(ctx, hipEventRecord(s->bend_ev, s->aux[0]->stream));
-                IBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->bend_ev, 0));
-            } else {
-                IBM_TRY(bending_chain(ctx));
-            }
+            IBM_TRY(bending_chain(ctx));
             for (int t = 0; t < nt; ++t) {
                 auto& T = s->types[t];
                 if (!T.nc || !T.membrane) continue;
@@ -702,9 +685,7 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
         s->aux[k]->stream = s->aux_stream[k];
     }
     if (st == IBM_OK && (hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess ||
-                         hipEventCreateWithFlags(&s->geo_ev, hipEventDisableTiming) != hipSuccess ||
-                         hipEventCreateWithFlags(&s->bend_fork_ev, hipEventDisableTiming) != hipSuccess ||
-                         hipEventCreateWithFlags(&s->bend_ev, hipEventDisableTiming) != hipSuccess))
+                         hipEventCreateWithFlags(&s->geo_ev, hipEventDisableTiming) != hipSuccess))
         st = fail(ctx, IBM_ERR_HIP, "solver: fork event");
     if (st == IBM_OK) {
         double* h = nullptr;
@@ -799,28 +780,6 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
             }
         }
         s->types.push_back(T);
-    }
-    // The bending chain of fixed-count steps packs its operands on the first
-    // auxiliary context: size that scratch now, so that a captured step
-    // allocates nothing.
-    if (st == IBM_OK && s->aux[0]) {
-        ibm_rbf* dops[MAX_TYPES];
-        ibm_rbf* hops[MAX_TYPES];
-        int dN[MAX_TYPES], hN[MAX_TYPES];
-        int nb = 0;
-        for (auto& T : s->types)
-            if (T.bend && T.nc) {
-                dops[nb] = T.dops;
-                hops[nb] = T.ops;
-                dN[nb] = 3 * T.nc;
-                hN[nb] = T.nc;
-                ++nb;
-            }
-        if (nb) {
-            st = rbf_reserve(s->aux[0], nb, dops, dN);
-            if (st == IBM_OK) st = rbf_reserve(s->aux[0], nb, hops, hN);
-            if (st != IBM_OK) st = step_status(ctx, s->aux[0], st);
-        }
     }
     if (st == IBM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(ctx, IBM_ERR_HIP, "solver setup");
     if (st != IBM_OK) {

@@ -392,7 +392,8 @@ ibm_status ibm_rbf_eval(ibm_ctx* ctx, int count, ibm_rbf* const* ops, const int*
 /* Kernel variant of the FP64 evaluation (no counterpart in the reference,
  * whose products go to cuBLAS): waves_n = 2 runs four 64 x 64-output waves
  * per 128 x 128 tile, 4 runs eight 64 x 32-output waves (half the
- * registers, twice the waves per SIMD), 0 (the default) times both on the
+ * registers, twice the waves per SIMD) over two LDS buffers (one barrier
+ * per contraction chunk), 0 (the default) times both on the
  * first eager launch of each shape and keeps the faster for the process.
  * Both variants give bitwise the same results.  Process-wide. */
 ibm_status ibm_rbf_gemm_variant(int waves_n);

@@ -126,8 +126,9 @@ def test_surface_geometry_matches_oracle_and_sphere(ibm, ctx, rbc_ops):
 @pytest.mark.parametrize("shape", [(8832, 2210, 600), (1552, 605, 1200), (300, 77, 5)],
                          ids=["rbc-c4", "platelet-c4", "small"])
 def test_f64_kernel_variants_agree_bitwise(ibm, ctx, shape):
-    """ibm_rbf_gemm_variant: the 4-wave (64 x 64) and 8-wave (64 x 32)
-    FP64 kernels and the timed choice give bitwise the same products, at the
+    """ibm_rbf_gemm_variant: the 4-wave (64 x 64) and the double-buffered
+    8-wave (64 x 32) FP64 kernels and the timed choice give bitwise the same
+    products, at the
     C4 shapes (200 RBCs, 400 platelets: N = 3 x cells) and a padded one."""
     import torch
     M, K, N = shape
