@@ -12,7 +12,8 @@
 //   = B_j,   mu = lambda_x(kx) + lambda_z(kz),
 // with the wall rows' ghost coefficients in d_0, d_{n-1} (fd/discretization.h:
 // 94-108).  The solve is: real-to-half-complex FFT along x, complex FFT
-// along z, one Thomas elimination per mode, and the inverse
+// along z, one tridiagonal solve per mode (parallel cyclic reduction in
+// LDS), and the inverse
 // transforms -- the exact solution of the same discrete system the multigrid
 // iterates on, in five memory passes.  The singular case (sigma = 0 with
 // Neumann walls: the pressure Poisson) solves the mean-free problem, as the
@@ -35,7 +36,6 @@ namespace {
 
 constexpr int FFT_THREADS = 256;
 constexpr int ZT = 8;              // kx columns per z-transform block
-constexpr int SWEEP = 32;          // rows per step of the tridiagonal sweeps (loads in flight per lane)
 constexpr int MAX_FFT_POINTS = 512;
 constexpr int MAX_WALL_POINTS = 800;
 constexpr size_t MAX_DYNAMIC_LDS = 160 * 1024;
@@ -175,113 +175,127 @@ __device__ inline void tri_row(const wall_axis& y, double sigma, double tau, dou
     diag = id * (sigma + tau * mu) + ty * d;
 }
 
-// The elimination of every mode for one (sigma, tau), computed once:
-// cp = the eliminated super-diagonal, inv = the reciprocal pivot (0 for the
-// pinned last unknown of the singular mode), laid out like the spectrum
-// without its complex pairs.  One lane per mode.
-__global__ void __launch_bounds__(FFT_THREADS) tridiagonal_factor_kernel(int nh, int nz, wall_axis y, double sigma,
-                                                                         double tau, int singular,
-                                                                         const double* __restrict__ lam_x,
-                                                                         const double* __restrict__ lam_z,
-                                                                         double* __restrict__ cp_out,
-                                                                         double* __restrict__ inv_out)
-{
-    const int kx = blockIdx.x * blockDim.x + threadIdx.x, kz = blockIdx.y;
-    if (kx >= nh) return;
-    const int ny = y.n;
-    const double mu = lam_x[kx] + lam_z[kz];
-    const bool pinned = singular && kx == 0 && kz == 0;
-    const size_t plane = (size_t) nh * ny;
-    double cprev = 0.0;
-    for (int j = 0; j < ny; ++j) {
-        double lo, hi, diag;
-        tri_row(y, sigma, tau, mu, j, lo, hi, diag);
-        double inv = 0.0, cp = 0.0;
-        if (!(pinned && j == ny - 1)) {
-            inv = 1.0 / (diag - lo * cprev);
-            cp = hi * inv;
-        }
-        const size_t at = (size_t) kz * plane + (size_t) j * nh + kx;
-        cp_out[at] = cp;
-        inv_out[at] = inv;
-        cprev = cp;
-    }
-}
+// Every Fourier mode's tridiagonal system along y by parallel cyclic
+// reduction in LDS: block (kx tile of zt modes, kz) holds zt x np2 rows
+// (np2 = ny rounded up to a power of two; rows past ny are identity rows),
+// 256 threads update PCR_ITEMS rows each per step (row j combined with rows
+// j -+ 2^s, log2(np2) steps), then x_j = d_j / b_j.  The singular mode's last
+// equation is replaced by x = 0 after its right-hand side loses its mean,
+// and its solution loses its mean.
+constexpr int PCR_THREADS = 256, PCR_ITEMS = 8, PCR_ROWS = PCR_THREADS * PCR_ITEMS;
 
-// One Thomas elimination along y per Fourier mode with the factors of
-// tridiagonal_factor_kernel: one lane per mode (consecutive lanes take
-// consecutive kx, so every row's accesses are coalesced), the spectrum
-// overwritten in place by d' = (r - lo d'_prev) / pivot and then by the
-// back substitution x = d' - c' x_next, SWEEP rows per step so that their
-// loads are issued before the dependent arithmetic.  The singular mode works
-// on its mean-free right-hand side and returns its mean-free solution.
-__global__ void __launch_bounds__(64) tridiagonal_kernel(int nh, int ny, double ty, int singular,
-                                                         const double* __restrict__ cp,
-                                                         const double* __restrict__ inv, double2* __restrict__ s)
+__global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np2, wall_axis y, double sigma,
+                                                          double tau, int singular, const double* __restrict__ lam_x,
+                                                          const double* __restrict__ lam_z, double2* __restrict__ s)
 {
-    const int kx = blockIdx.x * blockDim.x + threadIdx.x, kz = blockIdx.y;
-    if (kx >= nh) return;
-    const size_t base = (size_t) kz * nh * ny + kx;
-    auto at = [&](int j) { return base + (size_t) j * nh; };
-    const bool pinned = singular && kx == 0 && kz == 0;
-    double2 mean = make_double2(0.0, 0.0);
-    if (pinned) {
+    extern __shared__ double2 sh[];
+    double2* D = sh;
+    double* A = (double*) (sh + PCR_ROWS);
+    double* B = A + PCR_ROWS;
+    double* C = B + PCR_ROWS;
+    const int ny = y.n, kx0 = blockIdx.x * zt, kz = blockIdx.y;
+    const size_t plane = (size_t) nh * ny;
+    const bool pin_block = singular && kx0 == 0 && kz == 0;     // holds the singular mode (t = 0)
+    for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
+        const int t = e % zt, j = e / zt, kx = kx0 + t;
+        double a = 0.0, b = 1.0, c = 0.0;
+        double2 d = make_double2(0.0, 0.0);
+        if (kx < nh && j < ny) {
+            tri_row(y, sigma, tau, lam_x[kx] + lam_z[kz], j, a, c, b);
+            d = s[(size_t) kz * plane + (size_t) j * nh + kx];
+        }
+        A[e] = a;
+        B[e] = b;
+        C[e] = c;
+        D[e] = d;
+    }
+    __syncthreads();
+    if (pin_block && threadIdx.x == 0) {
+        double2 m = make_double2(0.0, 0.0);
         for (int j = 0; j < ny; ++j) {
-            mean.x += s[at(j)].x;
-            mean.y += s[at(j)].y;
+            m.x += D[j * zt].x;
+            m.y += D[j * zt].y;
         }
-        mean.x /= ny;
-        mean.y /= ny;
+        m.x /= ny;
+        m.y /= ny;
+        for (int j = 0; j < ny - 1; ++j) {
+            D[j * zt].x -= m.x;
+            D[j * zt].y -= m.y;
+        }
+        const int last = (ny - 1) * zt;
+        A[last] = 0.0;
+        B[last] = 1.0;
+        C[last] = 0.0;
+        D[last] = make_double2(0.0, 0.0);
     }
-    double2 dprev = make_double2(0.0, 0.0);
-    for (int j0 = 0; j0 < ny; j0 += SWEEP) {
-        double2 r[SWEEP];
-        double iv[SWEEP];
+    __syncthreads();
+    for (int sft = 1; sft < np2; sft <<= 1) {
+        double na[PCR_ITEMS], nb[PCR_ITEMS], nc[PCR_ITEMS];
+        double2 nd[PCR_ITEMS];
 #pragma unroll
-        for (int u = 0; u < SWEEP; ++u) {
-            const int j = j0 + u < ny ? j0 + u : ny - 1;
-            r[u] = s[at(j)];
-            iv[u] = inv[at(j)];
-        }
-#pragma unroll
-        for (int u = 0; u < SWEEP; ++u) {
-            const double lo = j0 + u > 0 ? ty : 0.0;
-            dprev = make_double2((r[u].x - mean.x - lo * dprev.x) * iv[u], (r[u].y - mean.y - lo * dprev.y) * iv[u]);
-            r[u] = dprev;
-        }
-#pragma unroll
-        for (int u = 0; u < SWEEP; ++u)
-            if (j0 + u < ny) s[at(j0 + u)] = r[u];
-    }
-    double2 xn = s[at(ny - 1)];
-    double2 xsum = xn;
-    for (int j1 = ny - 2; j1 >= 0; j1 -= SWEEP) {
-        double2 d[SWEEP];
-        double c[SWEEP];
-#pragma unroll
-        for (int u = 0; u < SWEEP; ++u) {
-            const int j = j1 - u >= 0 ? j1 - u : 0;
-            d[u] = s[at(j)];
-            c[u] = cp[at(j)];
-        }
-#pragma unroll
-        for (int u = 0; u < SWEEP; ++u) {
-            if (j1 - u >= 0) {
-                xn = make_double2(d[u].x - c[u] * xn.x, d[u].y - c[u] * xn.y);
-                d[u] = xn;
-                xsum.x += xn.x;
-                xsum.y += xn.y;
+        for (int i = 0; i < PCR_ITEMS; ++i) {
+            const int e = threadIdx.x + PCR_THREADS * i;
+            const int t = e % zt, j = e / zt;
+            double a = A[e], b = B[e], c = C[e];
+            double2 d = D[e];
+            if (j - sft >= 0) {
+                const int m = (j - sft) * zt + t;
+                const double al = -a / B[m];
+                b += al * C[m];
+                d.x += al * D[m].x;
+                d.y += al * D[m].y;
+                a = al * A[m];
+            } else {
+                a = 0.0;
             }
+            if (j + sft < np2) {
+                const int p = (j + sft) * zt + t;
+                const double ga = -c / B[p];
+                b += ga * A[p];
+                d.x += ga * D[p].x;
+                d.y += ga * D[p].y;
+                c = ga * C[p];
+            } else {
+                c = 0.0;
+            }
+            na[i] = a;
+            nb[i] = b;
+            nc[i] = c;
+            nd[i] = d;
         }
+        __syncthreads();
 #pragma unroll
-        for (int u = 0; u < SWEEP; ++u)
-            if (j1 - u >= 0) s[at(j1 - u)] = d[u];
-    }
-    if (pinned) {
-        for (int j = 0; j < ny; ++j) {
-            s[at(j)].x -= xsum.x / ny;
-            s[at(j)].y -= xsum.y / ny;
+        for (int i = 0; i < PCR_ITEMS; ++i) {
+            const int e = threadIdx.x + PCR_THREADS * i;
+            A[e] = na[i];
+            B[e] = nb[i];
+            C[e] = nc[i];
+            D[e] = nd[i];
         }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
+        const double inv = 1.0 / B[e];
+        D[e] = make_double2(D[e].x * inv, D[e].y * inv);
+    }
+    __syncthreads();
+    if (pin_block && threadIdx.x == 0) {
+        double2 m = make_double2(0.0, 0.0);
+        for (int j = 0; j < ny; ++j) {
+            m.x += D[j * zt].x;
+            m.y += D[j * zt].y;
+        }
+        m.x /= ny;
+        m.y /= ny;
+        for (int j = 0; j < ny; ++j) {
+            D[j * zt].x -= m.x;
+            D[j * zt].y -= m.y;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
+        const int t = e % zt, j = e / zt, kx = kx0 + t;
+        if (kx < nh && j < ny) s[(size_t) kz * plane + (size_t) j * nh + kx] = D[e];
     }
 }
 
@@ -327,23 +341,10 @@ struct ibm_fft {
     double2* twz = nullptr;
     double* lam_x = nullptr;
     double* lam_z = nullptr;
-    // Elimination factors per (sigma, tau) met outside graph capture (at
-    // most FACTOR_SETS; later pairs, and pairs first met under capture, are
-    // eliminated on the fly).  A set is never overwritten, so a captured
-    // solve keeps reading valid factors.
-    struct factor_set {
-        double sigma, tau;
-        double *cp, *inv;
-    };
-    std::vector<factor_set> factors;
-    double *scratch_cp = nullptr, *scratch_inv = nullptr;   // factors recomputed per solve when not cached
-    std::mutex lock;
     std::vector<void*> owned;
 };
 
 namespace ibm {
-
-constexpr size_t FACTOR_SETS = 8;
 
 // Whether the FFT solver takes grid g (and why not).
 static const char* fft_unsupported(const ibm_grid& g)
@@ -374,43 +375,9 @@ static ibm_status fft_lds_attributes(ibm_ctx* ctx)
     if (done) return IBM_OK;
     IBM_HIP(ctx, hipFuncSetAttribute((const void*) zfft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int) MAX_DYNAMIC_LDS));
+    IBM_HIP(ctx, hipFuncSetAttribute((const void*) pcr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int) MAX_DYNAMIC_LDS));
     done = true;
-    return IBM_OK;
-}
-
-// The elimination factors of (sigma, tau) into *cp, *inv: a cached set; or
-// a new set computed now on ctx's stream; or, under graph capture (no
-// allocation) or with every set taken, the plan's scratch set recomputed by
-// this solve.
-static ibm_status fft_factors(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, int singular, const double** cp,
-                              const double** inv)
-{
-    std::lock_guard<std::mutex> guard(p->lock);
-    for (const auto& f : p->factors)
-        if (f.sigma == sigma && f.tau == tau) {
-            *cp = f.cp;
-            *inv = f.inv;
-            return IBM_OK;
-        }
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    IBM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
-    double *c = p->scratch_cp, *v = p->scratch_inv;
-    const bool keep = cs == hipStreamCaptureStatusNone && p->factors.size() < FACTOR_SETS;
-    if (keep) {
-        const size_t n = (size_t) p->nh * p->ny * p->nz;
-        void *m0 = nullptr, *m1 = nullptr;
-        IBM_TRY(fft_alloc(ctx, p, sizeof(double) * n, &m0));
-        IBM_TRY(fft_alloc(ctx, p, sizeof(double) * n, &m1));
-        c = (double*) m0;
-        v = (double*) m1;
-    }
-    const dim3 grid((p->nh + FFT_THREADS - 1) / FFT_THREADS, p->nz);
-    tridiagonal_factor_kernel<<<grid, FFT_THREADS, 0, ctx->stream>>>(p->nh, p->nz, p->y, sigma, tau, singular,
-                                                                     p->lam_x, p->lam_z, c, v);
-    IBM_LAUNCH_CHECK(ctx);
-    if (keep) p->factors.push_back(ibm_fft::factor_set{sigma, tau, c, v});
-    *cp = c;
-    *inv = v;
     return IBM_OK;
 }
 
@@ -434,10 +401,13 @@ ibm_status fft_solve(ibm_ctx* ctx, ibm_fft* p, double sigma, double tau, const d
     zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 0);
     IBM_LAUNCH_CHECK(ctx);
     const int singular = sigma == 0.0 && p->neumann;
-    const double *cp = nullptr, *inv = nullptr;
-    IBM_TRY(fft_factors(ctx, p, sigma, tau, singular, &cp, &inv));
-    const dim3 tgrid((p->nh + 63) / 64, p->nz);
-    tridiagonal_kernel<<<tgrid, 64, 0, ctx->stream>>>(p->nh, p->ny, tau * p->y.r2, singular, cp, inv, p->spec);
+    int np2 = 1;
+    while (np2 < p->ny) np2 <<= 1;
+    const int zt = PCR_ROWS / np2;
+    const dim3 tgrid((p->nh + zt - 1) / zt, p->nz);
+    const size_t tlds = (sizeof(double2) + 3 * sizeof(double)) * PCR_ROWS;
+    pcr_kernel<<<tgrid, PCR_THREADS, tlds, ctx->stream>>>(p->nh, zt, np2, p->y, sigma, tau, singular, p->lam_x, p->lam_z,
+                                                          p->spec);
     IBM_LAUNCH_CHECK(ctx);
     zfft_kernel<<<zgrid, FFT_THREADS, zlds, ctx->stream>>>(p->nh, p->ny, p->nz, p->lz, p->spec, p->twz, 1);
     IBM_LAUNCH_CHECK(ctx);
@@ -493,14 +463,6 @@ ibm_status ibm_fft_create(ibm_ctx* ctx, const ibm_grid* g, ibm_fft** out)
     }
     if (st == IBM_OK) {
         p->lam_z = (double*) m;
-        st = fft_alloc(ctx, p, sizeof(double) * p->nh * p->ny * (size_t) p->nz, &m);
-    }
-    if (st == IBM_OK) {
-        p->scratch_cp = (double*) m;
-        st = fft_alloc(ctx, p, sizeof(double) * p->nh * p->ny * (size_t) p->nz, &m);
-    }
-    if (st == IBM_OK) {
-        p->scratch_inv = (double*) m;
         if (hipMemcpy(p->twx, twx.data(), sizeof(double2) * twx.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->twz, twz.data(), sizeof(double2) * twz.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->lam_x, lx.data(), sizeof(double) * lx.size(), hipMemcpyHostToDevice) != hipSuccess ||

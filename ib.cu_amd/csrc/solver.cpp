@@ -425,7 +425,9 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         }
         IBM_TRY(spread_sets_step(ctx, s->g, P.delta, s->tile_spread, nt, counts, xs, fs, s->F));
     }
-    // 4. advection
+    // 4. advection (on an auxiliary stream beside the geometry it was starved
+    // of compute units by the matrix-core blocks: 0.47 ms per component
+    // instead of 0.075, no gain)
     {
         const double* u[3] = {st->u[0], st->u[1], st->u[2]};
         IBM_TRY(ibm_advection(ctx, s->g, u, s->H));
