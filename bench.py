@@ -244,14 +244,16 @@ def cpu_baseline(orc, fd, rbf, forces, step_mod, L, types, gpu_iters, repeats=5)
                        f"cycles, the GPU's counts; scipy, one thread); total {total:.1f} s per step")}
 
 
-# Kernel-name prefixes of the counter files for each timed kernel group.
+# Kernel-name prefixes of the counter files for each timed kernel group (a
+# tuple: the kernels of one group launch, counted by the first).
 PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "cheb_march_kernel<double",
                "stencil_f32": "stencil_kernel<float", "stencil_f64": "stencil_kernel<double",
                "restrict": "::restrict_kernel<", "residual_restrict": "residual_restrict_kernel",
                "prolong": "::prolong_add_kernel", "prolong_residual": "prolong_residual_kernel",
                "rbf_gemm": "rbf_gemm_kernel",
                "spread_brick": "spread_brick_kernel", "spread_tile": "tile_spread_kernel",
-               "spread_merge": "tile_merge_kernel", "interp": "interp_kernel", "advection": "advection_kernel"}
+               "spread_merge": "tile_merge_kernel", "interp": "interp_kernel", "advection": "advection_kernel",
+               "fft_solve": ("xfft_forward_kernel", "zfft_kernel", "pcr_kernel", "xfft_inverse_kernel")}
 COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
 
 
@@ -285,8 +287,16 @@ def pmc_summary(d):
 
     def load(kind, counter):
         (path,) = glob.glob(os.path.join(d, f"{kind}_step_counter_collection.csv"))
-        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(open(path))
-                if r["Counter_Name"] == counter]
+        rows = [(int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"]))
+                for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        rows.sort()
+        return [(n, v) for _, n, v in rows]
+
+    def last_step(rows):
+        # every step ends with one history_kernel launch: keep the dispatches
+        # of the last (fixed-count) step
+        ends = [i for i, (n, _) in enumerate(rows) if "history_kernel" in n]
+        return rows[ends[-2] + 1:ends[-1] + 1] if len(ends) >= 2 else rows
     fetch, write = load("fetch", "FETCH_SIZE"), load("write", "WRITE_SIZE")
     copy_f = max(v for n, v in fetch if "copyBuffer" in n)
     copy_w = max(v for n, v in write if "copyBuffer" in n)
@@ -294,13 +304,17 @@ def pmc_summary(d):
     f_write = COPY_BYTES / 1024 / copy_w
     print(f"calibration: copy dispatch fetch {copy_f:.1f} KiB, write {copy_w:.1f} KiB of "
           f"{COPY_BYTES // 1024} KiB moved each way -> f_fetch {f_fetch:.4f}, f_write {f_write:.4f}")
-    for name, prefix in PMC_KERNELS.items():
-        fs = [v for n, v in fetch if prefix in n]
-        ws = [v for n, v in write if prefix in n]
-        if not fs or not ws:
+    fetch, write = last_step(fetch), last_step(write)
+    for name, prefixes in PMC_KERNELS.items():
+        prefixes = prefixes if isinstance(prefixes, tuple) else (prefixes,)
+        # a group's launches are counted by its first kernel's dispatches
+        n = len([1 for k, _ in fetch if prefixes[0] in k])
+        fs = [v for k, v in fetch if any(p in k for p in prefixes)]
+        ws = [v for k, v in write if any(p in k for p in prefixes)]
+        if not n or not fs or not ws:
             continue
-        per = 1024 * (sum(fs) / len(fs) * f_fetch + sum(ws) / len(ws) * f_write)
-        print(f"{name}: launches {len(fs)} fetch {sum(fs) / len(fs):.1f} KiB write {sum(ws) / len(ws):.1f} KiB "
+        per = 1024 * (sum(fs) / n * f_fetch + sum(ws) / n * f_write)
+        print(f"{name}: launches {n} fetch {sum(fs) / n:.1f} KiB write {sum(ws) / n:.1f} KiB "
               f"-> traffic_bytes_per_launch: {per:.0f}")
 
 

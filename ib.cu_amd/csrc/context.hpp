@@ -85,8 +85,10 @@ ibm_status mean_curvature(ibm_ctx* ctx, hipStream_t stream, int nd, int nc, cons
 ibm_status membrane_forces(ibm_ctx* ctx, hipStream_t stream, const ibm_membrane* m, double tether, int ns, int nc,
                            const double* geo, const double* normal, const double* sigma, const double* hd,
                            const double* table, const double* y, const double* yref, double beta, double* f);
-// Number of operators an RBF object holds.
+// Number of operators an RBF object holds, and the most (entry, operator)
+// blocks one rbf_eval_range launch takes.
 int rbf_nops(const ibm_rbf* op);
+constexpr int RBF_MAX_GROUPS = 24;
 
 // Fixed-count multigrid solves of several independent systems, solve k on
 // ctxs[k] (ibm_mg_solve_fixed each), issued cycle by cycle in turn; with

@@ -45,7 +45,7 @@ constexpr int TILE = 128;        // rows of a block tile (samples and columns)
 constexpr int ROWB = 128;        // bytes of one LDS row per contraction chunk
 constexpr int NTHREADS = 256;
 constexpr int KPAD = 64;         // contraction padded to a multiple of 64 elements
-constexpr int MAX_GROUPS = 24;
+constexpr int MAX_GROUPS = RBF_MAX_GROUPS;
 
 enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 

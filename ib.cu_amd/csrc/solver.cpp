@@ -299,6 +299,7 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     // sample sites (one launch).  It runs after the sample-site geometry on
     // the same stream: beside it on a second stream both matrix-core
     // launches slowed down more than they overlapped (5.2 against 4.5 ms).
+    bool data_done = false;     // bending's data-site geometry computed in the geometry launch
     auto bending_chain = [&](ibm_ctx* bc) -> ibm_status {
         ibm_rbf* dops[MAX_TYPES];
         ibm_rbf* hops[MAX_TYPES];
@@ -323,7 +324,8 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         }
         if (!nb) return IBM_OK;
         unsigned char* dpacked = nullptr;
-        IBM_TRY(step_status(ctx, bc, rbf_eval_range(bc, bc->stream, nb, dops, dN, dX, dY, 0, 5, true, &dpacked)));
+        if (!data_done)
+            IBM_TRY(step_status(ctx, bc, rbf_eval_range(bc, bc->stream, nb, dops, dN, dX, dY, 0, 5, true, &dpacked)));
         for (int t = 0; t < nt; ++t) {
             auto& T = s->types[t];
             if (!T.nc || !T.bend) continue;
@@ -338,11 +340,11 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     // every cell type (the positions in one launch for all types, the
     // tangents in a second one)
     {
-        ibm_rbf* ops[MAX_TYPES];
-        int N[MAX_TYPES];
-        const double* X[MAX_TYPES];
-        double* Y[MAX_TYPES];
-        int count = 0;
+        ibm_rbf* ops[2 * MAX_TYPES];
+        int N[2 * MAX_TYPES];
+        const double* X[2 * MAX_TYPES];
+        double* Y[2 * MAX_TYPES];
+        int count = 0, groups = 0;
         for (int t = 0; t < nt; ++t) {
             auto& T = s->types[t];
             if (!T.nc) continue;
@@ -350,10 +352,29 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
             N[count] = 3 * T.nc;
             X[count] = st->x[t];
             Y[count] = T.Y;
+            groups += rbf_nops(T.ops);
             ++count;
         }
         bool any_membrane = false;
         for (int t = 0; t < nt; ++t) any_membrane = any_membrane || (s->types[t].membrane && s->types[t].nc);
+        // Bending's data-site geometry depends on the data sites alone: it
+        // joins the same launch (one grouped launch balances better than
+        // two) when the groups fit one launch.
+        int dgroups = 0;
+        for (int t = 0; t < nt; ++t)
+            if (s->types[t].bend && s->types[t].nc) dgroups += rbf_nops(s->types[t].dops);
+        if (dgroups && groups + dgroups <= RBF_MAX_GROUPS) {
+            for (int t = 0; t < nt; ++t) {
+                auto& T = s->types[t];
+                if (!T.nc || !T.bend) continue;
+                ops[count] = T.dops;
+                N[count] = 3 * T.nc;
+                X[count] = st->x[t];
+                Y[count] = T.Yd;
+                ++count;
+            }
+            data_done = true;
+        }
         // Positions (operator 0) first: tether forces and the spreading need
         // only them.  Membrane forces need every operator, so with a
         // membrane model the whole geometry stays on the caller's stream.
