@@ -45,10 +45,16 @@ __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x *
 __device__ inline int bitrev(int k, int logn) { return (int) (__brev((unsigned) k) >> (32 - logn)); }
 
 // In-place radix-2 transforms of `count` arrays of n points held in LDS in
-// bit-reversed order (a[t * n + bitrev(k)] = input k of array t).  tw[m] =
-// exp(-2 pi i m / n), m < n / 2, also in LDS (a stage's twiddle reads would
-// otherwise be dependent global loads); `inverse` conjugates the twiddles
-// (no scaling).  Every thread of the block takes part in every stage.
+// bit-reversed order: point k of array t at a[idx(t, k)] holds input
+// bitrev(k).  Rows layout (idx = t n + k, the x transforms) maps consecutive
+// lanes to consecutive butterflies of one array; columns layout (idx =
+// k count + t, the z transforms, whose rows of count arrays are contiguous)
+// maps consecutive lanes to consecutive arrays, so that the lanes of a wave
+// touch consecutive LDS words either way.  tw[m] = exp(-2 pi i m / n),
+// m < n / 2, also in LDS (a stage's twiddle reads would otherwise be
+// dependent global loads); `inverse` conjugates the twiddles (no scaling).
+// Every thread of the block takes part in every stage.
+template <bool COLS>
 __device__ void fft_stages(double2* a, int count, int n, int logn, const double2* tw, bool inverse)
 {
     const int half_n = n >> 1;
@@ -57,9 +63,10 @@ __device__ void fft_stages(double2* a, int count, int n, int logn, const double2
         const int half = 1 << (s - 1);
         const int stride = n >> s;                 // twiddle index step
         for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int t = b / half_n, bb = b - t * half_n;
+            const int t = COLS ? b % count : b / half_n, bb = COLS ? b / count : b - t * half_n;
             const int grp = bb >> (s - 1), pos = bb & (half - 1);
-            const int i = t * n + grp * 2 * half + pos, j = i + half;
+            const int k0 = grp * 2 * half + pos, k1 = k0 + half;
+            const int i = COLS ? k0 * count + t : t * n + k0, j = COLS ? k1 * count + t : t * n + k1;
             double2 w = tw[pos * stride];
             if (inverse) w.y = -w.y;
             const double2 u = a[i], v = cmul(a[j], w);
@@ -71,7 +78,9 @@ __device__ void fft_stages(double2* a, int count, int n, int logn, const double2
 }
 
 // Forward transform along x of `rows` rows of nx reals into nh = nx / 2 + 1
-// coefficients each (row r of x -> row r of s, x fastest in both).
+// coefficients each (row r of x -> row r of s, x fastest in both).  The
+// bit reversal is applied to the (cached) global reads, so the LDS writes
+// are consecutive.
 __global__ void __launch_bounds__(FFT_THREADS) xfft_forward_kernel(int nx, int logn, int rows,
                                                                    const double* __restrict__ x,
                                                                    double2* __restrict__ s,
@@ -84,10 +93,10 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_forward_kernel(int nx, int l
     for (int e = threadIdx.x; e < nx / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
         const int t = e / nx, k = e - t * nx, row = r0 + t;
-        sh[t * nx + bitrev(k, logn)] = make_double2(row < rows ? x[(size_t) row * nx + k] : 0.0, 0.0);
+        sh[e] = make_double2(row < rows ? x[(size_t) row * nx + bitrev(k, logn)] : 0.0, 0.0);
     }
     __syncthreads();
-    fft_stages(sh, rpb, nx, logn, stw, false);
+    fft_stages<false>(sh, rpb, nx, logn, stw, false);
     for (int e = threadIdx.x; e < rpb * nh; e += blockDim.x) {
         const int t = e / nh, k = e - t * nh, row = r0 + t;
         if (row < rows) s[(size_t) row * nh + k] = sh[t * nx + k];
@@ -107,7 +116,7 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
     const int r0 = blockIdx.x * rpb;
     for (int e = threadIdx.x; e < nx / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
-        const int t = e / nx, k = e - t * nx, row = r0 + t;
+        const int t = e / nx, k = bitrev(e - t * nx, logn), row = r0 + t;
         double2 v = make_double2(0.0, 0.0);
         if (row < rows) {
             if (k < nh) {
@@ -117,10 +126,10 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
                 v.y = -v.y;
             }
         }
-        sh[t * nx + bitrev(k, logn)] = v;
+        sh[e] = v;
     }
     __syncthreads();
-    fft_stages(sh, rpb, nx, logn, stw, true);
+    fft_stages<false>(sh, rpb, nx, logn, stw, true);
     for (int e = threadIdx.x; e < rpb * nx; e += blockDim.x) {
         const int t = e / nx, k = e - t * nx, row = r0 + t;
         if (row < rows) x[(size_t) row * nx + k] = sh[t * nx + k].x * scale;
@@ -128,7 +137,8 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
 }
 
 // Transform along z, in place, of s laid out [k][j][kx] (kx fastest, nh of
-// them): block (kx tile, j) transforms ZT columns.
+// them): block (kx tile, j) transforms ZT columns, held in LDS as rows of
+// ZT consecutive kx.
 __global__ void __launch_bounds__(FFT_THREADS) zfft_kernel(int nh, int ny, int nz, int logn, double2* __restrict__ s,
                                                            const double2* __restrict__ tw, int inverse)
 {
@@ -138,15 +148,14 @@ __global__ void __launch_bounds__(FFT_THREADS) zfft_kernel(int nh, int ny, int n
     const size_t plane = (size_t) nh * ny;
     for (int e = threadIdx.x; e < nz / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
-        const int t = e % ZT, k = e / ZT, kx = kx0 + t;
-        sh[t * nz + bitrev(k, logn)] =
-            kx < nh ? s[(size_t) k * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
+        const int t = e % ZT, k = e / ZT, kx = kx0 + t, kr = bitrev(k, logn);
+        sh[e] = kx < nh ? s[(size_t) kr * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_stages(sh, ZT, nz, logn, stw, inverse != 0);
+    fft_stages<true>(sh, ZT, nz, logn, stw, inverse != 0);
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
         const int t = e % ZT, k = e / ZT, kx = kx0 + t;
-        if (kx < nh) s[(size_t) k * plane + (size_t) j * nh + kx] = sh[t * nz + k];
+        if (kx < nh) s[(size_t) k * plane + (size_t) j * nh + kx] = sh[e];
     }
 }
 
@@ -182,7 +191,7 @@ __device__ inline void tri_row(const wall_axis& y, double sigma, double tau, dou
 // j -+ 2^s, log2(np2) steps), then x_j = d_j / b_j.  The singular mode's last
 // equation is replaced by x = 0 after its right-hand side loses its mean,
 // and its solution loses its mean.
-constexpr int PCR_THREADS = 256, PCR_ITEMS = 8, PCR_ROWS = PCR_THREADS * PCR_ITEMS;
+constexpr int PCR_THREADS = 256, PCR_ITEMS = 4, PCR_ROWS = PCR_THREADS * PCR_ITEMS;
 
 __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np2, wall_axis y, double sigma,
                                                           double tau, int singular, const double* __restrict__ lam_x,
