@@ -99,7 +99,8 @@ def make_solver(ibm, ctx, L, types, precision, tol=TOL, multigrid=False):
     return s, st, ub, cell_types
 
 
-KERNELS = ("fft_solve", "spread_brick", "spread_tile", "spread_merge", "bin", "interp", "rbf_gemm", "geometry", "membrane", "stencil_f64", "stencil_f32", "restrict",
+KERNELS = ("fft_solve", "spread_brick", "spread_tile", "spread_merge", "bin", "interp", "rbf_gemm", "geometry", "membrane", "momentum_rhs",
+           "residual_update", "divergence", "stencil_f64", "stencil_f32", "restrict",
            "prolong", "twopoint", "coarse_vcycle", "advection", "chebyshev_f32", "chebyshev_f64", "residual_restrict",
            "prolong_residual")
 

@@ -101,6 +101,23 @@ ibm_status launch_max_abs(ibm_ctx* ctx, size_t n, const double* x, double* parti
 // *out = sqrt(*num / *den), 0 when *den is 0
 ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den, double* out);
 
+// rhs = dt I~ (F inv_rho - H) + dt nu (b + L u) in one pass: I~ and L are the
+// sigma and tau parts of A (the identity with its wall-row weights and the
+// Laplacian); b may be null.
+ibm_status launch_momentum_rhs(ibm_ctx* ctx, const op3<double>& A, double dt, double dtnu, double inv_rho,
+                               const double* F, const double* H, const double* u, const double* b, double* rhs);
+// *out = ||b - A x|| / ||b|| (sqrt_ratio's rules), and u += x when u is not
+// null, in one pass over the grid; `partial` holds `parts` doubles, at least
+// residual_parts(n) of A's extents.
+ibm_status launch_residual_update(ibm_ctx* ctx, const op3<double>& A, const double* x, const double* b, double* u,
+                                  double* partial, size_t parts, double* out);
+size_t residual_parts(int nx, int ny, int nz);
+// out = scale (D_0 u_0 + D_1 u_1 + D_2 u_2) for the differentials P (one
+// destination grid); with out null, *max_out = max |D_0 u_0 + D_1 u_1 + D_2 u_2|
+// instead (`partial`: REDUCE_PARTS doubles).
+ibm_status launch_divergence(ibm_ctx* ctx, const twopoint_op<double> P[3], double scale, const double* const* u,
+                             double* out, double* partial, double* max_out);
+
 // fd::lower_boundary / fd::upper_boundary applied to face data
 // (fd/boundary_ops.h:21-71): the ghost-cell boundary coefficient of the face
 // row along `view`, scaled by resolution^order, times the (corrected)
@@ -127,6 +144,10 @@ ins_boundary_op make_ins_boundary(const ibm_grid& g, int dim, int order);
 
 // The differential (kind 0) or average (kind 1) of grid `src` along `view`.
 ibm_status make_twopoint(const ibm_grid& src, int view, int kind, twopoint_op<double>* op, ibm_grid* dst);
+// ibm_divergence's sum times scale into out, or (out null) its maximum
+// modulus into *max_out; one launch.
+ibm_status divergence_of(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double scale, double* out,
+                         double* partial, double* max_out);
 
 // Block targets of the z-marching kernels (smoothing, residual + restriction,
 // prolongation + residual) per grid: the number of blocks the z chunking aims

@@ -9,7 +9,6 @@
 #include <hip/hip_runtime.h>
 #include <map>
 #include <mutex>
-#include <mutex>
 #include <algorithm>
 #include <utility>
 #include <cmath>
@@ -153,6 +152,169 @@ stencil_kernel(op3<T> A, T alpha, const T* __restrict__ x, T beta, const T* __re
             out[p] = v;
             if (OUT == 1) out32[p] = (float) v;
         }
+    }
+}
+
+// The momentum right-hand side of one velocity component in one pass (the
+// loads of stencil_kernel, with u as x):
+//   rhs = dt I~ (F inv_rho - H) + dt nu (b + L u),
+// I~ the identity weighted on the wall rows (sigma = 1, tau = 0 of A) and L
+// the Laplacian (sigma = 0, tau = 1 of A); b is optional.
+template <bool HB>
+__global__ void __launch_bounds__(STX * STY)
+momentum_rhs_kernel(op3<double> A, double dt, double dtnu, double inv_rho, const double* __restrict__ F,
+                    const double* __restrict__ H, const double* __restrict__ u, const double* __restrict__ b,
+                    double* __restrict__ rhs)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int i = blockIdx.x * STX + threadIdx.x;
+    const int j = blockIdx.y * STY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    if (i >= nx || j >= ny || k0 >= nz) return;
+    const size_t sxy = (size_t) nx * ny;
+    const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
+    const int ip = A.a[0].periodic ? (i == nx - 1 ? 0 : i + 1) : (i < nx - 1 ? i + 1 : i);
+    const int jm = A.a[1].periodic ? (j == 0 ? ny - 1 : j - 1) : (j > 0 ? j - 1 : j);
+    const int jp = A.a[1].periodic ? (j == ny - 1 ? 0 : j + 1) : (j < ny - 1 ? j + 1 : j);
+    const size_t col = (size_t) i + (size_t) nx * j;
+    const ptrdiff_t dxm = (ptrdiff_t) im - i, dxp = (ptrdiff_t) ip - i;
+    const ptrdiff_t dym = ((ptrdiff_t) jm - j) * nx, dyp = ((ptrdiff_t) jp - j) * nx;
+
+    double c[KZ + 2], xl[KZ], xr[KZ], yl[KZ], yr[KZ], fv[KZ], hv[KZ], bv[KZ];
+    {
+        int km = k0 - 1;
+        if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
+        c[0] = u[col + sxy * (size_t) km];
+    }
+#pragma unroll
+    for (int q = 0; q <= KZ; ++q) {
+        int k = k0 + q;
+        if (k >= nz) k = A.a[2].periodic ? k % nz : nz - 1;
+        c[q + 1] = u[col + sxy * (size_t) k];
+    }
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) {
+        const int k = min(k0 + q, nz - 1);
+        const size_t p = col + sxy * (size_t) k;
+        xl[q] = u[p + dxm];
+        xr[q] = u[p + dxp];
+        yl[q] = u[p + dym];
+        yr[q] = u[p + dyp];
+        fv[q] = F[p];
+        hv[q] = H[p];
+        if (HB) bv[q] = b[p];
+    }
+    const double Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) {
+        const int k = k0 + q;
+        if (k >= nz) break;
+        const double x0 = c[q + 1];
+        const double lx = lap1(A.a[0], i, xl[q], x0, xr[q]);
+        const double ly = lap1(A.a[1], j, yl[q], x0, yr[q]);
+        const double lz = lap1(A.a[2], k, c[q], x0, c[q + 2]);
+        const double Iz = id1(A.a[2], k);
+        const double h = fv[q] * inv_rho - hv[q];
+        double v = dt * ((Ix * Iy * Iz) * h);
+        if (HB) v += dtnu * bv[q];
+        rhs[col + sxy * (size_t) k] = dtnu * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz) + v;
+    }
+}
+
+// Sums of two values over a (STX x STY)-thread block, each in a fixed order
+// (wavefront shuffles, then the wavefronts in index order); thread 0 gets them.
+__device__ inline void block_sum2(double& a, double& b)
+{
+    __shared__ double part[2][STY];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int t = threadIdx.x + STX * threadIdx.y;
+    if ((t & 63) == 0) {
+        part[0][t >> 6] = a;
+        part[1][t >> 6] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+        a = 0;
+        b = 0;
+        for (int w = 0; w < STX * STY / 64; ++w) {
+            a += part[0][w];
+            b += part[1][w];
+        }
+    }
+}
+
+// r = b - A x at every point (stencil_kernel's expression of A x); the sums
+// of r^2 and b^2 over each block's points go to partial[block] and
+// partial[nblocks + block], and with UPD u += x (a direct solve's update and
+// its FP64 residual check in one pass; r itself is not stored).
+template <bool UPD>
+__global__ void __launch_bounds__(STX * STY)
+residual_update_kernel(op3<double> A, const double* __restrict__ x, const double* __restrict__ b,
+                       double* __restrict__ u, double* __restrict__ partial)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    const int i = blockIdx.x * STX + threadIdx.x;
+    const int j = blockIdx.y * STY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    double rr = 0, bb = 0;
+    if (i < nx && j < ny && k0 < nz) {
+        const size_t sxy = (size_t) nx * ny;
+        const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
+        const int ip = A.a[0].periodic ? (i == nx - 1 ? 0 : i + 1) : (i < nx - 1 ? i + 1 : i);
+        const int jm = A.a[1].periodic ? (j == 0 ? ny - 1 : j - 1) : (j > 0 ? j - 1 : j);
+        const int jp = A.a[1].periodic ? (j == ny - 1 ? 0 : j + 1) : (j < ny - 1 ? j + 1 : j);
+        const size_t col = (size_t) i + (size_t) nx * j;
+        const ptrdiff_t dxm = (ptrdiff_t) im - i, dxp = (ptrdiff_t) ip - i;
+        const ptrdiff_t dym = ((ptrdiff_t) jm - j) * nx, dyp = ((ptrdiff_t) jp - j) * nx;
+        double c[KZ + 2], xl[KZ], xr[KZ], yl[KZ], yr[KZ], bv[KZ], uv[KZ];
+        {
+            int km = k0 - 1;
+            if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
+            c[0] = x[col + sxy * (size_t) km];
+        }
+#pragma unroll
+        for (int q = 0; q <= KZ; ++q) {
+            int k = k0 + q;
+            if (k >= nz) k = A.a[2].periodic ? k % nz : nz - 1;
+            c[q + 1] = x[col + sxy * (size_t) k];
+        }
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            const int k = min(k0 + q, nz - 1);
+            const size_t p = col + sxy * (size_t) k;
+            xl[q] = x[p + dxm];
+            xr[q] = x[p + dxp];
+            yl[q] = x[p + dym];
+            yr[q] = x[p + dyp];
+            bv[q] = b[p];
+            if (UPD) uv[q] = u[p];
+        }
+        const double Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            const int k = k0 + q;
+            if (k >= nz) break;
+            const double x0 = c[q + 1];
+            const double lx = lap1(A.a[0], i, xl[q], x0, xr[q]);
+            const double ly = lap1(A.a[1], j, yl[q], x0, yr[q]);
+            const double lz = lap1(A.a[2], k, c[q], x0, c[q + 2]);
+            const double Iz = id1(A.a[2], k);
+            const double ax = A.sigma * (Ix * Iy * Iz) * x0 + A.tau * (Iy * Iz * lx + Ix * Iz * ly + Ix * Iy * lz);
+            const double r = bv[q] - ax;
+            rr += r * r;
+            bb += bv[q] * bv[q];
+            if (UPD) u[col + sxy * (size_t) k] = x0 + uv[q];
+        }
+    }
+    block_sum2(rr, bb);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y * gridDim.z;
+        const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        partial[blk] = rr;
+        partial[nb + blk] = bb;
     }
 }
 
@@ -1280,19 +1442,10 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_vcycle_kernel(coarse_le
 // Two-point operators (differential / average along one axis)
 // out = alpha * op x + beta * out
 // ---------------------------------------------------------------------------
+// Value of the operator at destination point c.
 template <typename T>
-__global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__ x, T beta, T* __restrict__ out)
+__device__ inline T twopoint_value(const twopoint_op<T>& P, const int c[3], const T* __restrict__ x)
 {
-    const size_t total = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
-    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    int c[3];
-    {
-        const unsigned u = (unsigned) t, q = u / (unsigned) P.dn[0];
-        c[0] = (int) (u - q * (unsigned) P.dn[0]);
-        c[1] = (int) (q % (unsigned) P.dn[1]);
-        c[2] = (int) (q / (unsigned) P.dn[1]);
-    }
     const int v = P.view;
     const int cols = P.sn[v];
     size_t stride = 1;
@@ -1306,7 +1459,23 @@ __global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__
         return idx < 0 ? P.int_lo * x[b0] : P.int_hi * x[b0 + stride * (cols - 1)];
     };
     const int r = c[v];
-    T val = P.c0 * fetch(r - P.s1) + P.c1 * fetch(r + 1 - P.s1);
+    return P.c0 * fetch(r - P.s1) + P.c1 * fetch(r + 1 - P.s1);
+}
+
+template <typename T>
+__global__ void twopoint_kernel(twopoint_op<T> P, T alpha, const T* __restrict__ x, T beta, T* __restrict__ out)
+{
+    const size_t total = (size_t) P.dn[0] * P.dn[1] * P.dn[2];
+    size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int c[3];
+    {
+        const unsigned u = (unsigned) t, q = u / (unsigned) P.dn[0];
+        c[0] = (int) (u - q * (unsigned) P.dn[0]);
+        c[1] = (int) (q % (unsigned) P.dn[1]);
+        c[2] = (int) (q / (unsigned) P.dn[1]);
+    }
+    T val = twopoint_value(P, c, x);
     T o = alpha * val;
     if (beta != T(0)) o += beta * out[t];
     out[t] = o;
@@ -1511,6 +1680,65 @@ __global__ void sqrt_ratio_kernel(const double* __restrict__ num, const double* 
     const double d = *den, n = *num;
     if (!isfinite(d) || !isfinite(n)) *out = __longlong_as_double(0x7ff8000000000000ll);
     else *out = d > 0 ? sqrt(n / d) : 0.0;
+}
+
+// ||r|| / ||b|| from residual_update_kernel's block sums (r^2 in
+// partial[0, nparts), b^2 in partial[nparts, 2 nparts)), with
+// sqrt_ratio_kernel's rules.
+__global__ void __launch_bounds__(REDUCE_THREADS)
+relres_finish_kernel(int nparts, const double* __restrict__ partial, double* __restrict__ out)
+{
+    double r = 0, b = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        r += partial[i];
+        b += partial[nparts + i];
+    }
+    r = block_sum(r);
+    b = block_sum(b);
+    if (threadIdx.x == 0) {
+        if (!isfinite(b) || !isfinite(r)) *out = __longlong_as_double(0x7ff8000000000000ll);
+        else *out = b > 0 ? sqrt(r / b) : 0.0;
+    }
+}
+
+// The divergence D_0 u_0 + D_1 u_1 + D_2 u_2 in one pass (twopoint_kernel's
+// value of each term, added in that order), times scale, stored in out; or,
+// with MAX, nothing stored and the largest |value| of each block's points
+// (NaN carried through) in partial[block].
+template <bool MAX>
+__global__ void __launch_bounds__(REDUCE_THREADS)
+divergence_kernel(twopoint_op<double> P0, twopoint_op<double> P1, twopoint_op<double> P2, double scale,
+                  const double* __restrict__ u0, const double* __restrict__ u1, const double* __restrict__ u2,
+                  double* __restrict__ out, double* __restrict__ partial)
+{
+    const unsigned n0 = (unsigned) P0.dn[0], n1 = (unsigned) P0.dn[1];
+    const size_t total = (size_t) n0 * n1 * P0.dn[2];
+    double m = 0;
+    for (size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t) gridDim.x * blockDim.x) {
+        int c[3];
+        {
+            const unsigned u = (unsigned) t, q = u / n0;
+            c[0] = (int) (u - q * n0);
+            c[1] = (int) (q % n1);
+            c[2] = (int) (q / n1);
+        }
+        double v = twopoint_value(P0, c, u0);
+        v = twopoint_value(P1, c, u1) + v;
+        v = twopoint_value(P2, c, u2) + v;
+        if (MAX) m = nan_max(m, fabs(v));
+        else out[t] = v * scale;
+    }
+    if (MAX) {
+        __shared__ double part[REDUCE_THREADS / 64];
+        for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0;
+            for (int w = 0; w < REDUCE_THREADS / 64; ++w) r = nan_max(r, part[w]);
+            partial[blockIdx.x] = r;
+        }
+    }
 }
 
 // y = alpha * B x + beta * y for the diagonal ins::boundary operator: one
@@ -1928,6 +2156,74 @@ ibm_status launch_max_abs(ibm_ctx* ctx, size_t n, const double* x, double* parti
 ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den, double* out)
 {
     sqrt_ratio_kernel<<<1, 1, 0, ctx->stream>>>(num, den, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+static dim3 stencil_grid(int nx, int ny, int nz)
+{
+    return dim3((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
+}
+
+size_t residual_parts(int nx, int ny, int nz)
+{
+    const dim3 g = stencil_grid(nx, ny, nz);
+    return 2 * (size_t) g.x * g.y * g.z;
+}
+
+ibm_status launch_momentum_rhs(ibm_ctx* ctx, const op3<double>& A, double dt, double dtnu, double inv_rho,
+                               const double* F, const double* H, const double* u, const double* b, double* rhs)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
+    prof_scope prof(ctx, "momentum_rhs", (b ? 40.0 : 32.0) * nx * ny * (double) nz);
+    const dim3 grid = stencil_grid(nx, ny, nz), block(STX, STY);
+    if (b) momentum_rhs_kernel<true><<<grid, block, 0, ctx->stream>>>(A, dt, dtnu, inv_rho, F, H, u, b, rhs);
+    else momentum_rhs_kernel<false><<<grid, block, 0, ctx->stream>>>(A, dt, dtnu, inv_rho, F, H, u, b, rhs);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status launch_residual_update(ibm_ctx* ctx, const op3<double>& A, const double* x, const double* b, double* u,
+                                  double* partial, size_t parts, double* out)
+{
+    const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
+    if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
+    const size_t need = residual_parts(nx, ny, nz);
+    if (need > parts || need / 2 > (size_t) INT32_MAX)
+        return fail(ctx, IBM_ERR_SIZE, "residual check: %zu partial sums, room for %zu", need, parts);
+    prof_scope prof(ctx, "residual_update", (u ? 32.0 : 16.0) * nx * ny * (double) nz);
+    const dim3 grid = stencil_grid(nx, ny, nz), block(STX, STY);
+    if (u) residual_update_kernel<true><<<grid, block, 0, ctx->stream>>>(A, x, b, u, partial);
+    else residual_update_kernel<false><<<grid, block, 0, ctx->stream>>>(A, x, b, u, partial);
+    IBM_LAUNCH_CHECK(ctx);
+    relres_finish_kernel<<<1, REDUCE_THREADS, 0, ctx->stream>>>((int) (need / 2), partial, out);
+    IBM_LAUNCH_CHECK(ctx);
+    return IBM_OK;
+}
+
+ibm_status launch_divergence(ibm_ctx* ctx, const twopoint_op<double> P[3], double scale, const double* const* u,
+                             double* out, double* partial, double* max_out)
+{
+    const size_t n = (size_t) P[0].dn[0] * P[0].dn[1] * P[0].dn[2];
+    if (!n) return IBM_OK;
+    size_t src = 0;
+    for (int i = 0; i < 3; ++i) {
+        const size_t ns = (size_t) P[i].sn[0] * P[i].sn[1] * P[i].sn[2];
+        if (ns > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
+        src += ns;
+    }
+    if (n > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
+    prof_scope prof(ctx, "divergence", 8.0 * (double) (src + (out ? n : 0)));
+    if (out) {
+        divergence_kernel<false><<<grid_for(n), REDUCE_THREADS, 0, ctx->stream>>>(P[0], P[1], P[2], scale, u[0], u[1],
+                                                                                  u[2], out, nullptr);
+    } else {
+        divergence_kernel<true><<<REDUCE_BLOCKS, REDUCE_THREADS, 0, ctx->stream>>>(P[0], P[1], P[2], scale, u[0],
+                                                                                   u[1], u[2], nullptr, partial);
+        IBM_LAUNCH_CHECK(ctx);
+        finish_max_kernel<<<1, REDUCE_THREADS, 0, ctx->stream>>>(REDUCE_BLOCKS, partial, max_out);
+    }
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }

@@ -71,6 +71,24 @@ static ibm_status check3(ibm_ctx* ctx, const ibm_grid* g, int n)
     return IBM_OK;
 }
 
+// ins::divergence (ins/differential.h:46-83): out = sum_i D_i u_i on the
+// grid reached by shifting u_i's grid along axis i (the cell centres for MAC),
+// the three terms in one pass.
+ibm_status divergence_of(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double scale, double* out,
+                         double* partial, double* max_out)
+{
+    twopoint_op<double> ops[3];
+    ibm_grid first;
+    for (int i = 0; i < 3; ++i) {
+        if (!u[i]) return fail(ctx, IBM_ERR_ARG, "null component");
+        ibm_grid dst;
+        IBM_TRY(make_twopoint(u_grids[i], i, 0, &ops[i], &dst));
+        if (i == 0) first = dst;
+        else if (!same_points(first, dst)) return fail(ctx, IBM_ERR_SIZE, "divergence terms on different grids");
+    }
+    return launch_divergence(ctx, ops, scale, u, out, partial, max_out);
+}
+
 } // namespace ibm
 
 using namespace ibm;
@@ -147,24 +165,12 @@ ibm_status ibm_ins_boundary_apply(ibm_ctx* ctx, const ibm_grid* g, int dim, int 
     return launch_ins_boundary(ctx, make_ins_boundary(*g, dim, order), alpha, x, beta, y);
 }
 
-// ins::divergence (ins/differential.h:46-83): out = sum_i D_i u_i on the
-// grid reached by shifting u_i's grid along axis i (the cell centres for MAC).
 ibm_status ibm_divergence(ibm_ctx* ctx, const ibm_grid* u_grids, const double* const* u, double* out)
 {
     if (!ctx) return IBM_ERR_ARG;
     IBM_TRY(check3(ctx, u_grids, 3));
     if (!u || !out) return fail(ctx, IBM_ERR_ARG, "null vector");
-    ibm_grid first;
-    for (int i = 0; i < 3; ++i) {
-        if (!u[i]) return fail(ctx, IBM_ERR_ARG, "null component");
-        twopoint_op<double> op;
-        ibm_grid dst;
-        IBM_TRY(make_twopoint(u_grids[i], i, 0, &op, &dst));
-        if (i == 0) first = dst;
-        else if (!same_points(first, dst)) return fail(ctx, IBM_ERR_SIZE, "divergence terms on different grids");
-        IBM_TRY(launch_twopoint<double>(ctx, op, 1.0, u[i], i ? 1.0 : 0.0, out));
-    }
-    return IBM_OK;
+    return divergence_of(ctx, u_grids, u, 1.0, out, nullptr, nullptr);
 }
 
 // ins::gradient (ins/differential.h:88-112): out_i = D_i p, on p's grid

@@ -63,13 +63,6 @@ __global__ void tether_kernel(size_t n, double k, const double* __restrict__ y, 
         f[i] = -k * (y[i] - ref[i]);
 }
 
-// h = F / rho - h
-__global__ void accel_kernel(size_t n, double inv_rho, const double* __restrict__ F, double* __restrict__ h)
-{
-    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
-        h[i] = F[i] * inv_rho - h[i];
-}
-
 // Initial guess of a warm-started solve from its last two solutions at the
 // same step size: x <- 2 x - prev (linear extrapolation in time) when the
 // device step counter *hist says both exist, x unchanged otherwise; prev <-
@@ -90,13 +83,6 @@ __global__ void extrapolate_kernel(size_t n, const int* hist, double* x, double*
 __global__ void history_kernel(int* hist, int v)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *hist = v >= 0 ? v : min(*hist + 1, 2);
-}
-
-// x *= a in place
-__global__ void scale_kernel(size_t n, double a, double* x)
-{
-    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
-        x[i] *= a;
 }
 
 int blocks(size_t n) { return (int) std::min<size_t>((n + 255) / 256, 65536); }
@@ -124,7 +110,8 @@ struct ibm_solver {
     bool direct = false;
     ibm_fft* fft_u[3]{};
     ibm_fft* fft_p = nullptr;
-    double* dpartial[3]{};             // per-component reduction partials of the residual checks
+    double* dpartial[4]{};             // reduction partials of the residual checks (u, v, w, p)
+    size_t dparts[4]{};
     ins_boundary_op Bw[3]{};           // ins::boundary(g[i], i), order 2 (pseudo-pressure walls)
     struct type_t {
         ibm_rbf* ops;
@@ -272,14 +259,6 @@ ibm_status track(ibm_ctx* ctx, ibm_solver* s)
     }
     for (int i = 0; i < 3; ++i) s->helm[i] = s->helm_e[e][i];
     return IBM_OK;
-}
-
-// ||r|| / ||b|| into *out (device), from the residual r of a direct solve.
-ibm_status direct_relres(ibm_ctx* c, size_t n, const double* b, const double* r, double* partial, double* out)
-{
-    IBM_TRY(launch_dot<double, double>(c, n, b, b, partial, out + 8));
-    IBM_TRY(launch_dot<double, double>(c, n, r, r, partial, out + 12));
-    return launch_sqrt_ratio(c, out + 12, out + 8, out);
 }
 
 // One timestep; `fixed` selects fixed V-cycle counts and no host reads.
@@ -463,16 +442,10 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     for (int i = 0; i < 3; ++i) {
         ibm_ctx* c = cc[i];
         const size_t n = s->nu[i];
-        accel_kernel<<<blocks(n), 256, 0, c->stream>>>(n, 1.0 / P.density, s->F[i], s->H[i]);
-        IBM_LAUNCH_CHECK(c);
-        op3<double> I = make_op<double>(s->g[i], 1.0, 0.0);
-        op3<double> L = make_op<double>(s->g[i], 0.0, 1.0);
-        // rhs = dt I~ (F/rho - H) + dt nu b
-        IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, I, dt, s->H[i], ub && ub[i] ? dt * nu : 0.0,
-                                                            ub ? ub[i] : nullptr, 0.0, nullptr, s->rhs[i])));
-        // rhs += dt nu L u
-        IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, L, dt * nu, st->u[i], 1.0, s->rhs[i], 0.0, nullptr,
-                                                            s->rhs[i])));
+        // rhs = dt I~ (F/rho - H) + dt nu (b + L u), one pass
+        const op3<double> L = make_op<double>(s->g[i], 0.0, 1.0);
+        IBM_TRY(step_status(ctx, c, launch_momentum_rhs(c, L, dt, dt * nu, 1.0 / P.density, s->F[i], s->H[i],
+                                                         st->u[i], ub ? ub[i] : nullptr, s->rhs[i])));
         if (P.flags & IBM_SOLVER_PSEUDO_PRESSURE) {
             // F[i] (consumed above) <- G_i p; rhs += lambda dt B_i F[i]
             twopoint_op<double> G;
@@ -483,15 +456,13 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
                                                              s->rhs[i])));
         }
         if (s->direct) {
-            // du = A^-1 rhs exactly; H[i] (consumed) receives rhs - A du for
-            // the FP64 residual check.
+            // du = A^-1 rhs exactly; then u += du with the FP64 residual
+            // check ||rhs - A du|| / ||rhs|| in the same pass.
             const double lambda = dt * nu / 2;
             IBM_TRY(step_status(ctx, c, fft_solve(c, s->fft_u[i], 1.0, -lambda, s->rhs[i], s->du[i])));
             const op3<double> A = make_op<double>(s->g[i], 1.0, -lambda);
-            IBM_TRY(step_status(ctx, c, launch_stencil<double>(c, A, -1.0, s->du[i], 0.0, nullptr, 1.0, s->rhs[i],
-                                                                s->H[i])));
-            IBM_TRY(step_status(ctx, c, direct_relres(c, n, s->rhs[i], s->H[i], s->dpartial[i], s->info + i)));
-            IBM_TRY(step_status(ctx, c, launch_axpby<double, double>(c, n, 1.0, s->du[i], 1.0, st->u[i])));
+            IBM_TRY(step_status(ctx, c, launch_residual_update(c, A, s->du[i], s->rhs[i], st->u[i], s->dpartial[i],
+                                                                s->dparts[i], s->info + i)));
             s->last_iters[i] = 1;
             continue;
         }
@@ -538,17 +509,15 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
     // 6. projection
     {
         const double* u[3] = {st->u[0], st->u[1], st->u[2]};
-        IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
-        scale_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, 1.0 / dt, s->div);
-        IBM_LAUNCH_CHECK(ctx);
+        IBM_TRY(divergence_of(ctx, s->g, u, 1.0 / dt, s->div, nullptr, nullptr));
         if (s->direct) {
-            // mean-free p with L p = div - mean(div); p_prev receives the
-            // residual for the FP64 check
+            // mean-free p with L p = div - mean(div), and the FP64 check
+            // ||div - mean - L p|| / ||div - mean||
             IBM_TRY(fft_solve(ctx, s->fft_p, 0.0, 1.0, s->div, st->p));
             const op3<double> Lp = make_op<double>(s->gp, 0.0, 1.0);
             IBM_TRY(launch_remove_mean<double>(ctx, s->np, s->div, s->partial, s->info + 7));
-            IBM_TRY(launch_stencil<double>(ctx, Lp, -1.0, st->p, 0.0, nullptr, 1.0, s->div, s->p_prev));
-            IBM_TRY(direct_relres(ctx, s->np, s->div, s->p_prev, s->partial, s->info + 3));
+            IBM_TRY(launch_residual_update(ctx, Lp, st->p, s->div, nullptr, s->dpartial[3], s->dparts[3],
+                                           s->info + 3));
             s->last_iters[3] = 1;
         } else {
         extrapolate_kernel<<<blocks(s->np), 256, 0, ctx->stream>>>(s->np, s->warm_hist, st->p, s->p_prev);
@@ -580,9 +549,8 @@ ibm_status step_impl(ibm_ctx* ctx, ibm_solver* s, ibm_state* st, const double* c
         }
         double* out[3] = {st->u[0], st->u[1], st->u[2]};
         IBM_TRY(ibm_gradient(ctx, &s->gp, -dt, st->p, 1.0, out));
-        // 7. FP64 divergence check
-        IBM_TRY(ibm_divergence(ctx, s->g, u, s->div));
-        IBM_TRY(launch_max_abs(ctx, s->np, s->div, s->partial, s->info + 4));
+        // 7. FP64 divergence check (max |D u|, the field itself not stored)
+        IBM_TRY(divergence_of(ctx, s->g, u, 1.0, nullptr, s->partial, s->info + 4));
     }
     // 8. interpolate at the data sites and move them
     {
@@ -672,7 +640,11 @@ ibm_status ibm_solver_create(ibm_ctx* ctx, const ibm_dimension* domain, int ndim
     if (s->direct) {
         for (int i = 0; i < 3 && st == IBM_OK; ++i) st = ibm_fft_create(ctx, &s->g[i], &s->fft_u[i]);
         if (st == IBM_OK) st = ibm_fft_create(ctx, &s->gp, &s->fft_p);
-        for (int i = 0; i < 3 && st == IBM_OK; ++i) st = alloc(ctx, s, REDUCE_PARTS, &s->dpartial[i]);
+        for (int i = 0; i < 4 && st == IBM_OK; ++i) {
+            const ibm_grid& g = i < 3 ? s->g[i] : s->gp;
+            s->dparts[i] = residual_parts(g.axis[0].points, g.axis[1].points, g.axis[2].points);
+            st = alloc(ctx, s, s->dparts[i], &s->dpartial[i]);
+        }
     } else {
         if (!(P.fscale > 0)) {
             const double lambda = P.timestep * P.diffusivity / 2;

@@ -75,6 +75,14 @@ def test_three_steps_match_oracle(ibm, orc, ctx, types, walls, pseudo, mg):
         assert max(info["residual"][:3]) <= 1e-12 and info["residual"][3] <= 1e-11
         umax = max(np.abs(c).max() for c in u)
         assert info["divergence"] <= 1e-9 * umax * 20
+        # the step's fused check (max |D u|, nothing stored) equals the
+        # maximum of the stored divergence field bitwise
+        g, _ = s.grids()
+        assert info["divergence"] == float(ibm.divergence(ctx, g, st.u).abs().max())
+        if walls and not mg:
+            # direct solves: the residuals are independent FP64 stencil
+            # evaluations at rounding level, not iteration tolerances
+            assert max(info["residual"][:3]) <= 1e-14 and info["residual"][3] <= 1e-12
     errs = [_rel(st.u[i].cpu().numpy(), u[i]) for i in range(3)]
     errs.append(_rel(st.p.cpu().numpy(), p))
     errs += [_rel(st.xs[j].cpu().numpy(), xs[j]) for j in range(len(types))]
