@@ -1156,7 +1156,16 @@ tile_spread_kernel(kgrids g, bin_desc bd, tile_desc td, int delta, int total, co
     const int bq[3] = {b0 * TB, b1 * TB, b2 * TB};
     const double inv_q = scale->inv_q;
     __syncthreads();
-    for (int i = s0 + tid; i < s1; i += TILE_THREADS) {
+    // Markers are sorted by bin, so neighbours in the sorted order often
+    // share a stencil origin, and lanes of one LDS atomic that hit the same
+    // address serialise.  Lane l of every wave therefore walks its own
+    // segment [l seg, (l + 1) seg) of the brick's markers (the waves
+    // interleaved in it): the lanes of one instruction are seg markers apart.
+    const int nm = s1 - s0, seg = (nm + 63) / 64, lane = tid & 63;
+    for (int r = tid >> 6; r < seg; r += TILE_THREADS / 64) {
+        const int li = lane * seg + r;
+        if (li >= nm) continue;
+        const int i = s0 + li;
         double p[3], fv[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
