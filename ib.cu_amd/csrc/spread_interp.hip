@@ -1225,64 +1225,58 @@ tile_spread_kernel(kgrids g, bin_desc bd, tile_desc td, int delta, int total, co
 }
 
 // Every grid point of every box and component: the sum of the tile entries
-// that cover it, times the quantum.
-__global__ void __launch_bounds__(256)
-tile_merge_kernel(kgrids g, bin_desc bd, tile_desc td, const unsigned long long* __restrict__ tiles,
+// that cover it, times the quantum.  Block (row, x chunk, box * 3 + c) takes
+// MERGE_THREADS points of one x-row (row = y + points_y z) of component c's
+// grid, so the y and z coverage is the same for the whole block.
+constexpr int MERGE_THREADS = 128;
+
+__global__ void __launch_bounds__(MERGE_THREADS)
+tile_merge_kernel(kgrids g, bin_desc bd, tile_desc td, int box0, const unsigned long long* __restrict__ tiles,
                   const int* __restrict__ flags, const tile_scale* __restrict__ scale, double* __restrict__ fe0,
                   double* __restrict__ fe1, double* __restrict__ fe2)
 {
-    const size_t n0 = (size_t) td.box_points[0], n1 = (size_t) td.box_points[1];
-    const size_t per_box = n0 + n1 + (size_t) td.box_points[2];
-    const size_t total = per_box * td.nboxes;
-    const double q = scale->q;
-    for (size_t gi = (size_t) blockIdx.x * blockDim.x + threadIdx.x; gi < total;
-         gi += (size_t) gridDim.x * blockDim.x) {
-        const int box = (int) (gi / per_box);
-        size_t r = gi - (size_t) box * per_box;
-        int c = 0;
-        if (r >= n0) { r -= n0; c = 1; if (r >= n1) { r -= n1; c = 2; } }
-        const int* pt = g.points[c];
-        int pp[3];
-        pp[0] = (int) (r % pt[0]);
-        pp[1] = (int) ((r / pt[0]) % pt[1]);
-        pp[2] = (int) (r / ((size_t) pt[0] * pt[1]));
-        // per axis: up to two (brick, tile index) pairs covering the point
-        int nb_c[3], bb[3][2], tt[3][2];
+    const int bz = blockIdx.z / 3, c = blockIdx.z - 3 * bz, box = box0 + bz;
+    const int* pt = g.points[c];
+    const int row = blockIdx.x;
+    const int x = blockIdx.y * MERGE_THREADS + threadIdx.x;
+    if (row >= pt[1] * pt[2] || x >= pt[0]) return;
+    const int pp[3] = {x, row % pt[1], row / pt[1]};
+    // per axis: up to two (brick, tile index) pairs covering the point
+    int nb_c[3], bb[3][2], tt[3][2];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const kaxis& a = g.a[c][j];
-            const int t0 = bd.qmin[j] + bd.lo[j] - 1;        // first tile point of brick 0
-            int rel = pp[j] - t0;
-            int cnt = 0;
-            if (!a.solid) {
-                // periodic: cells is a multiple of the brick edge here
-                rel = ((rel % a.cells) + a.cells) % a.cells;
-                int b = rel / TB, t = rel - b * TB;
-                bb[j][cnt] = b; tt[j][cnt] = t; ++cnt;
-                if (t + TB < TT) {
-                    int bp = b - 1 < 0 ? td.nbr[j] - 1 : b - 1;
-                    bb[j][cnt] = bp; tt[j][cnt] = t + TB; ++cnt;
-                }
-            } else {
-                int b = rel >= 0 ? rel / TB : -1;
-                int t = rel - b * TB;
-                if (b >= 0 && b < td.nbr[j]) { bb[j][cnt] = b; tt[j][cnt] = t; ++cnt; }
-                if (b - 1 >= 0 && b - 1 < td.nbr[j] && t + TB < TT) { bb[j][cnt] = b - 1; tt[j][cnt] = t + TB; ++cnt; }
+    for (int j = 0; j < 3; ++j) {
+        const kaxis& a = g.a[c][j];
+        const int t0 = bd.qmin[j] + bd.lo[j] - 1;        // first tile point of brick 0
+        int rel = pp[j] - t0;
+        int cnt = 0;
+        if (!a.solid) {
+            // periodic: cells is a multiple of the brick edge here
+            rel %= a.cells;
+            if (rel < 0) rel += a.cells;
+            const int b = rel / TB, t = rel - b * TB;
+            bb[j][cnt] = b; tt[j][cnt] = t; ++cnt;
+            if (t + TB < TT) {
+                bb[j][cnt] = b - 1 < 0 ? td.nbr[j] - 1 : b - 1; tt[j][cnt] = t + TB; ++cnt;
             }
-            nb_c[j] = cnt;
+        } else {
+            const int b = rel >= 0 ? rel / TB : -1;
+            const int t = rel - b * TB;
+            if (b >= 0 && b < td.nbr[j]) { bb[j][cnt] = b; tt[j][cnt] = t; ++cnt; }
+            if (b - 1 >= 0 && b - 1 < td.nbr[j] && t + TB < TT) { bb[j][cnt] = b - 1; tt[j][cnt] = t + TB; ++cnt; }
         }
-        long long sum = 0;
-        for (int iz = 0; iz < nb_c[2]; ++iz)
-            for (int iy = 0; iy < nb_c[1]; ++iy)
-                for (int ix = 0; ix < nb_c[0]; ++ix) {
-                    const int brick = ((box * td.nbr[2] + bb[2][iz]) * td.nbr[1] + bb[1][iy]) * td.nbr[0] + bb[0][ix];
-                    if (!flags[brick]) continue;
-                    const size_t at = ((size_t) brick * 3 + c) * TILE_PTS + tt[0][ix] + TT * (tt[1][iy] + TT * tt[2][iz]);
-                    sum += (long long) tiles[at];
-                }
-        double* fe = c == 0 ? fe0 : (c == 1 ? fe1 : fe2);
-        fe[(size_t) box * td.box_points[c] + r] = (double) sum * q;
+        nb_c[j] = cnt;
     }
+    long long sum = 0;
+    for (int iz = 0; iz < nb_c[2]; ++iz)
+        for (int iy = 0; iy < nb_c[1]; ++iy)
+            for (int ix = 0; ix < nb_c[0]; ++ix) {
+                const int brick = ((box * td.nbr[2] + bb[2][iz]) * td.nbr[1] + bb[1][iy]) * td.nbr[0] + bb[0][ix];
+                if (!flags[brick]) continue;
+                const size_t at = ((size_t) brick * 3 + c) * TILE_PTS + tt[0][ix] + TT * (tt[1][iy] + TT * tt[2][iz]);
+                sum += (long long) tiles[at];
+            }
+    double* fe = c == 0 ? fe0 : (c == 1 ? fe1 : fe2);
+    fe[(size_t) box * td.box_points[c] + (size_t) pt[0] * row + x] = (double) sum * scale->q;
 }
 
 // Interpolation over boxes: one lane per marker, grid vectors of box b at
@@ -1506,9 +1500,21 @@ ibm_status spread_tiles(ibm_ctx* ctx, const ibm_grid* grids, int delta, const ma
     }
     {
         prof_scope prof(ctx, timed ? "spread_merge" : nullptr, 8.0 * grid_points);
-        long long blocks = std::min<long long>(((long long) grid_points + 255) / 256, 65536);
-        tile_merge_kernel<<<(int) blocks, 256, 0, ctx->stream>>>(kg, bd, td, tiles, flags, sc, fe[0], fe[1], fe[2]);
-        IBM_LAUNCH_CHECK(ctx);
+        int rows = 0, xs = 0;
+        for (int c = 0; c < 3; ++c) {
+            rows = std::max(rows, kg.points[c][1] * kg.points[c][2]);
+            xs = std::max(xs, kg.points[c][0]);
+        }
+        const int xchunks = (xs + MERGE_THREADS - 1) / MERGE_THREADS;
+        if (xchunks > 65535) return fail(ctx, IBM_ERR_SIZE, "grid rows of %d points too long for the tile spread", xs);
+        constexpr int BOXES_PER_LAUNCH = 65535 / 3;
+        for (int b0 = 0; rows > 0 && b0 < td.nboxes; b0 += BOXES_PER_LAUNCH) {
+            const int nb = std::min(BOXES_PER_LAUNCH, td.nboxes - b0);
+            const dim3 mgrid((unsigned) rows, (unsigned) xchunks, (unsigned) (3 * nb));
+            tile_merge_kernel<<<mgrid, MERGE_THREADS, 0, ctx->stream>>>(kg, bd, td, b0, tiles, flags, sc, fe[0], fe[1],
+                                                                         fe[2]);
+            IBM_LAUNCH_CHECK(ctx);
+        }
     }
     return IBM_OK;
 }
