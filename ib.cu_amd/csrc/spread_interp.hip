@@ -1039,7 +1039,7 @@ __device__ inline int box_of(const int* __restrict__ off, int nboxes, int m)
 
 __global__ void tile_count_kernel(marker_sets sets, kgrids g, bin_desc bd, tile_desc td,
                                   const int* __restrict__ box_off, int* __restrict__ key, int* __restrict__ rank,
-                                  int* __restrict__ code, int* __restrict__ count, bin_stats* __restrict__ stats)
+                                  int* __restrict__ count, bin_stats* __restrict__ stats)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long fbits = 0;
@@ -1063,14 +1063,7 @@ __global__ void tile_count_kernel(marker_sets sets, kgrids g, bin_desc bd, tile_
         const int box = box_of(box_off, td.nboxes, m);
         const int brick = ((box * td.nbr[2] + br[2]) * td.nbr[1] + br[1]) * td.nbr[0] + br[0];
         bin = brick * (TB * TB * TB) + lb[0] + TB * (lb[1] + TB * lb[2]);
-        int c9 = 0;
-        for (int c = 0; c < 3; ++c)
-            for (int j = 0; j < 3; ++j) {
-                stencil_axis st = kstencil(g.a[c][j], p[j]);
-                c9 |= ((st.base - q[j]) & 1) << (3 * c + j);
-            }
         key[m] = bin;
-        code[m] = c9;
         const double* f = sets.f[k];
         const double f0 = fabs(f[l]), f1 = fabs(f[n + l]), f2 = fabs(f[2 * n + l]);
         double a = f0 > f1 ? f0 : f1;
@@ -1102,6 +1095,34 @@ __global__ void tile_count_kernel(marker_sets sets, kgrids g, bin_desc bd, tile_
     }
     fbits = wave_max(fbits);
     if ((threadIdx.x & 63) == 0 && fbits) atomicMax(&stats->max_abs_f[blockIdx.x % STAT_SLOTS], fbits);
+}
+
+// Marker m to slot start[key[m]] + rank[m] of its bin, positions and forces
+// gathered into `sorted` (SoA, 6 x total); the largest bin count into the
+// stats.  The order inside a bin is that of tile_count_kernel's atomics and
+// varies from run to run: the tiles sum integers, so it does not matter.
+__global__ void tile_scatter_kernel(marker_sets sets, const int* __restrict__ key, const int* __restrict__ rank,
+                                    const int* __restrict__ start, double* __restrict__ sorted,
+                                    bin_stats* __restrict__ stats)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = sets.total;
+    unsigned long long cnt = 0;
+    if (m < total) {
+        const int b = key[m];
+        const int s0 = start[b];
+        const int dst = s0 + rank[m];
+        cnt = (unsigned long long) (start[b + 1] - s0);
+        const int k = set_of(sets, m);
+        const int l = m - sets.offset[k];
+        const int n = sets.offset[k + 1] - sets.offset[k];
+        for (int j = 0; j < 3; ++j) {
+            sorted[(size_t) j * total + dst] = sets.x[k][(size_t) j * n + l];
+            sorted[(size_t) (3 + j) * total + dst] = sets.f[k][(size_t) j * n + l];
+        }
+    }
+    cnt = wave_max(cnt);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicMax(&stats->max_count[blockIdx.x % STAT_SLOTS], cnt);
 }
 
 // The quantum of the fixed-point tiles: q = 2^(e - 62) with e the exponent
@@ -1476,17 +1497,14 @@ ibm_status spread_tiles(ibm_ctx* ctx, const ibm_grid* grids, int delta, const ma
         IBM_LAUNCH_CHECK(ctx);
         if (sets.total > 0) {
             tile_count_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, kg, bd, td, box_off, w.key,
-                                                                                     w.rank, w.code, w.count, w.stats);
+                                                                                     w.rank, w.count, w.stats);
             IBM_LAUNCH_CHECK(ctx);
         }
         size_t tbytes = w.scan_bytes;
         IBM_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, tbytes, w.count, w.start, td.keys + 1, ctx->stream));
         if (sets.total > 0) {
-            bin_scatter_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets.total, w.key, w.rank, w.code,
-                                                                                      w.start, w.tmp);
-            IBM_LAUNCH_CHECK(ctx);
-            bin_canon_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, 1, w.key, w.start, w.tmp,
-                                                                                    w.perm, w.sorted, w.stats);
+            tile_scatter_kernel<<<blocks_for(sets.total, 256), 256, 0, ctx->stream>>>(sets, w.key, w.rank, w.start,
+                                                                                       w.sorted, w.stats);
             IBM_LAUNCH_CHECK(ctx);
         }
         tile_scale_kernel<<<1, 64, 0, ctx->stream>>>(kg, w.stats, sc);
