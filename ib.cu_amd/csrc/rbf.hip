@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -488,6 +489,14 @@ ibm_status rbf_eval_range(ibm_ctx* ctx, hipStream_t stream, int count, ibm_rbf* 
     }
     IBM_LAUNCH_CHECK(ctx);
     if (!tiles) return IBM_OK;
+    // Longest contraction first: each XCD takes its share of the groups in
+    // group order, so the short tiles come last and fill the tail.
+    std::stable_sort(G.g, G.g + G.count, [](const gemm_group& a, const gemm_group& b) { return a.Kp > b.Kp; });
+    tiles = 0;
+    for (int q = 0; q < G.count; ++q) {
+        G.g[q].first_tile = tiles;
+        tiles += (round_up(G.g[q].M, TILE) / TILE) * G.g[q].tiles_n;
+    }
     G.total_tiles = tiles;
     // blocks: 8 x the most tiles any XCD takes (its share of every group)
     int per_xcd = 0;
