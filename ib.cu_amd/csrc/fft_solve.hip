@@ -53,25 +53,58 @@ __device__ inline int bitrev(int k, int logn) { return (int) (__brev((unsigned) 
 // touch consecutive LDS words either way.  tw[m] = exp(-2 pi i m / n),
 // m < n / 2, also in LDS (a stage's twiddle reads would otherwise be
 // dependent global loads); `inverse` conjugates the twiddles (no scaling).
-// Every thread of the block takes part in every stage.
+// Every thread of the block takes part in every stage.  Stages are done in
+// pairs: a thread loads the quad k0, k0 + h, k0 + 2h, k0 + 3h (h = 2^(s-1)),
+// does stage s's two butterflies and stage s + 1's two, and stores it -- the
+// radix-2 butterflies and twiddles exactly, with one LDS round trip and one
+// barrier per two stages.
+__device__ inline void butterfly(double2& u, double2& v, double2 w, bool inverse)
+{
+    if (inverse) w.y = -w.y;
+    const double2 t = cmul(v, w);
+    v = make_double2(u.x - t.x, u.y - t.y);
+    u = make_double2(u.x + t.x, u.y + t.y);
+}
+
 template <bool COLS>
 __device__ void fft_stages(double2* a, int count, int n, int logn, const double2* tw, bool inverse)
 {
-    const int half_n = n >> 1;
-    const int total = count * half_n;
-    for (int s = 1; s <= logn; ++s) {
-        const int half = 1 << (s - 1);
-        const int stride = n >> s;                 // twiddle index step
+    int s = 1;
+    const int quads = count * (n >> 2);
+    for (; s + 1 <= logn; s += 2) {
+        const int h = 1 << (s - 1);
+        const int st0 = n >> s, st1 = n >> (s + 1);       // twiddle index steps of stages s and s + 1
+        for (int b = threadIdx.x; b < quads; b += blockDim.x) {
+            const int t = COLS ? b % count : b / (n >> 2), bb = COLS ? b / count : b - t * (n >> 2);
+            const int grp = bb >> (s - 1), pos = bb & (h - 1);
+            const int k0 = grp * 4 * h + pos;
+            const int i0 = COLS ? k0 * count + t : t * n + k0;
+            const int dh = COLS ? h * count : h;
+            double2 q0 = a[i0], q1 = a[i0 + dh], q2 = a[i0 + 2 * dh], q3 = a[i0 + 3 * dh];
+            const double2 w0 = tw[pos * st0];
+            butterfly(q0, q1, w0, inverse);
+            butterfly(q2, q3, w0, inverse);
+            butterfly(q0, q2, tw[pos * st1], inverse);
+            butterfly(q1, q3, tw[(pos + h) * st1], inverse);
+            a[i0] = q0;
+            a[i0 + dh] = q1;
+            a[i0 + 2 * dh] = q2;
+            a[i0 + 3 * dh] = q3;
+        }
+        __syncthreads();
+    }
+    if (s == logn) {
+        const int half_n = n >> 1, half = 1 << (s - 1), stride = n >> s;
+        const int total = count * half_n;
         for (int b = threadIdx.x; b < total; b += blockDim.x) {
             const int t = COLS ? b % count : b / half_n, bb = COLS ? b / count : b - t * half_n;
             const int grp = bb >> (s - 1), pos = bb & (half - 1);
             const int k0 = grp * 2 * half + pos, k1 = k0 + half;
             const int i = COLS ? k0 * count + t : t * n + k0, j = COLS ? k1 * count + t : t * n + k1;
-            double2 w = tw[pos * stride];
-            if (inverse) w.y = -w.y;
-            const double2 u = a[i], v = cmul(a[j], w);
-            a[i] = make_double2(u.x + v.x, u.y + v.y);
-            a[j] = make_double2(u.x - v.x, u.y - v.y);
+            double2 u = a[i], v = a[j];
+            butterfly(u, v, tw[pos * stride], inverse);
+            a[i] = u;
+            a[j] = v;
         }
         __syncthreads();
     }
