@@ -221,10 +221,12 @@ __device__ inline void tri_row(const wall_axis& y, double sigma, double tau, dou
 // reduction in LDS: block (kx tile of zt modes, kz) holds zt x np2 rows
 // (np2 = ny rounded up to a power of two; rows past ny are identity rows),
 // 256 threads update PCR_ITEMS rows each per step (row j combined with rows
-// j -+ 2^s, log2(np2) steps), then x_j = d_j / b_j.  The singular mode's last
+// j -+ 2^s) until P interleaved subsystems of np2 / P <= PCR_SPLIT rows are
+// left (P >= PCR_SPLIT where np2 allows), which one thread each solves by
+// the Thomas algorithm.  The singular mode's last
 // equation is replaced by x = 0 after its right-hand side loses its mean,
 // and its solution loses its mean.
-constexpr int PCR_THREADS = 256, PCR_ITEMS = 4, PCR_ROWS = PCR_THREADS * PCR_ITEMS;
+constexpr int PCR_THREADS = 256, PCR_ITEMS = 4, PCR_ROWS = PCR_THREADS * PCR_ITEMS, PCR_SPLIT = 16;
 
 __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np2, wall_axis y, double sigma,
                                                           double tau, int singular, const double* __restrict__ lam_x,
@@ -271,7 +273,11 @@ __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np
         D[last] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int sft = 1; sft < np2; sft <<= 1) {
+    // Reduction until the rows left coupled (j -+ P) form P interleaved
+    // subsystems of at most PCR_SPLIT rows, then one Thomas sweep per
+    // subsystem.
+    const int P = max(min(np2, PCR_SPLIT), np2 / PCR_SPLIT);
+    for (int sft = 1; sft < P; sft <<= 1) {
         double na[PCR_ITEMS], nb[PCR_ITEMS], nc[PCR_ITEMS];
         double2 nd[PCR_ITEMS];
 #pragma unroll
@@ -316,9 +322,29 @@ __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
-        const double inv = 1.0 / B[e];
-        D[e] = make_double2(D[e].x * inv, D[e].y * inv);
+    const int L = np2 / P, step = P * zt;
+    for (int id = threadIdx.x; id < zt * P; id += PCR_THREADS) {
+        int e = id;                                  // row r = id / zt of mode t = id % zt
+        double inv = 1.0 / B[e];
+        double cp = C[e] * inv;
+        double2 dp = make_double2(D[e].x * inv, D[e].y * inv);
+        C[e] = cp;
+        D[e] = dp;
+        for (int m = 1; m < L; ++m) {
+            e += step;
+            const double a = A[e];
+            inv = 1.0 / (B[e] - a * cp);
+            cp = C[e] * inv;
+            dp = make_double2((D[e].x - a * dp.x) * inv, (D[e].y - a * dp.y) * inv);
+            C[e] = cp;
+            D[e] = dp;
+        }
+        double2 xv = dp;
+        for (int m = L - 2; m >= 0; --m) {
+            e -= step;
+            xv = make_double2(D[e].x - C[e] * xv.x, D[e].y - C[e] * xv.y);
+            D[e] = xv;
+        }
     }
     __syncthreads();
     if (pin_block && threadIdx.x == 0) {
