@@ -885,7 +885,7 @@ int blocks_for(long long n, int threads) { return (int) ((n + threads - 1) / thr
 struct bin_work {
     int *count, *start, *key, *rank, *perm, *code;
     long long* tmp;
-    double* sorted;          // 6 * total doubles: positions, then forces (SoA)
+    double* sorted;          // 6 * total doubles: positions, then forces (SoA; per-marker records in the tile path)
     bin_stats* stats;
     void* scan_tmp;
     size_t scan_bytes;
@@ -1098,8 +1098,8 @@ __global__ void tile_count_kernel(marker_sets sets, kgrids g, bin_desc bd, tile_
 }
 
 // Marker m to slot start[key[m]] + rank[m] of its bin, positions and forces
-// gathered into `sorted` (SoA, 6 x total); the largest bin count into the
-// stats.  The order inside a bin is that of tile_count_kernel's atomics and
+// gathered into `sorted` (six consecutive doubles per slot: x, y, z, fx, fy,
+// fz, one 48-byte store); the largest bin count into the stats.  The order inside a bin is that of tile_count_kernel's atomics and
 // varies from run to run: the tiles sum integers, so it does not matter.
 __global__ void tile_scatter_kernel(marker_sets sets, const int* __restrict__ key, const int* __restrict__ rank,
                                     const int* __restrict__ start, double* __restrict__ sorted,
@@ -1116,10 +1116,12 @@ __global__ void tile_scatter_kernel(marker_sets sets, const int* __restrict__ ke
         const int k = set_of(sets, m);
         const int l = m - sets.offset[k];
         const int n = sets.offset[k + 1] - sets.offset[k];
-        for (int j = 0; j < 3; ++j) {
-            sorted[(size_t) j * total + dst] = sets.x[k][(size_t) j * n + l];
-            sorted[(size_t) (3 + j) * total + dst] = sets.f[k][(size_t) j * n + l];
-        }
+        const double* x = sets.x[k];
+        const double* f = sets.f[k];
+        double2* o = (double2*) (sorted + (size_t) 6 * dst);
+        o[0] = make_double2(x[l], x[(size_t) n + l]);
+        o[1] = make_double2(x[(size_t) 2 * n + l], f[l]);
+        o[2] = make_double2(f[(size_t) n + l], f[(size_t) 2 * n + l]);
     }
     cnt = wave_max(cnt);
     if ((threadIdx.x & 63) == 0 && cnt) atomicMax(&stats->max_count[blockIdx.x % STAT_SLOTS], cnt);
@@ -1187,12 +1189,9 @@ tile_spread_kernel(kgrids g, bin_desc bd, tile_desc td, int delta, int total, co
         const int li = lane * seg + r;
         if (li >= nm) continue;
         const int i = s0 + li;
-        double p[3], fv[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            p[j] = sorted[(size_t) j * total + i];
-            fv[j] = sorted[(size_t) (3 + j) * total + i];
-        }
+        const double2* rec = (const double2*) (sorted + (size_t) 6 * i);
+        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        const double p[3] = {r0.x, r0.y, r1.x}, fv[3] = {r1.y, r2.x, r2.y};
         // bin coordinates of the marker (relative to its brick), as binned
         int lq[3];
 #pragma unroll
