@@ -1446,17 +1446,18 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_vcycle_kernel(coarse_le
 template <typename T>
 __device__ inline T twopoint_value(const twopoint_op<T>& P, const int c[3], const T* __restrict__ x)
 {
+    // 32-bit indices: the launchers keep every grid below 2^32 points
     const int v = P.view;
     const int cols = P.sn[v];
-    size_t stride = 1;
-    for (int d = 0; d < v; ++d) stride *= (size_t) P.sn[d];
+    const unsigned sn0 = (unsigned) P.sn[0], sn1 = (unsigned) P.sn[1];
+    const unsigned stride = v == 0 ? 1u : (v == 1 ? sn0 : sn0 * sn1);
     int base[3] = {c[0], c[1], c[2]};
     base[v] = 0;
-    const size_t b0 = (size_t) base[0] + (size_t) P.sn[0] * ((size_t) base[1] + (size_t) P.sn[1] * base[2]);
+    const unsigned b0 = (unsigned) base[0] + sn0 * ((unsigned) base[1] + sn1 * (unsigned) base[2]);
     auto fetch = [&](int idx) -> T {
-        if (idx >= 0 && idx < cols) return x[b0 + stride * idx];
-        if (P.periodic) return x[b0 + stride * (idx < 0 ? idx + cols : idx - cols)];
-        return idx < 0 ? P.int_lo * x[b0] : P.int_hi * x[b0 + stride * (cols - 1)];
+        if (idx >= 0 && idx < cols) return x[b0 + stride * (unsigned) idx];
+        if (P.periodic) return x[b0 + stride * (unsigned) (idx < 0 ? idx + cols : idx - cols)];
+        return idx < 0 ? P.int_lo * x[b0] : P.int_hi * x[b0 + stride * (unsigned) (cols - 1)];
     };
     const int r = c[v];
     return P.c0 * fetch(r - P.s1) + P.c1 * fetch(r + 1 - P.s1);
@@ -1532,11 +1533,12 @@ __global__ void advection_kernel(advection_op<T> Op, const T* __restrict__ u0, c
         const twopoint_op<T>& B = Op.B[j];
         const T* uj = u[j];
         auto prod = [&](int x, int y, int z) -> T {
+            // 32-bit indices: the launcher keeps every grid below 2^32 points
             T a = twopoint_at(A, x, y, z, [&](int p, int q, int r) -> T {
-                return uj[(size_t) p + (size_t) A.sn[0] * ((size_t) q + (size_t) A.sn[1] * r)];
+                return uj[(unsigned) p + (unsigned) A.sn[0] * ((unsigned) q + (unsigned) A.sn[1] * (unsigned) r)];
             });
             T b = twopoint_at(B, x, y, z, [&](int p, int q, int r) -> T {
-                return ui[(size_t) p + (size_t) B.sn[0] * ((size_t) q + (size_t) B.sn[1] * r)];
+                return ui[(unsigned) p + (unsigned) B.sn[0] * ((unsigned) q + (unsigned) B.sn[1] * (unsigned) r)];
             });
             return a * b;
         };
@@ -2084,7 +2086,12 @@ ibm_status launch_advection(ibm_ctx* ctx, const advection_op<T>& Op, const T* co
     if (!n) return IBM_OK;
     if (n > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
     double nsrc = 0;
-    for (int d = 0; d < 3; ++d) nsrc += (double) Op.A[d].sn[0] * Op.A[d].sn[1] * Op.A[d].sn[2];
+    for (int d = 0; d < 3; ++d) {
+        const size_t na = (size_t) Op.A[d].sn[0] * Op.A[d].sn[1] * Op.A[d].sn[2];
+        const size_t nb = (size_t) Op.B[d].sn[0] * Op.B[d].sn[1] * Op.B[d].sn[2];
+        if (na > MAX_GRID_POINTS || nb > MAX_GRID_POINTS) return fail(ctx, IBM_ERR_SIZE, "grid too large");
+        nsrc += (double) na;
+    }
     prof_scope prof(ctx, "advection", (double) sizeof(T) * (nsrc + n));   // u^0..2 read once, H^i written
     advection_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], comp, H);
     IBM_LAUNCH_CHECK(ctx);
