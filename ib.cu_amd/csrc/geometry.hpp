@@ -9,6 +9,7 @@
 // falls in is decided by floor() of a rounded expression, and fusing a
 // multiply-add would move markers that sit within an ulp of a cell face.
 #pragma once
+#include <cmath>
 #include "ibm.h"
 
 #if defined(__HIPCC__)
@@ -21,10 +22,12 @@
 namespace ibm {
 
 // floor() as the reference writes it (util/math.h:158-162): truncation toward
-// zero, corrected by one for negative non-integers.
+// zero, corrected by one for negative non-integers.  The reference truncates
+// through a cast to long; trunc() is the same value wherever that cast is
+// defined (|x| < 2^63) and a single instruction on the GPU.
 IBM_HD double ref_floor(double x)
 {
-    return (double) (long) (x - (x < 0 && (-x != (double) (long) -x)));
+    return trunc(x - (x < 0 && (-x != trunc(-x))));
 }
 
 // modulo(x, s) = x - s * floor(x / s) (util/math.h:164-168).
