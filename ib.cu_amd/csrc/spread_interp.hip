@@ -707,6 +707,9 @@ spread_brick_kernel(kgrids g, bin_desc bd, int delta, int total,
 // ----------------------------------------------------------------------
 // Interpolation: one lane per marker.
 // ----------------------------------------------------------------------
+// Four doubles at 8-byte alignment (global loads need only dword alignment).
+typedef double d4u8 __attribute__((ext_vector_type(4), aligned(8)));
+
 __global__ void __launch_bounds__(256)
 interp_kernel(marker_sets sets, kgrids g, int delta,
               const double* __restrict__ u0, const double* __restrict__ u1,
@@ -735,15 +738,29 @@ interp_kernel(marker_sets sets, kgrids g, int delta,
             }
         }
         const double* u = ue[c];
+        // the four x points are adjacent except across a periodic seam or a
+        // clamped wall: then one 32-byte load per row instead of four
+        const bool adjacent = ix[1] == ix[0] + 1 && ix[2] == ix[0] + 2 && ix[3] == ix[0] + 3;
         double acc = 0.0;
         for (int a = 0; a < 4; ++a) {
             double sy = 0.0;
             for (int b = 0; b < 4; ++b) {
                 const double* row = u + oz[a] + oy[b];
-                double sx = w[0][0] * row[ix[0]];
-                sx = fma(w[0][1], row[ix[1]], sx);
-                sx = fma(w[0][2], row[ix[2]], sx);
-                sx = fma(w[0][3], row[ix[3]], sx);
+                double v[4];
+                if (adjacent) {
+                    const d4u8 q = *(const d4u8*) (row + ix[0]);
+                    v[0] = q.x;
+                    v[1] = q.y;
+                    v[2] = q.z;
+                    v[3] = q.w;
+                } else {
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) v[o] = row[ix[o]];
+                }
+                double sx = w[0][0] * v[0];
+                sx = fma(w[0][1], v[1], sx);
+                sx = fma(w[0][2], v[2], sx);
+                sx = fma(w[0][3], v[3], sx);
                 sy = fma(w[1][b], sx, sy);
             }
             acc = fma(w[2][a], sy, acc);
