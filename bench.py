@@ -254,6 +254,9 @@ PMC_KERNELS = {"chebyshev_f32": "cheb_march_kernel<float", "chebyshev_f64": "che
                "rbf_gemm": "rbf_gemm_kernel",
                "spread_brick": "spread_brick_kernel", "spread_tile": "tile_spread_kernel",
                "spread_merge": "tile_merge_kernel", "interp": "interp_kernel", "advection": "advection_kernel",
+               "momentum_rhs": "momentum_rhs_kernel", "residual_update": "residual_update_kernel",
+               "divergence": "divergence_kernel",
+               "bin": ("tile_count_kernel", "tile_scatter_kernel"),
                "fft_solve": ("xfft_forward_kernel", "zfft_kernel", "pcr_kernel", "xfft_inverse_kernel")}
 COPY_BYTES = 1 << 28     # the calibration copy: 256 MiB read, 256 MiB written
 
