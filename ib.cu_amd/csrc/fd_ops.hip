@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 #include <cmath>
 #include <vector>
@@ -1509,12 +1510,39 @@ __device__ inline T twopoint_at(const twopoint_op<T>& P, int x, int y, int z, co
     return P.c0 * f0 + P.c1 * f1;
 }
 
+// twopoint_at with the view a compile-time constant (no run-time indexing
+// of the coordinate triple); the same fetches and the same arithmetic.
+template <int V, typename T, typename F>
+__device__ inline T twopoint_at_v(const twopoint_op<T>& P, int x, int y, int z, const F& src)
+{
+    const int cols = P.sn[V], r = V == 0 ? x : (V == 1 ? y : z);
+    auto at = [&](int idx) -> T {
+        return V == 0 ? src(idx, y, z) : (V == 1 ? src(x, idx, z) : src(x, y, idx));
+    };
+    auto fetch = [&](int idx) -> T {
+        if (idx < 0 || idx >= cols) {
+            if (P.periodic) {
+                idx = idx < 0 ? idx + cols : idx - cols;
+            } else {
+                T g = idx < 0 ? P.int_lo : P.int_hi;
+                return g * at(idx < 0 ? 0 : cols - 1);
+            }
+        }
+        return at(idx);
+    };
+    T f0 = fetch(r - P.s1);
+    T f1 = fetch(r + 1 - P.s1);
+    return P.c0 * f0 + P.c1 * f1;
+}
+
 // ins::advection for one component in one pass:
 //   H^i = sum_j D_j [ (A_i u^j) o (A_j u^i) ]
-// evaluated point by point on u^i's grid (j summed in order 0, 1, 2).
-template <typename T>
+// evaluated point by point on u^i's grid (j summed in order 0, 1, 2).  The
+// component I is a template parameter, so every operator's axis is known
+// at compile time (D_j and A_j along j, A_i along I).
+template <typename T, int I>
 __global__ void advection_kernel(advection_op<T> Op, const T* __restrict__ u0, const T* __restrict__ u1,
-                                 const T* __restrict__ u2, int comp, T* __restrict__ H)
+                                 const T* __restrict__ u2, T* __restrict__ H)
 {
     const int* dn = Op.D[0].dn;
     const size_t total = (size_t) dn[0] * dn[1] * dn[2];
@@ -1524,27 +1552,31 @@ __global__ void advection_kernel(advection_op<T> Op, const T* __restrict__ u0, c
     const int cx = (int) (tt - q * (unsigned) dn[0]);
     const int cy = (int) (q % (unsigned) dn[1]);
     const int cz = (int) (q / (unsigned) dn[1]);
-    const T* u[3] = {u0, u1, u2};
-    const T* ui = u[comp];
+    const T* ui = I == 0 ? u0 : (I == 1 ? u1 : u2);
     T acc = 0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    auto term = [&](auto J) {
+        constexpr int j = decltype(J)::value;
         const twopoint_op<T>& A = Op.A[j];
         const twopoint_op<T>& B = Op.B[j];
-        const T* uj = u[j];
+        const T* uj = j == 0 ? u0 : (j == 1 ? u1 : u2);
+        // 32-bit indices: the launcher keeps every grid below 2^32 points
+        const unsigned as0 = (unsigned) A.sn[0], as1 = (unsigned) A.sn[1];
+        const unsigned bs0 = (unsigned) B.sn[0], bs1 = (unsigned) B.sn[1];
         auto prod = [&](int x, int y, int z) -> T {
-            // 32-bit indices: the launcher keeps every grid below 2^32 points
-            T a = twopoint_at(A, x, y, z, [&](int p, int q, int r) -> T {
-                return uj[(unsigned) p + (unsigned) A.sn[0] * ((unsigned) q + (unsigned) A.sn[1] * (unsigned) r)];
+            T a = twopoint_at_v<I>(A, x, y, z, [&](int p, int q2, int r) -> T {
+                return uj[(unsigned) p + as0 * ((unsigned) q2 + as1 * (unsigned) r)];
             });
-            T b = twopoint_at(B, x, y, z, [&](int p, int q, int r) -> T {
-                return ui[(unsigned) p + (unsigned) B.sn[0] * ((unsigned) q + (unsigned) B.sn[1] * (unsigned) r)];
+            T b = twopoint_at_v<j>(B, x, y, z, [&](int p, int q2, int r) -> T {
+                return ui[(unsigned) p + bs0 * ((unsigned) q2 + bs1 * (unsigned) r)];
             });
             return a * b;
         };
-        T d = twopoint_at(Op.D[j], cx, cy, cz, prod);
+        T d = twopoint_at_v<j>(Op.D[j], cx, cy, cz, prod);
         acc = j == 0 ? d : acc + d;
-    }
+    };
+    term(std::integral_constant<int, 0>{});
+    term(std::integral_constant<int, 1>{});
+    term(std::integral_constant<int, 2>{});
     H[t] = acc;
 }
 
@@ -2093,7 +2125,14 @@ ibm_status launch_advection(ibm_ctx* ctx, const advection_op<T>& Op, const T* co
         nsrc += (double) na;
     }
     prof_scope prof(ctx, "advection", (double) sizeof(T) * (nsrc + n));   // u^0..2 read once, H^i written
-    advection_kernel<T><<<(unsigned) ((n + 255) / 256), 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], comp, H);
+    if (comp < 0 || comp > 2) return fail(ctx, IBM_ERR_ARG, "advection component %d", comp);
+    for (int j = 0; j < 3; ++j)
+        if (Op.D[j].view != j || Op.B[j].view != j || Op.A[j].view != comp)
+            return fail(ctx, IBM_ERR_ARG, "advection operators along unexpected axes");
+    const unsigned blocks = (unsigned) ((n + 255) / 256);
+    if (comp == 0) advection_kernel<T, 0><<<blocks, 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], H);
+    else if (comp == 1) advection_kernel<T, 1><<<blocks, 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], H);
+    else advection_kernel<T, 2><<<blocks, 256, 0, ctx->stream>>>(Op, u[0], u[1], u[2], H);
     IBM_LAUNCH_CHECK(ctx);
     return IBM_OK;
 }
