@@ -40,6 +40,11 @@ constexpr int MAX_FFT_POINTS = 512;
 constexpr int MAX_WALL_POINTS = 800;
 constexpr size_t MAX_DYNAMIC_LDS = 160 * 1024;
 
+// Row stride of the half-spectrum in coefficients: nh rounded up to 8 (rows
+// start on 128-byte boundaries, so the column tiles of the z transform and
+// the cyclic reduction read whole cache lines).
+__host__ __device__ constexpr int spec_ld(int nh) { return (nh + 7) & ~7; }
+
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
 __device__ inline int bitrev(int k, int logn) { return (int) (__brev((unsigned) k) >> (32 - logn)); }
@@ -132,7 +137,7 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_forward_kernel(int nx, int l
     fft_stages<false>(sh, rpb, nx, logn, stw, false);
     for (int e = threadIdx.x; e < rpb * nh; e += blockDim.x) {
         const int t = e / nh, k = e - t * nh, row = r0 + t;
-        if (row < rows) s[(size_t) row * nh + k] = sh[t * nx + k];
+        if (row < rows) s[(size_t) row * spec_ld(nh) + k] = sh[t * nx + k];
     }
 }
 
@@ -153,9 +158,9 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
         double2 v = make_double2(0.0, 0.0);
         if (row < rows) {
             if (k < nh) {
-                v = s[(size_t) row * nh + k];
+                v = s[(size_t) row * spec_ld(nh) + k];
             } else {
-                v = s[(size_t) row * nh + (nx - k)];
+                v = s[(size_t) row * spec_ld(nh) + (nx - k)];
                 v.y = -v.y;
             }
         }
@@ -170,25 +175,26 @@ __global__ void __launch_bounds__(FFT_THREADS) xfft_inverse_kernel(int nx, int l
 }
 
 // Transform along z, in place, of s laid out [k][j][kx] (kx fastest, nh of
-// them): block (kx tile, j) transforms ZT columns, held in LDS as rows of
-// ZT consecutive kx.
+// them in rows of spec_ld(nh)): block (kx tile, j) transforms ZT columns,
+// held in LDS as rows of ZT consecutive kx.
 __global__ void __launch_bounds__(FFT_THREADS) zfft_kernel(int nh, int ny, int nz, int logn, double2* __restrict__ s,
                                                            const double2* __restrict__ tw, int inverse)
 {
     extern __shared__ double2 sh[];
     double2* stw = sh + ZT * nz;
     const int kx0 = blockIdx.x * ZT, j = blockIdx.y;
-    const size_t plane = (size_t) nh * ny;
+    const int ld = spec_ld(nh);
+    const size_t plane = (size_t) ld * ny;
     for (int e = threadIdx.x; e < nz / 2; e += blockDim.x) stw[e] = tw[e];
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
         const int t = e % ZT, k = e / ZT, kx = kx0 + t, kr = bitrev(k, logn);
-        sh[e] = kx < nh ? s[(size_t) kr * plane + (size_t) j * nh + kx] : make_double2(0.0, 0.0);
+        sh[e] = kx < nh ? s[(size_t) kr * plane + (size_t) j * ld + kx] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_stages<true>(sh, ZT, nz, logn, stw, inverse != 0);
     for (int e = threadIdx.x; e < ZT * nz; e += blockDim.x) {
         const int t = e % ZT, k = e / ZT, kx = kx0 + t;
-        if (kx < nh) s[(size_t) k * plane + (size_t) j * nh + kx] = sh[e];
+        if (kx < nh) s[(size_t) k * plane + (size_t) j * ld + kx] = sh[e];
     }
 }
 
@@ -238,7 +244,8 @@ __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np
     double* B = A + PCR_ROWS;
     double* C = B + PCR_ROWS;
     const int ny = y.n, kx0 = blockIdx.x * zt, kz = blockIdx.y;
-    const size_t plane = (size_t) nh * ny;
+    const int ld = spec_ld(nh);
+    const size_t plane = (size_t) ld * ny;
     const bool pin_block = singular && kx0 == 0 && kz == 0;     // holds the singular mode (t = 0)
     for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
         const int t = e % zt, j = e / zt, kx = kx0 + t;
@@ -246,7 +253,7 @@ __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np
         double2 d = make_double2(0.0, 0.0);
         if (kx < nh && j < ny) {
             tri_row(y, sigma, tau, lam_x[kx] + lam_z[kz], j, a, c, b);
-            d = s[(size_t) kz * plane + (size_t) j * nh + kx];
+            d = s[(size_t) kz * plane + (size_t) j * ld + kx];
         }
         A[e] = a;
         B[e] = b;
@@ -363,7 +370,7 @@ __global__ void __launch_bounds__(PCR_THREADS) pcr_kernel(int nh, int zt, int np
     __syncthreads();
     for (int e = threadIdx.x; e < PCR_ROWS; e += PCR_THREADS) {
         const int t = e % zt, j = e / zt, kx = kx0 + t;
-        if (kx < nh && j < ny) s[(size_t) kz * plane + (size_t) j * nh + kx] = D[e];
+        if (kx < nh && j < ny) s[(size_t) kz * plane + (size_t) j * ld + kx] = D[e];
     }
 }
 
@@ -404,7 +411,7 @@ struct ibm_fft {
     int nx, ny, nz, nh, lx, lz;
     ibm::wall_axis y;
     bool neumann;                 // both walls Neumann: sigma = 0 is the singular (mean-free) case
-    double2* spec = nullptr;      // nh x ny x nz coefficients
+    double2* spec = nullptr;      // nz x ny rows of nh coefficients, row stride spec_ld(nh)
     double2* twx = nullptr;
     double2* twz = nullptr;
     double* lam_x = nullptr;
@@ -513,7 +520,7 @@ ibm_status ibm_fft_create(ibm_ctx* ctx, const ibm_grid* g, ibm_fft** out)
     const std::vector<double> lz = eigenvalues(p->nz, g->axis[2].resolution * g->axis[2].resolution, p->nz);
     void* m = nullptr;
     ibm_status st = IBM_OK;
-    if ((st = fft_alloc(ctx, p, sizeof(double2) * p->nh * p->ny * (size_t) p->nz, &m)) == IBM_OK) {
+    if ((st = fft_alloc(ctx, p, sizeof(double2) * spec_ld(p->nh) * p->ny * (size_t) p->nz, &m)) == IBM_OK) {
         p->spec = (double2*) m;
         st = fft_alloc(ctx, p, sizeof(double2) * twx.size(), &m);
     }
