@@ -47,6 +47,10 @@ __device__ inline T id1(const op_axis<T>& a, int i)
 }
 
 constexpr int STX = 64, STY = 4, KZ = 8;
+// Planes per thread of the fused momentum and residual kernels: they hold
+// three or four more vectors per plane than stencil_kernel, so fewer planes
+// keep their registers (and so the waves per SIMD) in line with it.
+constexpr int KZF = 4;
 
 // One point of out = alpha * A x + beta * y + gamma * z (the expression of
 // stencil_kernel, reading the neighbours directly).
@@ -170,7 +174,7 @@ momentum_rhs_kernel(op3<double> A, double dt, double dtnu, double inv_rho, const
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     const int i = blockIdx.x * STX + threadIdx.x;
     const int j = blockIdx.y * STY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * KZF;
     if (i >= nx || j >= ny || k0 >= nz) return;
     const size_t sxy = (size_t) nx * ny;
     const int im = A.a[0].periodic ? (i == 0 ? nx - 1 : i - 1) : (i > 0 ? i - 1 : i);
@@ -181,20 +185,20 @@ momentum_rhs_kernel(op3<double> A, double dt, double dtnu, double inv_rho, const
     const ptrdiff_t dxm = (ptrdiff_t) im - i, dxp = (ptrdiff_t) ip - i;
     const ptrdiff_t dym = ((ptrdiff_t) jm - j) * nx, dyp = ((ptrdiff_t) jp - j) * nx;
 
-    double c[KZ + 2], xl[KZ], xr[KZ], yl[KZ], yr[KZ], fv[KZ], hv[KZ], bv[KZ];
+    double c[KZF + 2], xl[KZF], xr[KZF], yl[KZF], yr[KZF], fv[KZF], hv[KZF], bv[KZF];
     {
         int km = k0 - 1;
         if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
         c[0] = u[col + sxy * (size_t) km];
     }
 #pragma unroll
-    for (int q = 0; q <= KZ; ++q) {
+    for (int q = 0; q <= KZF; ++q) {
         int k = k0 + q;
         if (k >= nz) k = A.a[2].periodic ? k % nz : nz - 1;
         c[q + 1] = u[col + sxy * (size_t) k];
     }
 #pragma unroll
-    for (int q = 0; q < KZ; ++q) {
+    for (int q = 0; q < KZF; ++q) {
         const int k = min(k0 + q, nz - 1);
         const size_t p = col + sxy * (size_t) k;
         xl[q] = u[p + dxm];
@@ -207,7 +211,7 @@ momentum_rhs_kernel(op3<double> A, double dt, double dtnu, double inv_rho, const
     }
     const double Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
 #pragma unroll
-    for (int q = 0; q < KZ; ++q) {
+    for (int q = 0; q < KZF; ++q) {
         const int k = k0 + q;
         if (k >= nz) break;
         const double x0 = c[q + 1];
@@ -259,7 +263,7 @@ residual_update_kernel(op3<double> A, const double* __restrict__ x, const double
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     const int i = blockIdx.x * STX + threadIdx.x;
     const int j = blockIdx.y * STY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * KZF;
     double rr = 0, bb = 0;
     if (i < nx && j < ny && k0 < nz) {
         const size_t sxy = (size_t) nx * ny;
@@ -270,20 +274,20 @@ residual_update_kernel(op3<double> A, const double* __restrict__ x, const double
         const size_t col = (size_t) i + (size_t) nx * j;
         const ptrdiff_t dxm = (ptrdiff_t) im - i, dxp = (ptrdiff_t) ip - i;
         const ptrdiff_t dym = ((ptrdiff_t) jm - j) * nx, dyp = ((ptrdiff_t) jp - j) * nx;
-        double c[KZ + 2], xl[KZ], xr[KZ], yl[KZ], yr[KZ], bv[KZ], uv[KZ];
+        double c[KZF + 2], xl[KZF], xr[KZF], yl[KZF], yr[KZF], bv[KZF], uv[KZF];
         {
             int km = k0 - 1;
             if (km < 0) km = A.a[2].periodic ? nz - 1 : 0;
             c[0] = x[col + sxy * (size_t) km];
         }
 #pragma unroll
-        for (int q = 0; q <= KZ; ++q) {
+        for (int q = 0; q <= KZF; ++q) {
             int k = k0 + q;
             if (k >= nz) k = A.a[2].periodic ? k % nz : nz - 1;
             c[q + 1] = x[col + sxy * (size_t) k];
         }
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
+        for (int q = 0; q < KZF; ++q) {
             const int k = min(k0 + q, nz - 1);
             const size_t p = col + sxy * (size_t) k;
             xl[q] = x[p + dxm];
@@ -295,7 +299,7 @@ residual_update_kernel(op3<double> A, const double* __restrict__ x, const double
         }
         const double Ix = id1(A.a[0], i), Iy = id1(A.a[1], j);
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
+        for (int q = 0; q < KZF; ++q) {
             const int k = k0 + q;
             if (k >= nz) break;
             const double x0 = c[q + 1];
@@ -2206,14 +2210,14 @@ ibm_status launch_sqrt_ratio(ibm_ctx* ctx, const double* num, const double* den,
     return IBM_OK;
 }
 
-static dim3 stencil_grid(int nx, int ny, int nz)
+static dim3 fused_grid(int nx, int ny, int nz)
 {
-    return dim3((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZ - 1) / KZ);
+    return dim3((nx + STX - 1) / STX, (ny + STY - 1) / STY, (nz + KZF - 1) / KZF);
 }
 
 size_t residual_parts(int nx, int ny, int nz)
 {
-    const dim3 g = stencil_grid(nx, ny, nz);
+    const dim3 g = fused_grid(nx, ny, nz);
     return 2 * (size_t) g.x * g.y * g.z;
 }
 
@@ -2223,7 +2227,7 @@ ibm_status launch_momentum_rhs(ibm_ctx* ctx, const op3<double>& A, double dt, do
     const int nx = A.a[0].n, ny = A.a[1].n, nz = A.a[2].n;
     if (nx < 1 || ny < 1 || nz < 1) return IBM_OK;
     prof_scope prof(ctx, "momentum_rhs", (b ? 40.0 : 32.0) * nx * ny * (double) nz);
-    const dim3 grid = stencil_grid(nx, ny, nz), block(STX, STY);
+    const dim3 grid = fused_grid(nx, ny, nz), block(STX, STY);
     if (b) momentum_rhs_kernel<true><<<grid, block, 0, ctx->stream>>>(A, dt, dtnu, inv_rho, F, H, u, b, rhs);
     else momentum_rhs_kernel<false><<<grid, block, 0, ctx->stream>>>(A, dt, dtnu, inv_rho, F, H, u, b, rhs);
     IBM_LAUNCH_CHECK(ctx);
@@ -2239,7 +2243,7 @@ ibm_status launch_residual_update(ibm_ctx* ctx, const op3<double>& A, const doub
     if (need > parts || need / 2 > (size_t) INT32_MAX)
         return fail(ctx, IBM_ERR_SIZE, "residual check: %zu partial sums, room for %zu", need, parts);
     prof_scope prof(ctx, "residual_update", (u ? 32.0 : 16.0) * nx * ny * (double) nz);
-    const dim3 grid = stencil_grid(nx, ny, nz), block(STX, STY);
+    const dim3 grid = fused_grid(nx, ny, nz), block(STX, STY);
     if (u) residual_update_kernel<true><<<grid, block, 0, ctx->stream>>>(A, x, b, u, partial);
     else residual_update_kernel<false><<<grid, block, 0, ctx->stream>>>(A, x, b, u, partial);
     IBM_LAUNCH_CHECK(ctx);
