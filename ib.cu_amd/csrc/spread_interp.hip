@@ -722,7 +722,6 @@ interp_kernel(marker_sets sets, kgrids g, int delta,
     int n = sets.offset[k + 1] - sets.offset[k];
     const double* x = sets.x[k];
     double p[3] = {x[l], x[n + l], x[2 * n + l]};
-    const double* ue[3] = {u0, u1, u2};
     for (int c = 0; c < 3; ++c) {
         double w[3][4];
         size_t ix[4], oy[4], oz[4];
@@ -737,7 +736,7 @@ interp_kernel(marker_sets sets, kgrids g, int delta,
                 else oz[o] = idx * (size_t) pt[0] * (size_t) pt[1];
             }
         }
-        const double* u = ue[c];
+        const double* u = c == 0 ? u0 : (c == 1 ? u1 : u2);    // global loads (an indexed array would be flat)
         // the four x points are adjacent except across a periodic seam or a
         // clamped wall: then one 32-byte load per row instead of four
         const bool adjacent = ix[1] == ix[0] + 1 && ix[2] == ix[0] + 2 && ix[3] == ix[0] + 3;
