@@ -83,8 +83,17 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ sr
 {
     long total = (long) rows_pad * Kp;
     for (long i = blockIdx.x * (long) blockDim.x + threadIdx.x; i < total; i += (long) gridDim.x * blockDim.x) {
-        int r = (int) (i / Kp);
-        int k = (int) (i - (long) r * Kp);
+        // 32-bit quotient where the index fits (a 64-bit division is a long
+        // instruction sequence)
+        int r, k;
+        if (total <= 0xffffffffl) {
+            const unsigned u = (unsigned) i;
+            r = (int) (u / (unsigned) Kp);
+            k = (int) (u - (unsigned) r * (unsigned) Kp);
+        } else {
+            r = (int) (i / Kp);
+            k = (int) (i - (long) r * Kp);
+        }
         double v = (r < rows && k < K) ? src[k + ld * r] : 0.0;
         if constexpr (MODE == MODE_F64) {
             ((double*) hi)[i] = v;
